@@ -1,0 +1,535 @@
+// C ABI (include/svo_amd.h) over the stage drivers, and the in-memory main.cpp flow.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <sstream>
+#include <string>
+
+#include "../../include/svo_amd.h"
+#include "svo_internal.h"
+
+struct svo_ctx {
+  svo::Context c;
+};
+
+namespace svo {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+hipError_t ensure(DevBuf& b, size_t bytes) {
+  if (bytes <= b.cap && b.p) return hipSuccess;
+  if (b.p) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return e;
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  size_t want = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    (void)hipGetLastError();
+    set_error("hipMalloc of " + std::to_string(want) + " bytes failed");
+    return hipErrorOutOfMemory;
+  }
+  b.cap = want;
+  return hipSuccess;
+}
+
+hipError_t ensure_keep(DevBuf& b, size_t bytes, size_t used, hipStream_t s) {
+  if (bytes <= b.cap && b.p) return hipSuccess;
+  size_t want = std::max<size_t>(bytes, 256);
+  void* np = nullptr;
+  hipError_t e = hipMalloc(&np, want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("hipMalloc of " + std::to_string(want) + " bytes failed");
+    return hipErrorOutOfMemory;
+  }
+  if (b.p) {
+    if (used) {
+      e = hipMemcpyAsync(np, b.p, std::min(used, b.cap), hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) return e;
+    }
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    (void)hipFree(b.p);
+  }
+  b.p = np;
+  b.cap = want;
+  return hipSuccess;
+}
+
+void release(DevBuf& b) {
+  if (b.p) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(b.p);
+  }
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+static hipEvent_t take_event(Context& c) {
+  if (!c.evpool.empty()) {
+    hipEvent_t e = c.evpool.back();
+    c.evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void ktimer_begin(Context& c, const char* name) {
+  if (!c.ktiming) return;
+  PendingTimer t;
+  t.name = name;
+  t.a = take_event(c);
+  t.b = take_event(c);
+  if (!t.a || !t.b) return;
+  (void)hipEventRecord(t.a, c.stream);
+  c.kpending.push_back(t);
+}
+
+void ktimer_end(Context& c) {
+  if (!c.ktiming || c.kpending.empty()) return;
+  (void)hipEventRecord(c.kpending.back().b, c.stream);
+}
+
+void ktimer_flush(Context& c) {
+  for (PendingTimer& t : c.kpending) {
+    float ms = 0;
+    if (hipEventSynchronize(t.b) == hipSuccess && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      KernelStat& k = c.kstats[t.name];
+      k.launches++;
+      k.ms += ms;
+    }
+    c.evpool.push_back(t.a);
+    c.evpool.push_back(t.b);
+  }
+  c.kpending.clear();
+}
+
+}  // namespace svo
+
+using namespace svo;
+
+static int fail(hipError_t e, const char* where) {
+  if (e == hipErrorOutOfMemory) {
+    if (g_err.empty()) g_err = std::string(where) + ": out of device memory";
+    return SVO_ERR_NOMEM;
+  }
+  if (e == hipErrorNotSupported) return SVO_ERR_UNSUPPORTED;
+  if (e == hipErrorInvalidValue) {
+    if (g_err.empty()) g_err = std::string(where) + ": invalid value";
+    return SVO_ERR_INVALID;
+  }
+  std::string m = std::string(where) + ": " + hipGetErrorString(e);
+  if (!g_err.empty()) m += " (" + g_err + ")";
+  g_err = m;
+  return SVO_ERR_DEVICE;
+}
+
+#define CHECK_CTX()                         \
+  do {                                      \
+    if (!ctx) {                             \
+      g_err = "null context";               \
+      return SVO_ERR_INVALID;               \
+    }                                       \
+    g_err.clear();                          \
+    hipError_t _d = hipSetDevice(ctx->c.device); \
+    if (_d != hipSuccess) return fail(_d, "hipSetDevice"); \
+  } while (0)
+
+#define TRYC(x, where)                          \
+  do {                                          \
+    hipError_t _e = (x);                        \
+    if (_e != hipSuccess) return fail(_e, where); \
+  } while (0)
+
+static int is_pow2_u64(uint64_t x) { return x != 0 && !(x & (x - 1)); }
+
+// The 6-significant-digit text round trip a bbox value takes through the .trip header
+// (trip_tools.h:112 writes it, main.cpp:306-307 re-parses it).
+static float text_roundtrip(float v) {
+  std::ostringstream o;
+  o << v;
+  std::istringstream i(o.str());
+  float r = 0.0f;
+  i >> r;
+  return r;
+}
+
+extern "C" {
+
+const char* svo_last_error(void) { return g_err.c_str(); }
+const char* svo_version(void) { return "ooc_svo_builder_amd 0.1 (svo_builder 1.6.4 formats)"; }
+
+int svo_ctx_create(int device, svo_ctx** out) {
+  g_err.clear();
+  if (!out) {
+    g_err = "null output pointer";
+    return SVO_ERR_INVALID;
+  }
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) {
+    g_err = "no HIP device available";
+    return SVO_ERR_DEVICE;
+  }
+  if (device < 0 || device >= ndev) {
+    g_err = "device index out of range";
+    return SVO_ERR_INVALID;
+  }
+  svo_ctx* ctx = new (std::nothrow) svo_ctx();
+  if (!ctx) {
+    g_err = "host allocation failed";
+    return SVO_ERR_NOMEM;
+  }
+  ctx->c.device = device;
+  if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreate(&ctx->c.stream)) != hipSuccess) {
+    delete ctx;
+    return fail(e, "svo_ctx_create");
+  }
+  ctx->c.own_stream = true;
+  if ((e = ensure(ctx->c.counters, kNumCounters * 8)) != hipSuccess ||
+      (e = hipEventCreate(&ctx->c.ev0)) != hipSuccess ||
+      (e = hipEventCreate(&ctx->c.ev1)) != hipSuccess) {
+    svo_ctx_destroy(ctx);
+    return fail(e, "svo_ctx_create");
+  }
+  *out = ctx;
+  return SVO_OK;
+}
+
+void svo_ctx_destroy(svo_ctx* ctx) {
+  if (!ctx) return;
+  Context& c = ctx->c;
+  (void)hipSetDevice(c.device);
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  DevBuf* bufs[] = {&c.tris, &c.hitbox, &c.part_lists, &c.table, &c.pool, &c.counters,
+                    &c.large_tri, &c.large_cols, &c.large_off, &c.occ_list, &c.bcount,
+                    &c.wprefix, &c.winner, &c.scan_scratch, &c.codes, &c.records, &c.nodes,
+                    &c.data, &c.lvl_code, &c.lvl_size, &c.lvl_mask, &c.lvl_first,
+                    &c.lvl_start, &c.lvl_aux};
+  for (DevBuf* b : bufs) release(*b);
+  ktimer_flush(c);
+  for (hipEvent_t e : c.evpool) (void)hipEventDestroy(e);
+  if (c.ev0) (void)hipEventDestroy(c.ev0);
+  if (c.ev1) (void)hipEventDestroy(c.ev1);
+  if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
+  delete ctx;
+}
+
+int svo_ctx_set_stream(svo_ctx* ctx, void* hip_stream) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  TRYC(hipStreamSynchronize(c.stream), "svo_ctx_set_stream");
+  if (hip_stream) {
+    if (c.own_stream) (void)hipStreamDestroy(c.stream);
+    c.own_stream = false;
+    c.stream = (hipStream_t)hip_stream;
+  } else if (!c.own_stream) {
+    TRYC(hipStreamCreate(&c.stream), "svo_ctx_set_stream");
+    c.own_stream = true;
+  }
+  return SVO_OK;
+}
+
+void* svo_ctx_stream(svo_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
+
+uint64_t svo_estimate_partitions(uint64_t gridsize, uint64_t memory_limit_mb) {
+  const uint64_t required = (gridsize * gridsize * gridsize) / 1024 / 1024;
+  if (required <= memory_limit_mb) return 1;
+  uint64_t np = 1, req = required;
+  while (req > memory_limit_mb) {
+    req /= 8;
+    np *= 8;
+  }
+  return np;
+}
+
+int svo_mesh_load(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payload,
+                  int src_on_device) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (n_triangles && !records) {
+    g_err = "null records";
+    return SVO_ERR_INVALID;
+  }
+  const size_t bytes = n_triangles * (payload ? 21 : 9) * sizeof(float);
+  TRYC(ensure(c.tris, bytes), "svo_mesh_load");
+  if (bytes)
+    TRYC(hipMemcpyAsync(c.tris.p, records, bytes,
+                        src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c.stream),
+         "svo_mesh_load");
+  TRYC(hipStreamSynchronize(c.stream), "svo_mesh_load");
+  c.n_tris = n_triangles;
+  c.payload = payload ? 1 : 0;
+  c.n_parts = 0;
+  c.n_vox = 0;
+  c.nfilled = 0;
+  c.n_nodes = c.n_data = 0;
+  return SVO_OK;
+}
+
+int svo_partition(svo_ctx* ctx, float unitlength, uint64_t gridsize, uint64_t n_partitions,
+                  uint64_t* counts_out) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (!is_pow2_u64(gridsize) || n_partitions == 0) {
+    g_err = "gridsize must be a power of 2 and n_partitions > 0";
+    return SVO_ERR_INVALID;
+  }
+  uint64_t p = n_partitions;
+  while (p % 8 == 0) p /= 8;
+  if (p != 1 || n_partitions > gridsize * gridsize * gridsize) {
+    g_err = "n_partitions must be a power of 8 not above gridsize^3";
+    return SVO_ERR_INVALID;
+  }
+  TRYC(partition_mesh(c, unitlength, gridsize, n_partitions), "svo_partition");
+  if (counts_out)
+    for (uint64_t j = 0; j < n_partitions; j++) counts_out[j] = c.part_cnt[j];
+  return SVO_OK;
+}
+
+int svo_partition_indices(svo_ctx* ctx, uint64_t partition, uint32_t* host_out) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (partition >= c.n_parts) {
+    g_err = "partition index out of range";
+    return SVO_ERR_INVALID;
+  }
+  if (c.n_parts == 1) {
+    for (uint64_t i = 0; i < c.n_tris; i++) host_out[i] = (uint32_t)i;
+    return SVO_OK;
+  }
+  if (c.part_cnt[partition])
+    TRYC(hipMemcpy(host_out, c.part_lists.as<uint32_t>() + c.part_off[partition],
+                   c.part_cnt[partition] * 4, hipMemcpyDeviceToHost),
+         "svo_partition_indices");
+  return SVO_OK;
+}
+
+int svo_voxelize(svo_ctx* ctx, uint64_t partition, uint64_t morton_start, uint64_t morton_end,
+                 float unitlength, uint64_t gridsize, int color_mode, uint64_t* nfilled_out) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (!is_pow2_u64(gridsize) || gridsize > (1u << 15) || morton_end <= morton_start ||
+      morton_end > gridsize * gridsize * gridsize || color_mode < 0 || color_mode > 3) {
+    g_err = "invalid voxelize arguments";
+    return SVO_ERR_INVALID;
+  }
+  if (c.n_parts > 1 && partition >= c.n_parts) {
+    g_err = "partition index out of range";
+    return SVO_ERR_INVALID;
+  }
+  if (c.n_parts <= 1) partition = 0;
+  uint64_t nf = 0;
+  TRYC(voxelize_partition(c, partition, morton_start, morton_end, unitlength, gridsize,
+                          color_mode, &nf),
+       "svo_voxelize");
+  c.nfilled += nf;
+  if (nfilled_out) *nfilled_out = nf;
+  return SVO_OK;
+}
+
+int svo_add_voxels(svo_ctx* ctx, const uint64_t* codes, const void* records32, uint64_t n) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (n == 0) return SVO_OK;
+  if (!codes || (c.payload && !records32)) {
+    g_err = "null codes or records";
+    return SVO_ERR_INVALID;
+  }
+  uint64_t last = 0;
+  if (c.n_vox)
+    TRYC(hipMemcpy(&last, c.codes.as<uint64_t>() + c.n_vox - 1, 8, hipMemcpyDeviceToHost),
+         "svo_add_voxels");
+  for (uint64_t i = 0; i < n; i++) {
+    if ((c.n_vox || i) && codes[i] <= last) {
+      g_err = "voxel codes must be strictly increasing";
+      return SVO_ERR_INVALID;
+    }
+    last = codes[i];
+  }
+  TRYC(ensure_keep(c.codes, (c.n_vox + n) * 8, c.n_vox * 8, c.stream), "svo_add_voxels");
+  TRYC(hipMemcpyAsync(c.codes.as<uint64_t>() + c.n_vox, codes, n * 8, hipMemcpyHostToDevice,
+                      c.stream),
+       "svo_add_voxels");
+  if (c.payload) {
+    TRYC(ensure_keep(c.records, (1 + c.n_vox + n) * 32, (1 + c.n_vox) * 32, c.stream),
+         "svo_add_voxels");
+    TRYC(hipMemcpyAsync((char*)c.records.p + (1 + c.n_vox) * 32, records32, n * 32,
+                        hipMemcpyHostToDevice, c.stream),
+         "svo_add_voxels");
+  }
+  TRYC(hipStreamSynchronize(c.stream), "svo_add_voxels");
+  c.n_vox += n;
+  return SVO_OK;
+}
+
+int svo_build(svo_ctx* ctx, uint64_t gridsize, int generate_levels, uint64_t* n_nodes_out,
+              uint64_t* n_data_out, int* hazard_out) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (!is_pow2_u64(gridsize)) {
+    g_err = "gridsize must be a power of 2";
+    return SVO_ERR_INVALID;
+  }
+  hipError_t e = build_octree(c, gridsize, generate_levels);
+  if (e != hipSuccess) return fail(e, "svo_build");
+  if (n_nodes_out) *n_nodes_out = c.n_nodes;
+  if (n_data_out) *n_data_out = c.n_data;
+  if (hazard_out) *hazard_out = c.hazard;
+  return SVO_OK;
+}
+
+int svo_copy_nodes(svo_ctx* ctx, void* host_dst, uint64_t first, uint64_t count) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (first + count > c.n_nodes) {
+    g_err = "node range out of bounds";
+    return SVO_ERR_INVALID;
+  }
+  if (count)
+    TRYC(hipMemcpy(host_dst, (const char*)c.nodes.p + first * 24, count * 24,
+                   hipMemcpyDeviceToHost),
+         "svo_copy_nodes");
+  return SVO_OK;
+}
+
+int svo_copy_data(svo_ctx* ctx, void* host_dst, uint64_t first, uint64_t count) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (first + count > c.n_data) {
+    g_err = "data range out of bounds";
+    return SVO_ERR_INVALID;
+  }
+  const void* src = c.payload ? c.records.p : c.data.p;
+  if (count)
+    TRYC(hipMemcpy(host_dst, (const char*)src + first * 32, count * 32, hipMemcpyDeviceToHost),
+         "svo_copy_data");
+  return SVO_OK;
+}
+
+const void* svo_nodes_device(svo_ctx* ctx) { return ctx ? ctx->c.nodes.p : nullptr; }
+const void* svo_data_device(svo_ctx* ctx) {
+  return ctx ? (ctx->c.payload ? ctx->c.records.p : ctx->c.data.p) : nullptr;
+}
+
+uint64_t svo_voxel_count(svo_ctx* ctx) { return ctx ? ctx->c.n_vox : 0; }
+
+int svo_copy_codes(svo_ctx* ctx, uint64_t* host_dst, uint64_t first, uint64_t count) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (first + count > c.n_vox) {
+    g_err = "code range out of bounds";
+    return SVO_ERR_INVALID;
+  }
+  if (count)
+    TRYC(hipMemcpy(host_dst, c.codes.as<uint64_t>() + first, count * 8, hipMemcpyDeviceToHost),
+         "svo_copy_codes");
+  return SVO_OK;
+}
+
+int svo_ctx_set_kernel_timing(svo_ctx* ctx, int on) {
+  CHECK_CTX();
+  TRYC(hipStreamSynchronize(ctx->c.stream), "svo_ctx_set_kernel_timing");
+  ktimer_flush(ctx->c);
+  ctx->c.ktiming = on != 0;
+  return SVO_OK;
+}
+
+int svo_kernel_timing(svo_ctx* ctx, const char* name, uint64_t* launches, double* total_ms) {
+  CHECK_CTX();
+  TRYC(hipStreamSynchronize(ctx->c.stream), "svo_kernel_timing");
+  ktimer_flush(ctx->c);
+  auto it = ctx->c.kstats.find(name ? name : "");
+  if (launches) *launches = it == ctx->c.kstats.end() ? 0 : it->second.launches;
+  if (total_ms) *total_ms = it == ctx->c.kstats.end() ? 0.0 : it->second.ms;
+  return SVO_OK;
+}
+
+int svo_kernel_timing_reset(svo_ctx* ctx) {
+  CHECK_CTX();
+  TRYC(hipStreamSynchronize(ctx->c.stream), "svo_kernel_timing_reset");
+  ktimer_flush(ctx->c);
+  ctx->c.kstats.clear();
+  return SVO_OK;
+}
+
+int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payload,
+            int src_on_device, const float bbox[6], uint64_t gridsize, uint64_t memory_limit_mb,
+            int color_mode, int generate_levels, svo_run_info* info) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (!bbox || !is_pow2_u64(gridsize) || memory_limit_mb <= 1) {
+    g_err = "invalid run arguments";
+    return SVO_ERR_INVALID;
+  }
+  int r = SVO_OK;
+  if (records) {
+    r = svo_mesh_load(ctx, records, n_triangles, payload, src_on_device);
+    if (r) return r;
+  } else {  // the mesh already loaded: reset the stream and outputs only
+    if ((payload ? 1 : 0) != c.payload) {
+      g_err = "payload flag differs from the loaded mesh";
+      return SVO_ERR_INVALID;
+    }
+    c.n_parts = 0;
+    c.n_vox = 0;
+    c.nfilled = 0;
+    c.n_nodes = c.n_data = 0;
+  }
+  const uint64_t np = svo_estimate_partitions(gridsize, memory_limit_mb);
+  const float u_part = (bbox[3] - bbox[0]) / (float)gridsize;  // partitioner.cpp:45
+  const float u_vox =
+      (text_roundtrip(bbox[3]) - text_roundtrip(bbox[0])) / (float)gridsize;  // main.cpp:311
+  float ms_p = 0, ms_v = 0, ms_b = 0;
+  TRYC(hipEventRecord(c.ev0, c.stream), "svo_run");
+  std::vector<uint64_t> counts(np, 0);
+  r = svo_partition(ctx, u_part, gridsize, np, counts.data());
+  if (r) return r;
+  TRYC(hipEventRecord(c.ev1, c.stream), "svo_run");
+  TRYC(hipEventSynchronize(c.ev1), "svo_run");
+  TRYC(hipEventElapsedTime(&ms_p, c.ev0, c.ev1), "svo_run");
+  const uint64_t morton_part = (gridsize * gridsize * gridsize) / np;
+  TRYC(hipEventRecord(c.ev0, c.stream), "svo_run");
+  for (uint64_t i = 0; i < np; i++) {
+    if (counts[i] == 0) continue;  // main.cpp:330
+    r = svo_voxelize(ctx, i, i * morton_part, (i + 1) * morton_part, u_vox, gridsize, color_mode,
+                     nullptr);
+    if (r) return r;
+  }
+  TRYC(hipEventRecord(c.ev1, c.stream), "svo_run");
+  TRYC(hipEventSynchronize(c.ev1), "svo_run");
+  TRYC(hipEventElapsedTime(&ms_v, c.ev0, c.ev1), "svo_run");
+  TRYC(hipEventRecord(c.ev0, c.stream), "svo_run");
+  uint64_t nn = 0, nd = 0;
+  int hz = 0;
+  r = svo_build(ctx, gridsize, generate_levels, &nn, &nd, &hz);
+  if (r) return r;
+  TRYC(hipEventRecord(c.ev1, c.stream), "svo_run");
+  TRYC(hipEventSynchronize(c.ev1), "svo_run");
+  TRYC(hipEventElapsedTime(&ms_b, c.ev0, c.ev1), "svo_run");
+  if (info) {
+    info->n_partitions = np;
+    info->nfilled = c.nfilled;
+    info->n_nodes = nn;
+    info->n_data = nd;
+    info->hazard = hz;
+    info->ms_partition = ms_p;
+    info->ms_voxelize = ms_v;
+    info->ms_build = ms_b;
+  }
+  return SVO_OK;
+}
+
+}  // extern "C"

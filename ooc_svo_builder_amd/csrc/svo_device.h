@@ -1,0 +1,130 @@
+// Device-side building blocks shared by the voxelize, partition and build kernels.
+//
+// Everything here is integer/bit work or FP32 arithmetic in the exact expression order
+// of voxelizer.cpp:189-303 (compiled with -ffp-contract=off, IEEE division/sqrt and
+// denormals preserved, so each operation rounds exactly as the reference's x86 SSE
+// build does).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace svo {
+
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------------------------------
+// Morton codes (interleave x -> bit 0, y -> bit 1, z -> bit 2 of each triple), the
+// closed form of libmorton's morton3D_64_encode/decode for coordinates below 2^21.
+// ------------------------------------------------------------------------------------
+__host__ __device__ __forceinline__ uint64_t spread3(uint64_t v) {
+  v &= 0x1fffffull;
+  v = (v | (v << 32)) & 0x1f00000000ffffull;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+__host__ __device__ __forceinline__ uint32_t compact3(uint64_t v) {
+  v &= 0x1249249249249249ull;
+  v = (v ^ (v >> 2)) & 0x10c30c30c30c30c3ull;
+  v = (v ^ (v >> 4)) & 0x100f00f00f00f00full;
+  v = (v ^ (v >> 8)) & 0x1f0000ff0000ffull;
+  v = (v ^ (v >> 16)) & 0x1f00000000ffffull;
+  v = (v ^ (v >> 32)) & 0x1fffffull;
+  return (uint32_t)v;
+}
+__host__ __device__ __forceinline__ uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+}
+__host__ __device__ __forceinline__ void unmorton3(uint64_t m, uint32_t& x, uint32_t& y,
+                                                   uint32_t& z) {
+  x = compact3(m);
+  y = compact3(m >> 1);
+  z = compact3(m >> 2);
+}
+
+// ------------------------------------------------------------------------------------
+// Scalar semantics of the reference's x86 build
+// ------------------------------------------------------------------------------------
+// static_cast<int>(float) as cvttss2si: NaN / out of range -> INT_MIN
+__device__ __forceinline__ int trunc_i32_x86(float f) {
+  if (!(f >= -2147483648.0f && f < 2147483648.0f)) return (int)0x80000000;
+  return (int)f;
+}
+__device__ __forceinline__ float min_std(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float max_std(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ int clamp_i(int v, int lo, int hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+// NaN written to .octreedata is stored as the x86 default NaN 0xFFC00000
+__device__ __forceinline__ float canon_nan(float f) {
+  return (f != f) ? __uint_as_float(0xFFC00000u) : f;
+}
+
+// ------------------------------------------------------------------------------------
+// Wave-level helpers (64 lanes)
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+template <typename T>
+__device__ __forceinline__ T shfl_up_t(T v, int d);
+template <>
+__device__ __forceinline__ uint32_t shfl_up_t<uint32_t>(uint32_t v, int d) {
+  return (uint32_t)__shfl_up((int)v, d, kWave);
+}
+template <>
+__device__ __forceinline__ uint64_t shfl_up_t<uint64_t>(uint64_t v, int d) {
+  int lo = __shfl_up((int)(uint32_t)v, d, kWave);
+  int hi = __shfl_up((int)(uint32_t)(v >> 32), d, kWave);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+template <typename T>
+__device__ __forceinline__ T shfl_t(T v, int src);
+template <>
+__device__ __forceinline__ uint32_t shfl_t<uint32_t>(uint32_t v, int s) {
+  return (uint32_t)__shfl((int)v, s, kWave);
+}
+template <>
+__device__ __forceinline__ uint64_t shfl_t<uint64_t>(uint64_t v, int s) {
+  int lo = __shfl((int)(uint32_t)v, s, kWave);
+  int hi = __shfl((int)(uint32_t)(v >> 32), s, kWave);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// inclusive scan over the 64 lanes of a wave (all lanes must be active)
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    T o = shfl_up_t<T>(v, d);
+    if (l >= d) v += o;
+  }
+  return v;
+}
+
+// inclusive scan over a block of BLOCK threads; 'total' receives the block sum.
+// smem must hold BLOCK/64 elements. All threads of the block must call it.
+template <typename T, int BLOCK>
+__device__ __forceinline__ T block_incl_scan(T v, T* smem, T& total) {
+  constexpr int NW = BLOCK / kWave;
+  const int w = threadIdx.x / kWave;
+  const int l = threadIdx.x % kWave;
+  T inc = wave_incl_scan<T>(v);
+  if (l == kWave - 1) smem[w] = inc;
+  __syncthreads();
+  T before = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < NW; i++) {
+    T s = smem[i];
+    if (i < w) before += s;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return before + inc;
+}
+
+}  // namespace svo

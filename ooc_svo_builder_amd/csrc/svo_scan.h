@@ -1,0 +1,92 @@
+// Device-wide exclusive scan (reduce-then-scan), parameterised by load/store functors so
+// that producers and consumers fuse into the two passes: a Load functor yields element
+// i's value, a Store functor receives (i, exclusive prefix, value). Tiles are 256
+// threads x 8 items in striped order, so every load of a wave is one contiguous span.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "svo_device.h"
+
+namespace svo {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr uint64_t kScanTile = (uint64_t)kScanBlock * kScanItems;
+
+template <typename T, typename Load>
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(Load ld, uint64_t n, T* sums) {
+  __shared__ T sm[kScanBlock / kWave];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  T acc = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    uint64_t i = base + (uint64_t)k * kScanBlock + threadIdx.x;
+    if (i < n) acc += ld(i);
+  }
+  T tot;
+  block_incl_scan<T, kScanBlock>(acc, sm, tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+template <typename T, typename Load, typename Store>
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(Load ld, Store st, uint64_t n,
+                                                           const T* prefix) {
+  __shared__ T sm[kScanBlock / kWave];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  T carry = prefix ? prefix[blockIdx.x] : (T)0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    uint64_t i = base + (uint64_t)k * kScanBlock + threadIdx.x;
+    T v = (i < n) ? ld(i) : (T)0;
+    T tot;
+    T inc = block_incl_scan<T, kScanBlock>(v, sm, tot);
+    if (i < n) st(i, carry + inc - v, v);
+    carry += tot;
+  }
+}
+
+template <typename T>
+struct ArrayLoad {
+  const T* a;
+  __device__ T operator()(uint64_t i) const { return a[i]; }
+};
+template <typename T>
+struct ArrayStoreExcl {
+  T* a;
+  __device__ void operator()(uint64_t i, T ex, T) const { a[i] = ex; }
+};
+
+// Number of T elements of scratch that scan_exclusive needs for n elements.
+inline uint64_t scan_scratch_elems(uint64_t n) {
+  uint64_t tot = 0;
+  while (n > kScanTile) {
+    n = (n + kScanTile - 1) / kScanTile;
+    tot += n;
+  }
+  return tot + 1;
+}
+
+// Exclusive scan of n elements. 'scratch' holds scan_scratch_elems(n) T's.
+template <typename T, typename Load, typename Store>
+hipError_t scan_exclusive(Load ld, Store st, uint64_t n, T* scratch, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint64_t nb = (n + kScanTile - 1) / kScanTile;
+  if (nb == 1) {
+    hipLaunchKernelGGL((k_scan_apply<T, Load, Store>), dim3(1), dim3(kScanBlock), 0, s, ld, st,
+                       n, (const T*)nullptr);
+    return hipGetLastError();
+  }
+  T* sums = scratch;
+  hipLaunchKernelGGL((k_scan_reduce<T, Load>), dim3((unsigned)nb), dim3(kScanBlock), 0, s, ld,
+                     n, sums);
+  hipError_t e = scan_exclusive<T>(ArrayLoad<T>{sums}, ArrayStoreExcl<T>{sums}, nb,
+                                   scratch + nb, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_scan_apply<T, Load, Store>), dim3((unsigned)nb), dim3(kScanBlock), 0, s,
+                     ld, st, n, (const T*)sums);
+  return hipGetLastError();
+}
+
+}  // namespace svo

@@ -1,0 +1,119 @@
+"""File formats (tri_tools.h:118-127, octree_io.h:49-83, Node.h:16-18, VoxelData.h:10-18),
+the C ABI's exports, and the CLI's argument handling (main.cpp:100-196), all on the CPU.
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from ooc_svo_builder_amd import (CLI_BINARY, CLI_PAYLOAD, DATA_DTYPE, NODE_DTYPE, SIGNATURES,
+                                 load_library, read_tri, write_tri)
+from ooc_svo_builder_amd.formats import octree_header_text
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_record_layouts():
+    assert NODE_DTYPE.itemsize == 24
+    assert NODE_DTYPE.fields["data"][1] == 0
+    assert NODE_DTYPE.fields["children_base"][1] == 8
+    assert NODE_DTYPE.fields["offsets"][1] == 16
+    assert DATA_DTYPE.itemsize == 32
+    assert DATA_DTYPE.fields["morton"][1] == 0
+    assert DATA_DTYPE.fields["color"][1] == 8
+    assert DATA_DTYPE.fields["normal"][1] == 20
+
+
+def test_tri_roundtrip(tmp_path):
+    rng = np.random.default_rng(0)
+    for payload in (False, True):
+        rec = rng.uniform(0, 1, (17, 21 if payload else 9)).astype(np.float32)
+        path = write_tri(str(tmp_path / ("m%d" % payload)), rec, payload)
+        text = open(path).read()
+        assert text == "#tri 1\nntriangles 17\ngeo_only %d\nbbox  0 0 0 1 1 1\nEND\n" % (
+            0 if payload else 1)
+        info, back = read_tri(path)
+        assert info.n_triangles == 17 and info.geo_only == (0 if payload else 1)
+        assert back.tobytes() == rec.tobytes()
+
+
+def test_octree_header_text():
+    assert octree_header_text(1024, 1474721, 484322) == (
+        "#octreeheader 1\ngridlength 1024\nn_nodes 1474721\nn_data 484322\nEND\n")
+
+
+def _declared_functions():
+    text = open(os.path.join(ROOT, "include", "svo_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(svo_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = load_library()
+    names = _declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in SIGNATURES, n
+    assert set(SIGNATURES) == set(names)
+    assert b"1.6.4" in L.svo_version()
+
+
+def test_context_create_fails_cleanly_without_device():
+    """With no GPU visible the library reports an error instead of faulting."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    L = load_library()
+    h = ctypes.c_void_p()
+    rc = L.svo_ctx_create(0, ctypes.byref(h))
+    assert rc != 0 and not h.value
+    assert L.svo_last_error()
+
+
+@pytest.mark.parametrize("exe_pair", [(CLI_BINARY, oracle.CLI_BINARY),
+                                      (CLI_PAYLOAD, oracle.CLI_PAYLOAD)])
+def test_cli_argument_errors_match_oracle(tmp_path, exe_pair):
+    """Bad arguments print the reference's messages and exit with status 0, before any
+    device work, for both the drop-in and the oracle CLI."""
+    m = write_tri(str(tmp_path / "m"), np.zeros((1, 9), np.float32), False)
+    p = write_tri(str(tmp_path / "p"), np.zeros((1, 21), np.float32), True)
+    cases = [[], ["-f"], ["-f", "x.obj"], ["-f", m, "-s", "100"], ["-f", m, "-l", "1"],
+             ["-f", m, "-bogus"], ["-h", "x"]]
+    geo_case = ["-f", p] if exe_pair[0] == CLI_BINARY else ["-f", m]
+    cases.append(geo_case)
+    cases.append(["-f", str(tmp_path / "missing.tri")])
+    for args in cases:
+        outs = []
+        for exe in exe_pair:
+            r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=60)
+            assert r.returncode == 0, (exe, args, r.stderr)
+            outs.append(r.stdout)
+        tail_gpu = outs[0].split("Reading program parameters ...")[-1]
+        tail_ref = outs[1].split("Reading program parameters ...")[-1]
+        assert tail_gpu == tail_ref, args
+        assert not any(os.path.exists(str(tmp_path / (b + ".octree"))) for b in ("m", "p"))
+
+
+def test_oracle_cli_matches_in_memory_pipeline(tmp_path):
+    """The oracle's file-based CLI (trip header round trip, partition files) and its
+    in-memory pipeline agree byte for byte, for 1 and 8 partitions."""
+    from ooc_svo_builder_amd import generators
+    tris = generators.icosphere(subdivisions=3, radius=0.35)
+    for G, L in ((64, 2048), (256, 2)):
+        tri = write_tri(str(tmp_path / "ico"), tris, False)
+        r = subprocess.run([oracle.CLI_BINARY, "-f", tri, "-s", str(G), "-l", str(L)],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout
+        P = oracle.estimate_partitions(G, L)
+        base = str(tmp_path / ("ico%d_%d" % (G, P)))
+        nodes = np.fromfile(base + ".octreenodes", NODE_DTYPE)
+        data = np.fromfile(base + ".octreedata", DATA_DTYPE)
+        n2, d2, hz, np_, nf = oracle.pipeline(tris, False, G, L)
+        assert nodes.tobytes() == n2.tobytes() and data.tobytes() == d2.tobytes()
+        assert open(base + ".octree").read() == octree_header_text(G, len(n2), len(d2))
+        assert not os.path.exists(base + ".trip")

@@ -1,0 +1,250 @@
+"""GPU parity: libsvo_amd.so (through its C ABI) against the CPU oracle, bit-exact.
+
+Every comparison is on bytes: sorted voxel codes, 24-byte node records and 32-byte data
+records, and whole .octree files for the drop-in CLI.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from ooc_svo_builder_amd import CLI_BINARY, CLI_PAYLOAD, SvoContext, generators, write_tri
+from ooc_svo_builder_amd.formats import files_identical
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def gpu_codes(ctx, tris, payload, G, unit, mstart=0, mend=None):
+    mend = G ** 3 if mend is None else mend
+    ctx.load_mesh(tris, payload)
+    ctx.voxelize(0, mstart, mend, unit, G)
+    return ctx.codes()
+
+
+def oracle_codes(tris, payload, G, unit, mstart=0, mend=None):
+    mend = G ** 3 if mend is None else mend
+    codes, recs = oracle.voxelize(tris, payload, mstart, mend, unit)
+    order = np.argsort(codes, kind="stable")
+    return codes[order], (None if recs is None else recs[order])
+
+
+def adversarial_soup(rng, n, G, big=True):
+    """Random triangles of every size class plus degenerate and boundary cases, in
+    world coordinates of a [0,1]^3 grid with G cells (big=False keeps every triangle
+    within a few voxels, since the oracle's full-bbox walk is slow on large ones)."""
+    u = 1.0 / G
+    parts = []
+    # small: within a few voxels
+    c = rng.uniform(0, 1, (n, 1, 3))
+    parts.append(c + rng.normal(0, 2 * u, (n, 3, 3)))
+    # vertices exactly on voxel corners, near each other
+    k = rng.integers(0, G + 1, (n // 4, 1, 3)) + rng.integers(-3, 4, (n // 4, 3, 3))
+    parts.append(k.astype(np.float64) * u)
+    # axis-aligned: z on a voxel face
+    a = rng.uniform(0, 1, (n // 8, 1, 3)) + rng.normal(0, 3 * u, (n // 8, 3, 3))
+    a[:, :, 2] = rng.integers(0, G, (n // 8, 1)) * u
+    parts.append(a)
+    # degenerate: repeated vertex, collinear, a single point
+    d = rng.uniform(0, 1, (n // 8, 1, 3)) + rng.normal(0, 2 * u, (n // 8, 3, 3))
+    d[:, 1] = d[:, 0]
+    parts.append(d)
+    e = rng.uniform(0, 1, (n // 8, 1, 3))
+    f = rng.uniform(-3 * u, 3 * u, (n // 8, 1, 3))
+    parts.append(np.concatenate([e, e + f, e + 2 * f], axis=1))
+    parts.append(np.repeat(rng.uniform(0, 1, (4, 1, 3)), 3, axis=1))
+    if big:
+        # medium / large
+        c = rng.uniform(0, 1, (n // 4, 1, 3))
+        parts.append(c + rng.normal(0, 0.15, (n // 4, 3, 3)))
+        # grid-spanning, and spanning with corner-exact vertices
+        parts.append(rng.uniform(-0.2, 1.2, (8, 3, 3)))
+        parts.append(rng.integers(0, G + 1, (8, 3, 3)).astype(np.float64) * u)
+    tri = np.concatenate(parts).astype(np.float32).reshape(-1, 9)
+    return tri
+
+
+@pytest.mark.parametrize("G", [64, 256])
+def test_icosphere_codes(gpu_ctx, G):
+    _say("icosphere codes G=%d" % G)
+    tris = generators.icosphere()
+    got = gpu_codes(gpu_ctx, tris, False, G, 1.0 / G)
+    want, _ = oracle_codes(tris, False, G, 1.0 / G)
+    assert got.dtype == want.dtype and np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("seed,G", [(11, 32), (12, 64), (13, 128)])
+def test_adversarial_codes(gpu_ctx, seed, G):
+    _say("adversarial codes seed=%d G=%d" % (seed, G))
+    tris = adversarial_soup(np.random.default_rng(seed), 800, G)
+    got = gpu_codes(gpu_ctx, tris, False, G, np.float32(1.0) / np.float32(G))
+    want, _ = oracle_codes(tris, False, G, 1.0 / G)
+    assert np.array_equal(got, want)
+
+
+def test_partition_window_codes(gpu_ctx):
+    """One Morton-range partition: the clamp to the partition box (voxelizer.cpp:198-204)."""
+    _say("partition window codes")
+    G = 64
+    tris = adversarial_soup(np.random.default_rng(5), 600, G)
+    part = G ** 3 // 8
+    for i in (0, 3, 7):
+        got = gpu_codes(gpu_ctx, tris, False, G, 1.0 / G, i * part, (i + 1) * part)
+        want, _ = oracle_codes(tris, False, G, 1.0 / G, i * part, (i + 1) * part)
+        assert np.array_equal(got, want), i
+
+
+@pytest.mark.parametrize("G", [64, 128])
+def test_payload_records(gpu_ctx, G):
+    _say("payload records G=%d" % G)
+    tris = generators.terrain(n=65, seed=2, payload=True)
+    want_codes, want_recs = oracle_codes(tris, True, G, 1.0 / G)
+    gpu_ctx.load_mesh(tris, True)
+    gpu_ctx.voxelize(0, 0, G ** 3, 1.0 / G, G)
+    nn, nd, hz = gpu_ctx.build(G)
+    data = gpu_ctx.data(nd)
+    assert nd == 1 + len(want_codes)
+    assert data[0].tobytes() == bytes(32)
+    assert data[1:].tobytes() == want_recs.tobytes()
+
+
+def test_payload_first_triangle_wins(gpu_ctx):
+    """Overlapping payload triangles: the voxel keeps the first triangle's data
+    (voxelizer.cpp:263), decided per voxel."""
+    _say("payload first triangle wins")
+    G = 32
+    rng = np.random.default_rng(9)
+    base = rng.uniform(0.2, 0.8, (200, 1, 3))
+    v = base + rng.normal(0, 0.05, (200, 3, 3))
+    rec = np.zeros((400, 21), np.float32)
+    rec[:200, :9] = v.reshape(200, 9)
+    rec[200:, :9] = v.reshape(200, 9)[::-1]  # the same triangles again, reversed
+    rec[:, 9:12] = rng.normal(0, 1, (400, 3))
+    rec[:, 12:21] = rng.uniform(0, 1, (400, 9))
+    want_codes, want_recs = oracle_codes(rec, True, G, 1.0 / G)
+    gpu_ctx.load_mesh(rec, True)
+    gpu_ctx.voxelize(0, 0, G ** 3, 1.0 / G, G)
+    nn, nd, hz = gpu_ctx.build(G)
+    assert gpu_ctx.data(nd)[1:].tobytes() == want_recs.tobytes()
+
+
+def _random_codes(rng, G, n):
+    n = min(n, G ** 3)
+    return np.unique(rng.choice(G ** 3, size=n, replace=False).astype(np.uint64))
+
+
+@pytest.mark.parametrize("G", [1, 2, 4, 16, 64])
+def test_builder_binary(gpu_ctx, G):
+    _say("builder binary G=%d" % G)
+    rng = np.random.default_rng(G)
+    for n in (0, 1, 7, 300, 5000):
+        codes = _random_codes(rng, G, n)
+        if G > 1 and len(codes) and codes[-1] == G ** 3 - 2:
+            codes = codes[:-1]
+        nodes_w, data_w, hz_w = oracle.build_octree(codes, G, False)
+        gpu_ctx.load_mesh(np.zeros((0, 9), np.float32), False)
+        gpu_ctx.add_voxels(codes)
+        nn, nd, hz = gpu_ctx.build(G)
+        assert hz == hz_w
+        assert nn == len(nodes_w) and nd == len(data_w)
+        assert gpu_ctx.nodes(nn).tobytes() == nodes_w.tobytes()
+        assert gpu_ctx.data(nd).tobytes() == data_w.tobytes()
+
+
+def test_builder_payload_and_hazard(gpu_ctx):
+    _say("builder payload + hazard")
+    from ooc_svo_builder_amd.formats import DATA_DTYPE
+    rng = np.random.default_rng(3)
+    G = 16
+    for codes in (_random_codes(rng, G, 900),
+                  np.array([5, 100, G ** 3 - 2], np.uint64)):  # hazard: last = max - 1
+        recs = np.zeros(len(codes), DATA_DTYPE)
+        recs["morton"] = codes
+        recs["color"] = rng.uniform(0, 1, (len(codes), 3))
+        recs["normal"] = rng.normal(0, 1, (len(codes), 3))
+        nodes_w, data_w, hz_w = oracle.build_octree(recs, G, True)
+        gpu_ctx.load_mesh(np.zeros((0, 21), np.float32), True)
+        gpu_ctx.add_voxels(codes, recs)
+        nn, nd, hz = gpu_ctx.build(G)
+        assert hz == hz_w
+        assert gpu_ctx.nodes(nn).tobytes() == nodes_w.tobytes()
+        assert gpu_ctx.data(nd).tobytes() == data_w.tobytes()
+
+
+def _run_both(ctx, tris, payload, G, L, color_mode=0):
+    info = ctx.run(tris, payload, G, L, color_mode=color_mode)
+    nodes_w, data_w, hz_w, np_w, nf_w = oracle.pipeline(tris, payload, G, L, color_mode=color_mode)
+    assert info.n_partitions == np_w
+    assert info.nfilled == nf_w
+    assert info.hazard == hz_w
+    assert info.n_nodes == len(nodes_w) and info.n_data == len(data_w)
+    assert ctx.nodes(info.n_nodes).tobytes() == nodes_w.tobytes()
+    assert ctx.data(info.n_data).tobytes() == data_w.tobytes()
+    return info
+
+
+def test_pipeline_c0_256(gpu_ctx):
+    """The icosphere workload (20,480 triangles, -s 256 -l 2048, one partition)."""
+    _say("pipeline C0 256")
+    _run_both(gpu_ctx, generators.icosphere(), False, 256, 2048)
+
+
+@pytest.mark.parametrize("G,L,parts", [(256, 2, 8), (512, 2, 64)])
+def test_pipeline_partitions(gpu_ctx, G, L, parts):
+    """Out-of-core path: -s 256 -l 2 (8 partitions) and -s 512 -l 2 (64 partitions)."""
+    _say("pipeline partitions G=%d L=%d" % (G, L))
+    rng = np.random.default_rng(77)
+    tris = np.concatenate([generators.icosphere(subdivisions=3, radius=0.3),
+                           adversarial_soup(rng, 400, G, big=(G <= 256))])
+    info = _run_both(gpu_ctx, tris, False, G, L)
+    assert info.n_partitions == parts
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_pipeline_payload_color_modes(gpu_ctx, mode):
+    _say("pipeline payload color mode %d" % mode)
+    tris = generators.terrain(n=97, seed=4, payload=True)
+    _run_both(gpu_ctx, tris, True, 128, 2048, color_mode=mode)
+
+
+def test_pipeline_payload_partitions(gpu_ctx):
+    _say("pipeline payload partitions")
+    tris = generators.terrain(n=65, seed=6, payload=True)
+    info = _run_both(gpu_ctx, tris, True, 256, 2)
+    assert info.n_partitions == 8
+
+
+def test_empty_and_outside(gpu_ctx):
+    _say("empty and outside meshes")
+    _run_both(gpu_ctx, np.zeros((0, 9), np.float32), False, 64, 2048)
+    far = np.full((5, 9), 7.5, np.float32)  # clamps onto the far corner voxel column
+    _run_both(gpu_ctx, far, False, 64, 2048)
+
+
+def test_cli_files_identical(tmp_path):
+    """Drop-in CLI end to end: GPU svo_builder(_binary) and the oracle CLI write
+    byte-identical .octree/.octreenodes/.octreedata."""
+    _say("cli files identical")
+    cases = [("ico", generators.icosphere(subdivisions=4), False, ["-s", "128", "-l", "2048"]),
+             ("ter", generators.terrain(n=49, seed=3, payload=True), True, ["-s", "64", "-c", "linear"]),
+             ("part", generators.icosphere(subdivisions=3), False, ["-s", "256", "-l", "2"])]
+    for name, tris, payload, args in cases:
+        for who in ("gpu", "ref"):
+            d = tmp_path / who
+            d.mkdir(exist_ok=True)
+            tri = write_tri(str(d / name), tris, payload)
+            exe = (CLI_PAYLOAD if payload else CLI_BINARY) if who == "gpu" else (
+                oracle.CLI_PAYLOAD if payload else oracle.CLI_BINARY)
+            r = subprocess.run([exe, "-f", tri] + args, capture_output=True, text=True,
+                               timeout=120)
+            assert r.returncode == 0, r.stdout + r.stderr
+        size = args[1]
+        import ooc_svo_builder_amd as pkg
+        P = pkg.estimate_partitions(int(size), int(args[3]) if args[2] == "-l" else 2048)
+        base = "%s%s_%d" % (name, size, P)
+        assert files_identical(str(tmp_path / "gpu" / base), str(tmp_path / "ref" / base)), name

@@ -1,0 +1,84 @@
+"""Pins the oracle's partitioner (partitioner.cpp:12-149, BBoxBuffer.h:70-84,
+intersection.h:9-18,49-53) and the library's estimate_partitions rule.
+
+Closed form: P = 8^j for the smallest j with floor(G^3 / 2^20 / 8^j) <= L. Known
+answers: a triangle inside one octant goes to one partition, one touching a boundary
+plane goes to both sides (the box test is inclusive), one with a NaN x coordinate goes
+to every partition its y and z ranges touch (NaN comparisons never reject), one outside
+the grid goes nowhere. Lists keep the input order.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from ooc_svo_builder_amd import estimate_partitions as lib_estimate_partitions
+
+
+def closed_form(G, L):
+    req = G ** 3 // 2 ** 20
+    j = 0
+    while req // 8 ** j > L:
+        j += 1
+    return 8 ** j
+
+
+@pytest.mark.parametrize("G", [1, 2, 64, 256, 512, 1024, 2048, 4096, 8192])
+@pytest.mark.parametrize("L", [2, 3, 7, 8, 9, 63, 64, 65, 512, 1024, 2048, 8192, 65536])
+def test_estimate_partitions(G, L):
+    want = closed_form(G, L)
+    assert oracle.estimate_partitions(G, L) == want
+    assert lib_estimate_partitions(G, L) == want
+
+
+def test_survey_table():
+    assert oracle.estimate_partitions(256, 2048) == 1
+    assert oracle.estimate_partitions(1024, 2048) == 1
+    assert oracle.estimate_partitions(2048, 2048) == 8
+    assert oracle.estimate_partitions(2048, 8192) == 1
+    assert oracle.estimate_partitions(8192, 65536) == 8
+
+
+def _tri(*pts):
+    return np.array(pts, np.float32).reshape(1, 9)
+
+
+def test_partition_known_answers():
+    G = 256
+    bbox = (0, 0, 0, 1, 1, 1)
+    # octant order = Morton order of the octant corners: i = x + 2y + 4z
+    inside = _tri([0.1, 0.1, 0.1], [0.2, 0.1, 0.1], [0.1, 0.2, 0.1])          # octant 0
+    upper = _tri([0.9, 0.9, 0.9], [0.8, 0.9, 0.9], [0.9, 0.8, 0.9])           # octant 7
+    touch = _tri([0.5, 0.1, 0.1], [0.6, 0.1, 0.1], [0.6, 0.2, 0.1])           # x = 0.5
+    nan = _tri([np.nan, 0.1, 0.1], [0.2, 0.1, 0.1], [0.1, 0.2, 0.1])
+    outside = _tri([1.5, 1.5, 1.5], [1.6, 1.5, 1.5], [1.5, 1.6, 1.5])
+    tris = np.concatenate([inside, upper, touch, nan, outside])
+    counts, lists = oracle.partition(tris, False, bbox, G, 8)
+    member = [set(int(t) for t in l) for l in lists]
+    assert [0 in m for m in member] == [True] + [False] * 7
+    assert [1 in m for m in member] == [False] * 7 + [True]
+    assert [2 in m for m in member] == [True, True] + [False] * 6
+    # NaN in x: the x test never rejects, y and z still do -> both x halves of (y0, z0)
+    assert [3 in m for m in member] == [True, True] + [False] * 6
+    assert not any(4 in m for m in member)
+    for l in lists:
+        assert list(l) == sorted(l)
+
+
+def test_partition_matches_box_test():
+    """Random triangles against the 64 partition boxes of a 512 grid."""
+    rng = np.random.default_rng(3)
+    G, P = 512, 64
+    c = rng.uniform(-0.05, 1.05, (3000, 1, 3))
+    tris = (c + rng.normal(0, 0.03, (3000, 3, 3))).astype(np.float32).reshape(-1, 9)
+    counts, lists = oracle.partition(tris, False, (0, 0, 0, 1, 1, 1), G, P)
+    u = np.float32(1.0) / np.float32(G)
+    side = G // 4
+    t = tris.reshape(-1, 3, 3)
+    mn, mx = t.min(1), t.max(1)
+    for j in range(P):
+        cx, cy, cz = (int(v) for v in oracle.morton_decode([j]))
+        lo = np.array([cx, cy, cz], np.float32) * np.float32(side) * u
+        hi = (np.array([cx, cy, cz], np.float32) + 1) * np.float32(side) * u
+        hit = ~((mx < lo).any(1) | (mn > hi).any(1))
+        assert np.array_equal(np.nonzero(hit)[0], lists[j].astype(np.int64)), j
+        assert counts[j] == hit.sum()
