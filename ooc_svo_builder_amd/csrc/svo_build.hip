@@ -45,81 +45,130 @@ __device__ __forceinline__ void put_node(uint64_t* nodes, uint64_t pos, uint64_t
   r[2] = offs;
 }
 
-struct LevelView {
-  const uint64_t* code;  // sorted codes of this level
-  const uint64_t* size;  // subtree sizes (nullptr on the leaf level: all 1)
-  uint64_t n;
+// Level arrays live in arenas (code, size, mask, first child, start), level l at element
+// offset off[l]; the per-level counts n[l] and offsets are kept on the device so that the
+// passes run back to back without host round trips. Level D (the leaves) is the voxel
+// code stream itself, with implicit size 1.
+struct LevelMeta {
+  unsigned long long* n;    // n[l], l = 0..D
+  unsigned long long* off;  // off[l], l = 0..D-1
+  unsigned long long* flag; // [0] arena overflow, [1] node range error, [2] root size
 };
 
-// Parent level from a child level: head = first child of a parent group.
+struct Arena {
+  uint64_t* code;
+  uint64_t* size;
+  uint8_t* mask;
+  uint32_t* first;
+  uint64_t* start;
+  uint64_t cap;
+};
+
+// Parent level l from child level l+1: head = first child of a parent group.
 struct HeadLoad {
-  const uint64_t* code;
+  Arena A;
+  LevelMeta M;
+  const uint64_t* leaves;
+  int l, D;
   __device__ uint32_t operator()(uint64_t i) const {
+    // an overflowed child level is not read (its tail lies beyond the arena)
+    if (l + 1 < D && M.off[l + 1] + M.n[l + 1] > A.cap) return 0u;
+    const uint64_t* code = (l + 1 == D) ? leaves : A.code + M.off[l + 1];
     return (i == 0 || (code[i] >> 3) != (code[i - 1] >> 3)) ? 1u : 0u;
   }
 };
 struct ParentStore {
-  LevelView ch;
-  uint64_t* pcode;
-  uint64_t* psize;
-  uint8_t* pmask;
-  uint32_t* pfirst;
-  ull* total;
+  Arena A;
+  LevelMeta M;
+  const uint64_t* leaves;
+  int l, D;
   __device__ void operator()(uint64_t i, uint32_t ex, uint32_t head) const {
-    if (head) {
-      const uint64_t pc = ch.code[i] >> 3;
-      uint32_t m = 0;
-      uint64_t sz = 1;
-      for (uint64_t j = i; j < ch.n && j < i + 8; j++) {
-        const uint64_t cj = ch.code[j];
-        if ((cj >> 3) != pc) break;
-        m |= 1u << (cj & 7);
-        sz += ch.size ? ch.size[j] : 1ull;
-      }
-      pcode[ex] = pc;
-      psize[ex] = sz;
-      pmask[ex] = (uint8_t)m;
-      pfirst[ex] = (uint32_t)i;
+    const bool leaf = (l + 1 == D);
+    if (!leaf && M.off[l + 1] + M.n[l + 1] > A.cap) {
+      M.flag[0] = 1;
+      return;
     }
-    if (i == ch.n - 1) *total = (ull)(ex + head);
+    const uint64_t* code = leaf ? leaves : A.code + M.off[l + 1];
+    const uint64_t* csize = leaf ? nullptr : A.size + M.off[l + 1];
+    const uint64_t nch = M.n[l + 1];
+    const uint64_t base = M.off[l];
+    if (head) {
+      if (base + ex >= A.cap) {
+        M.flag[0] = 1;
+      } else {
+        const uint64_t pc = code[i] >> 3;
+        uint32_t m = 0;
+        uint64_t sz = 1;
+        for (uint64_t j = i; j < nch && j < i + 8; j++) {
+          const uint64_t cj = code[j];
+          if ((cj >> 3) != pc) break;
+          m |= 1u << (cj & 7);
+          sz += csize ? csize[j] : 1ull;
+        }
+        A.code[base + ex] = pc;
+        A.size[base + ex] = sz;
+        A.mask[base + ex] = (uint8_t)m;
+        A.first[base + ex] = (uint32_t)i;
+      }
+    }
+    if (i == nch - 1) {
+      const uint64_t np = (uint64_t)ex + head;
+      M.n[l] = np;
+      if (l > 0) M.off[l - 1] = base + np;
+    }
   }
 };
 
 // One thread per parent at level l writes its children's records (level l+1).
-__global__ __launch_bounds__(256) void k_emit_children(
-    uint64_t np, const uint64_t* __restrict__ pstart, const uint64_t* __restrict__ psize,
-    const uint8_t* __restrict__ pmask, const uint32_t* __restrict__ pfirst,
-    const uint64_t* __restrict__ csize, const uint8_t* __restrict__ cmask,
-    uint64_t* __restrict__ cstart, int child_is_leaf, int payload, uint64_t leaf_base,
-    uint64_t* __restrict__ nodes) {
+__global__ __launch_bounds__(256) void k_emit_children(Arena A, LevelMeta M, int l, int D,
+                                                       int payload, uint64_t node_cap,
+                                                       uint64_t* __restrict__ nodes) {
+  if (M.flag[0]) return;  // arena overflow: the build is redone with a larger arena
+  const uint64_t np = M.n[l];
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const uint32_t m = pmask[p];
+  const bool child_is_leaf = (l + 1 == D);
+  const uint64_t po = M.off[l] + p;
+  const uint64_t co = child_is_leaf ? 0 : M.off[l + 1];
+  const uint32_t m = A.mask[po];
   const uint32_t cnt = (uint32_t)__popc(m);
-  const uint64_t st = pstart[p];
-  const uint64_t cb = st + psize[p] - 1 - cnt;
-  const uint32_t f = pfirst[p];
+  const uint64_t st = A.start[po];
+  const uint64_t cb = st + A.size[po] - 1 - cnt;
+  const uint32_t f = A.first[po];
+  if (cb + cnt > node_cap) {
+    M.flag[1] = 1;
+    return;
+  }
   uint64_t run = st;
   for (uint32_t r = 0; r < cnt; r++) {
     const uint64_t c = (uint64_t)f + r;
     if (child_is_leaf) {
       // OctreeBuilder.cpp:137-139 (binary: data 1) / :157-159 (payload: its data index)
-      put_node(nodes, cb + r, payload ? (leaf_base + c) : 1ull, 0ull, ~0ull);
+      put_node(nodes, cb + r, payload ? (1ull + c) : 1ull, 0ull, ~0ull);
     } else {
-      const uint64_t sz = csize[c];
-      const uint32_t cm = cmask[c];
-      cstart[c] = run;
+      const uint64_t sz = A.size[co + c];
+      const uint32_t cm = A.mask[co + c];
+      A.start[co + c] = run;
       put_node(nodes, cb + r, 0ull, run + sz - 1 - (uint32_t)__popc(cm), offsets_of(cm));
       run += sz - 1;
     }
   }
 }
 
-__global__ void k_emit_root(const uint64_t* size, const uint8_t* mask, uint64_t* start,
-                            uint64_t* nodes) {
-  const uint32_t m = mask[0];
-  const uint64_t sz = size[0];
-  start[0] = 0;
+__global__ void k_emit_root(Arena A, LevelMeta M, uint64_t node_cap, uint64_t* nodes) {
+  if (M.flag[0] || M.n[0] != 1) {  // arena overflow, or not a single root
+    M.flag[0] = 1;
+    return;
+  }
+  const uint64_t o = M.off[0];
+  const uint32_t m = A.mask[o];
+  const uint64_t sz = A.size[o];
+  A.start[o] = 0;
+  M.flag[2] = sz;
+  if (sz > node_cap) {
+    M.flag[1] = 1;
+    return;
+  }
   put_node(nodes, sz - 1, 0ull, sz - 1 - (uint32_t)__popc(m), offsets_of(m));
 }
 
@@ -139,6 +188,10 @@ hipError_t build_octree(Context& c, uint64_t gridsize, int levels) {
   const uint64_t V = c.n_vox;
   const uint64_t total_codes = gridsize * gridsize * gridsize;
   c.hazard = 0;
+  if (V >= (1ull << 32)) {
+    set_error("more than 2^32 voxels in one octree");
+    return hipErrorInvalidValue;
+  }
 
   // data records (octree_io.h:49-53, OctreeBuilder.cpp:25-29)
   if (!c.payload) {
@@ -152,106 +205,83 @@ hipError_t build_octree(Context& c, uint64_t gridsize, int levels) {
     c.n_data = 1 + V;
   }
 
-  if (V == 0) {  // one null root (finalizeTree padding the whole grid)
-    if (gridsize == 1) c.hazard = 1;
-    static const uint64_t root[3] = {0, 0, ~0ull};
+  if (V == 0 || D == 0) {
+    // empty grid: one null root (finalizeTree pads the whole grid); 1^3 grid with its
+    // single voxel: the root is that leaf
+    if (V == 0 && gridsize == 1) c.hazard = 1;
+    const uint64_t rec[3] = {V ? 1ull : 0ull, 0, ~0ull};
     SVO_TRY(ensure(c.nodes, 24));
-    SVO_TRY(hipMemcpyAsync(c.nodes.p, root, 24, hipMemcpyHostToDevice, c.stream));
+    SVO_TRY(hipMemcpyAsync(c.nodes.p, rec, 24, hipMemcpyHostToDevice, c.stream));
     c.n_nodes = 1;
     return hipStreamSynchronize(c.stream);
   }
+
+  // per-level upper bounds (host) size the launch grids
+  std::vector<uint64_t> bound(D + 1);
+  for (int l = 0; l <= D; l++) bound[l] = (3 * l >= 63) ? V : std::min<uint64_t>(V, 1ull << (3 * l));
+  uint64_t cap = std::max<uint64_t>(V / 2 + 1024, 4096);
+  for (int attempt = 0;; attempt++) {
+    SVO_TRY(ensure(c.lvl_code, cap * 8));
+    SVO_TRY(ensure(c.lvl_size, cap * 8));
+    SVO_TRY(ensure(c.lvl_mask, cap));
+    SVO_TRY(ensure(c.lvl_first, cap * 4));
+    SVO_TRY(ensure(c.lvl_start, cap * 8));
+    SVO_TRY(ensure(c.lvl_aux, (2 * 64 + 8) * 8));
+    const uint64_t node_cap = cap + V + 1;
+    SVO_TRY(ensure(c.nodes, node_cap * 24));
+    SVO_TRY(ensure(c.scan_scratch, scan_scratch_elems(V) * 4 + 256));
+    Arena A{c.lvl_code.as<uint64_t>(), c.lvl_size.as<uint64_t>(), c.lvl_mask.as<uint8_t>(),
+            c.lvl_first.as<uint32_t>(), c.lvl_start.as<uint64_t>(), cap};
+    unsigned long long* meta = c.lvl_aux.as<unsigned long long>();
+    LevelMeta M{meta, meta + 64, meta + 128};
+    // n[D] = V, off[D-1] = 0, flags = 0
+    SVO_TRY(fill_u64((uint64_t*)meta, 0, 2 * 64 + 8, c.stream));
+    SVO_TRY(hipMemcpyAsync(meta + D, &V, 8, hipMemcpyHostToDevice, c.stream));
+
+    // ---- bottom-up: levels D-1 .. 0
+    for (int l = D - 1; l >= 0; l--) {
+      HeadLoad ld{A, M, c.codes.as<uint64_t>(), l, D};
+      ParentStore st{A, M, c.codes.as<uint64_t>(), l, D};
+      SVO_TRY((scan_exclusive<uint32_t>(ld, st, bound[l + 1], c.scan_scratch.as<uint32_t>(),
+                                        c.stream, M.n + (l + 1))));
+    }
+    // ---- top-down
+    hipLaunchKernelGGL(k_emit_root, dim3(1), dim3(1), 0, c.stream, A, M, node_cap,
+                       c.nodes.as<uint64_t>());
+    SVO_TRY(hipGetLastError());
+    for (int l = 0; l < D; l++) {
+      ktimer_begin(c, "emit_children");
+      hipLaunchKernelGGL(k_emit_children, dim3((unsigned)((bound[l] + 255) / 256)), dim3(256), 0,
+                         c.stream, A, M, l, D, c.payload, node_cap, c.nodes.as<uint64_t>());
+      ktimer_end(c);
+      SVO_TRY(hipGetLastError());
+    }
+    unsigned long long flags[3] = {0, 0, 0};
+    SVO_TRY(hipMemcpyAsync(flags, M.flag, 24, hipMemcpyDeviceToHost, c.stream));
+    SVO_TRY(hipStreamSynchronize(c.stream));
+    ktimer_flush(c);
+    if (flags[1]) {
+      set_error("internal: node record out of range");
+      return hipErrorUnknown;
+    }
+    if (!flags[0]) {
+      c.n_nodes = flags[2];
+      break;
+    }
+    if (attempt > 8) {
+      set_error("internal: level arena could not be sized");
+      return hipErrorUnknown;
+    }
+    cap *= 4;
+  }
   {
     uint64_t last = 0;
-    SVO_TRY(hipMemcpyAsync(&last, c.codes.as<uint64_t>() + V - 1, 8, hipMemcpyDeviceToHost,
-                           c.stream));
-    SVO_TRY(hipStreamSynchronize(c.stream));
+    SVO_TRY(hipMemcpy(&last, c.codes.as<uint64_t>() + V - 1, 8, hipMemcpyDeviceToHost));
     // OctreeBuilder.cpp:36-42 leaves the root undefined when the last voxel is the
     // second-to-last code; the completed tree is built and the hazard reported
     if (total_codes >= 2 && last == total_codes - 2) c.hazard = 1;
   }
-  if (D == 0) {  // 1^3 grid with its single voxel: the root is the leaf
-    const uint64_t leaf[3] = {c.payload ? 1ull : 1ull, 0, ~0ull};
-    SVO_TRY(ensure(c.nodes, 24));
-    SVO_TRY(hipMemcpyAsync(c.nodes.p, leaf, 24, hipMemcpyHostToDevice, c.stream));
-    c.n_nodes = 1;
-    return hipStreamSynchronize(c.stream);
-  }
-
-  // ---- bottom-up: levels D-1 .. 0. Level arrays are packed one after another in
-  // arenas; off[l] is the first element of level l.
-  std::vector<uint64_t> n(D + 1, 0), off(D + 1, 0);
-  n[D] = V;
-  uint64_t cap = std::max<uint64_t>(V + V / 2 + 64, 1024), used = 0;
-  auto alloc_levels = [&](uint64_t nc) -> hipError_t {
-    SVO_TRY(ensure_keep(c.lvl_code, nc * 8, used * 8, c.stream));
-    SVO_TRY(ensure_keep(c.lvl_size, nc * 8, used * 8, c.stream));
-    SVO_TRY(ensure_keep(c.lvl_mask, nc, used, c.stream));
-    SVO_TRY(ensure_keep(c.lvl_first, nc * 4, used * 4, c.stream));
-    SVO_TRY(ensure(c.lvl_start, nc * 8));
-    cap = nc;
-    return hipSuccess;
-  };
-  auto grow = [&](uint64_t need) -> hipError_t {
-    if (need <= cap) return hipSuccess;
-    return alloc_levels(std::max(need, cap * 2));
-  };
-  SVO_TRY(alloc_levels(cap));
-  ull* ctr = c.counters.as<ull>();
-  ull hn = 0;
-  for (int l = D - 1; l >= 0; l--) {
-    const uint64_t nch = n[l + 1];
-    SVO_TRY(grow(used + nch));
-    off[l] = used;
-    LevelView ch;
-    ch.code = (l + 1 == D) ? c.codes.as<uint64_t>() : c.lvl_code.as<uint64_t>() + off[l + 1];
-    ch.size = (l + 1 == D) ? nullptr : c.lvl_size.as<uint64_t>() + off[l + 1];
-    ch.n = nch;
-    ParentStore st{ch,
-                   c.lvl_code.as<uint64_t>() + used,
-                   c.lvl_size.as<uint64_t>() + used,
-                   c.lvl_mask.as<uint8_t>() + used,
-                   c.lvl_first.as<uint32_t>() + used,
-                   ctr + kCtrLevelN};
-    SVO_TRY(ensure(c.scan_scratch, scan_scratch_elems(nch) * 4 + 256));
-    SVO_TRY((scan_exclusive<uint32_t>(HeadLoad{ch.code}, st, nch,
-                                      c.scan_scratch.as<uint32_t>(), c.stream)));
-    SVO_TRY(hipMemcpyAsync(&hn, ctr + kCtrLevelN, 8, hipMemcpyDeviceToHost, c.stream));
-    SVO_TRY(hipStreamSynchronize(c.stream));
-    n[l] = hn;
-    used += hn;
-  }
-  // root level must hold exactly the root
-  if (n[0] != 1) {
-    set_error("internal: root level does not hold one node");
-    return hipErrorUnknown;
-  }
-  uint64_t size_root = 0;
-  SVO_TRY(hipMemcpyAsync(&size_root, c.lvl_size.as<uint64_t>() + off[0], 8,
-                         hipMemcpyDeviceToHost, c.stream));
-  SVO_TRY(hipStreamSynchronize(c.stream));
-  c.n_nodes = size_root;
-  SVO_TRY(ensure(c.nodes, size_root * 24));
-
-  // ---- top-down
-  uint64_t* start = c.lvl_start.as<uint64_t>();
-  hipLaunchKernelGGL(k_emit_root, dim3(1), dim3(1), 0, c.stream, c.lvl_size.as<uint64_t>() + off[0],
-                     c.lvl_mask.as<uint8_t>() + off[0], start + off[0], c.nodes.as<uint64_t>());
-  SVO_TRY(hipGetLastError());
-  for (int l = 0; l < D; l++) {
-    const bool leaf = (l + 1 == D);
-    const uint64_t np = n[l];
-    ktimer_begin(c, "emit_children");
-    hipLaunchKernelGGL(k_emit_children, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c.stream,
-                       np, start + off[l], c.lvl_size.as<uint64_t>() + off[l],
-                       c.lvl_mask.as<uint8_t>() + off[l], c.lvl_first.as<uint32_t>() + off[l],
-                       leaf ? nullptr : c.lvl_size.as<uint64_t>() + off[l + 1],
-                       leaf ? nullptr : c.lvl_mask.as<uint8_t>() + off[l + 1],
-                       leaf ? nullptr : start + off[l + 1], leaf ? 1 : 0, c.payload, 1ull,
-                       c.nodes.as<uint64_t>());
-    ktimer_end(c);
-    SVO_TRY(hipGetLastError());
-  }
-  return hipStreamSynchronize(c.stream);
+  return hipSuccess;
 }
 
 }  // namespace svo

@@ -269,6 +269,7 @@ int svo_mesh_load(svo_ctx* ctx, const float* records, uint64_t n_triangles, int 
     TRYC(hipMemcpyAsync(c.tris.p, records, bytes,
                         src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c.stream),
          "svo_mesh_load");
+  TRYC(fill_u64(c.counters.as<uint64_t>(), 0, kNumCounters, c.stream), "svo_mesh_load");
   TRYC(hipStreamSynchronize(c.stream), "svo_mesh_load");
   c.n_tris = n_triangles;
   c.payload = payload ? 1 : 0;
@@ -385,6 +386,14 @@ int svo_build(svo_ctx* ctx, uint64_t gridsize, int generate_levels, uint64_t* n_
   }
   hipError_t e = build_octree(c, gridsize, generate_levels);
   if (e != hipSuccess) return fail(e, "svo_build");
+  unsigned long long err = 0;
+  TRYC(hipMemcpy(&err, c.counters.as<unsigned long long>() + kCtrError, 8, hipMemcpyDeviceToHost),
+       "svo_build");
+  if (err) {
+    g_err = "internal: " + std::to_string(err) +
+            " voxels without a covering triangle in the payload pass";
+    return SVO_ERR_DEVICE;
+  }
   if (n_nodes_out) *n_nodes_out = c.n_nodes;
   if (n_data_out) *n_data_out = c.n_data;
   if (hazard_out) *hazard_out = c.hazard;
@@ -487,6 +496,7 @@ int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payloa
     c.n_vox = 0;
     c.nfilled = 0;
     c.n_nodes = c.n_data = 0;
+    TRYC(fill_u64(c.counters.as<uint64_t>(), 0, kNumCounters, c.stream), "svo_run");
   }
   const uint64_t np = svo_estimate_partitions(gridsize, memory_limit_mb);
   const float u_part = (bbox[3] - bbox[0]) / (float)gridsize;  // partitioner.cpp:45

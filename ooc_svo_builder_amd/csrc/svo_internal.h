@@ -25,6 +25,7 @@ enum Counter : int {
   kCtrNumVox = 4,     // voxels of the current partition
   kCtrLevelN = 5,     // node count of the level being built
   kCtrCols = 6,       // columns of the large tier
+  kCtrError = 7,      // sticky: internal consistency failures (cleared per job)
   kNumCounters = 32
 };
 
@@ -39,7 +40,7 @@ struct DevBuf {
 
 // Voxelizer tuning: small/large split threshold (columns per triangle) and block size.
 struct VoxTuning {
-  int small_cols = 16;
+  int small_cols = 1024;
   int block = 256;
 };
 

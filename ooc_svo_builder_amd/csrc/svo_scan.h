@@ -15,9 +15,15 @@ constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
 constexpr uint64_t kScanTile = (uint64_t)kScanBlock * kScanItems;
 
+// n_dev (optional) holds the actual element count on the device; n is then an upper
+// bound used only to size the grid, and elements at or beyond *n_dev are treated as
+// absent (loads give 0, stores are skipped).
 template <typename T, typename Load>
-__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(Load ld, uint64_t n, T* sums) {
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(Load ld, uint64_t n,
+                                                            const unsigned long long* n_dev,
+                                                            T* sums) {
   __shared__ T sm[kScanBlock / kWave];
+  if (n_dev) n = min(n, (uint64_t)*n_dev);
   const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
   T acc = 0;
 #pragma unroll
@@ -32,9 +38,12 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(Load ld, uint64_t n,
 
 template <typename T, typename Load, typename Store>
 __global__ __launch_bounds__(kScanBlock) void k_scan_apply(Load ld, Store st, uint64_t n,
+                                                           const unsigned long long* n_dev,
                                                            const T* prefix) {
   __shared__ T sm[kScanBlock / kWave];
+  if (n_dev) n = min(n, (uint64_t)*n_dev);
   const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  if (base >= n && blockIdx.x != 0) return;  // uniform per block
   T carry = prefix ? prefix[blockIdx.x] : (T)0;
 #pragma unroll
   for (int k = 0; k < kScanItems; k++) {
@@ -68,24 +77,28 @@ inline uint64_t scan_scratch_elems(uint64_t n) {
   return tot + 1;
 }
 
-// Exclusive scan of n elements. 'scratch' holds scan_scratch_elems(n) T's.
+// Exclusive scan of n elements ('n_dev' optional: actual count on the device, n being an
+// upper bound). 'scratch' holds scan_scratch_elems(n) T's. Stores for an empty input
+// (actual count 0) never run, so a Store that publishes the total must have its total
+// slot cleared beforehand.
 template <typename T, typename Load, typename Store>
-hipError_t scan_exclusive(Load ld, Store st, uint64_t n, T* scratch, hipStream_t s) {
+hipError_t scan_exclusive(Load ld, Store st, uint64_t n, T* scratch, hipStream_t s,
+                          const unsigned long long* n_dev = nullptr) {
   if (n == 0) return hipSuccess;
   const uint64_t nb = (n + kScanTile - 1) / kScanTile;
   if (nb == 1) {
     hipLaunchKernelGGL((k_scan_apply<T, Load, Store>), dim3(1), dim3(kScanBlock), 0, s, ld, st,
-                       n, (const T*)nullptr);
+                       n, n_dev, (const T*)nullptr);
     return hipGetLastError();
   }
   T* sums = scratch;
   hipLaunchKernelGGL((k_scan_reduce<T, Load>), dim3((unsigned)nb), dim3(kScanBlock), 0, s, ld,
-                     n, sums);
+                     n, n_dev, sums);
   hipError_t e = scan_exclusive<T>(ArrayLoad<T>{sums}, ArrayStoreExcl<T>{sums}, nb,
                                    scratch + nb, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_scan_apply<T, Load, Store>), dim3((unsigned)nb), dim3(kScanBlock), 0, s,
-                     ld, st, n, (const T*)sums);
+                     ld, st, n, n_dev, (const T*)sums);
   return hipGetLastError();
 }
 

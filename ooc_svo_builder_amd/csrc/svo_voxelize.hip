@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 
 #include "svo_device.h"
 #include "svo_internal.h"
@@ -43,6 +44,7 @@ struct VoxParams {
   int pmin[3], pmax[3];
   float unit, unit_div;
   float coord_limit;  // |vertex| beyond this walks full columns
+  double slab_margin; // voxels added on each side of the exact plane slab
   int small_cols;
 };
 
@@ -142,91 +144,214 @@ __device__ __forceinline__ void setup_tri(const float* __restrict__ p, const Vox
   S.nm = ((double)nx * (u - ccx) + (double)ny * (u - ccy)) + (double)nz * (u - ccz);
 }
 
-__device__ __forceinline__ uint64_t tri_columns(const Setup& S) {
-  const int ua = (S.w == 0) ? 1 : 0;
-  const int va = (S.w == 2) ? 1 : 2;
-  return (uint64_t)(S.g1[ua] - S.g0[ua] + 1) * (uint64_t)(S.g1[va] - S.g0[va] + 1);
+// ---------------------------------------------------------------- column walk
+// A triangle's voxels are tested column by column along its column axis w (the axis of
+// the dominant normal component); a column is fixed by its coordinates (a, b) on the two
+// other axes ua < va. The per-triangle constants are re-slotted once so that one
+// branch-free code path serves all three axes:
+//   * plane test t = (n0*px + n1*py) + n2*pz becomes t = (nw*pc + Ka) + Kb with
+//     Ka, Kb per column (Kb = +0 when w = z; a zero's sign never changes the decision);
+//   * the projection test in the plane across w (3 edges) depends on (pa, pb) only and
+//     is evaluated once per column: (h0*pa + h1*pb) + hd;
+//   * each edge of the two other planes is (cv*pc + cc*pconst) + de per voxel.
+// Every product and sum is the reference's own (voxelizer.cpp:257-287); only operands of
+// single additions are swapped, which is exact in IEEE arithmetic.
+enum WalkF {
+  F_NW = 0, F_D1, F_D2, F_NUA, F_NVA,
+  F_H0 = 5, F_H1 = 8, F_HD = 11,
+  F_AV = 14, F_AC = 17, F_AD = 20,
+  F_BV = 23, F_BC = 26, F_BD = 29,
+  F_KA = 32, F_KB, F_ALO, F_AHI,
+  kWalkF = 36
+};
+enum WalkI { I_G0U = 0, I_G1U, I_G0V, I_G1V, I_G0W, I_G1W, I_WS, I_TRI, kWalkI = 8 };
+constexpr int kWalkWords = kWalkF + kWalkI;
+
+struct Walk {
+  float f[kWalkF];
+  int i[kWalkI];
+  __device__ __forceinline__ int w() const { return i[I_WS] & 3; }
+  __device__ __forceinline__ bool slab() const { return (i[I_WS] >> 2) & 1; }
+  __device__ __forceinline__ uint64_t columns() const {
+    return (uint64_t)(i[I_G1U] - i[I_G0U] + 1) * (uint64_t)(i[I_G1V] - i[I_G0V] + 1);
+  }
+  __device__ __forceinline__ uint64_t work() const {
+    return slab() ? columns() : columns() * (uint64_t)(i[I_G1W] - i[I_G0W] + 1);
+  }
+};
+
+// Copy one axis' plane roles: HP/AP/BP = first edge of the hoisted plane and of planes
+// A and B; HS = the hoisted plane's operands come as (pb, pa); AV0/BV0 = the column
+// coordinate is the plane's first operand.
+template <int HP, int AP, int BP, bool HS, bool AV0, bool BV0>
+__device__ __forceinline__ void fill_planes(const Setup& S, Walk& K) {
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float hx = S.ne[HP + k][0], hy = S.ne[HP + k][1];
+    K.f[F_H0 + k] = HS ? hy : hx;
+    K.f[F_H1 + k] = HS ? hx : hy;
+    K.f[F_HD + k] = S.de[HP + k];
+    const float ax = S.ne[AP + k][0], ay = S.ne[AP + k][1];
+    K.f[F_AV + k] = AV0 ? ax : ay;
+    K.f[F_AC + k] = AV0 ? ay : ax;
+    K.f[F_AD + k] = S.de[AP + k];
+    const float bx = S.ne[BP + k][0], by = S.ne[BP + k][1];
+    K.f[F_BV + k] = BV0 ? bx : by;
+    K.f[F_BC + k] = BV0 ? by : bx;
+    K.f[F_BD + k] = S.de[BP + k];
+  }
 }
 
-// Test the voxels of column (a, b) (coordinates on the two axes other than S.w) and hand
-// every accepted voxel to act(code, tri). voxelizer.cpp:257-303.
-template <typename Act>
-__device__ __forceinline__ void walk_column(const Setup& S, const VoxParams& P, int a, int b,
-                                            uint32_t tri, const Act& act) {
-  const float u = P.unit;
+// Re-slot a triangle's setup (voxelizer.cpp:189-254, see setup_tri) for the column walk.
+__device__ __forceinline__ void make_walk(const Setup& S, const VoxParams& P, uint32_t tri,
+                                          Walk& K) {
   const int w = S.w;
   const int ua = (w == 0) ? 1 : 0;
   const int va = (w == 2) ? 1 : 2;
+  // edges of the three planes: xy = ne[0..2], yz = ne[3..5], zx = ne[6..8]
+  if (w == 2)         // hoisted xy(pa, pb); A = yz(pb, pc); B = zx(pc, pa)
+    fill_planes<0, 3, 6, false, false, true>(S, K);
+  else if (w == 0)    // hoisted yz(pa, pb); A = xy(pc, pa); B = zx(pb, pc)
+    fill_planes<3, 0, 6, false, true, false>(S, K);
+  else                // hoisted zx(pb, pa); A = xy(pa, pc); B = yz(pc, pb)
+    fill_planes<6, 0, 3, true, false, true>(S, K);
+  const float n0 = S.n[0], n1 = S.n[1], n2 = S.n[2];
+  const float nw = (w == 0) ? n0 : (w == 1 ? n1 : n2);
+  const float nua = (ua == 0) ? n0 : n1;
+  const float nva = (va == 1) ? n1 : n2;
+  K.f[F_NW] = nw;
+  K.f[F_D1] = S.d1;
+  K.f[F_D2] = S.d2;
+  K.f[F_NUA] = nua;
+  K.f[F_NVA] = nva;
+  // plane slab in index space along w: c in [ka*a + kb*b + alpha_lo, ... + alpha_hi]
+  if (S.slab) {
+    const double inw = 1.0 / (double)nw, iu = 1.0 / (double)P.unit;
+    const double a1 = -(S.nc - S.nv0) * inw * iu, a2 = -(S.nm - S.nv0) * inw * iu;
+    K.f[F_KA] = (float)(-(double)nua * inw);
+    K.f[F_KB] = (float)(-(double)nva * inw);
+    K.f[F_ALO] = (float)(fmin(a1, a2) - P.slab_margin);
+    K.f[F_AHI] = (float)(fmax(a1, a2) + P.slab_margin);
+  } else {
+    K.f[F_KA] = K.f[F_KB] = K.f[F_ALO] = K.f[F_AHI] = 0.0f;
+  }
+  if (w == 0) {
+    K.i[I_G0U] = S.g0[1]; K.i[I_G1U] = S.g1[1];
+    K.i[I_G0V] = S.g0[2]; K.i[I_G1V] = S.g1[2];
+    K.i[I_G0W] = S.g0[0]; K.i[I_G1W] = S.g1[0];
+  } else if (w == 1) {
+    K.i[I_G0U] = S.g0[0]; K.i[I_G1U] = S.g1[0];
+    K.i[I_G0V] = S.g0[2]; K.i[I_G1V] = S.g1[2];
+    K.i[I_G0W] = S.g0[1]; K.i[I_G1W] = S.g1[1];
+  } else {
+    K.i[I_G0U] = S.g0[0]; K.i[I_G1U] = S.g1[0];
+    K.i[I_G0V] = S.g0[1]; K.i[I_G1V] = S.g1[1];
+    K.i[I_G0W] = S.g0[2]; K.i[I_G1W] = S.g1[2];
+  }
+  K.i[I_WS] = w | (S.slab ? 4 : 0);
+  K.i[I_TRI] = (int)tri;
+}
+
+// Test the voxels of column (a, b) and hand every accepted one to act.add(code, tri).
+template <typename Act>
+__device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, int a, int b,
+                                            Act& act) {
+  const float u = P.unit;
   const float pa = (float)a * u;
   const float pb = (float)b * u;
-  // projection test in the plane across the column axis (constant along the column)
-  {
-    const int base = (w == 2) ? 0 : (w == 0 ? 3 : 6);
-    // xy uses (x, y) = (a, b); yz uses (y, z) = (a, b); zx uses (z, x) = (b, a)
-    const float q0 = (w == 1) ? pb : pa;
-    const float q1 = (w == 1) ? pa : pb;
 #pragma unroll
-    for (int k = 0; k < 3; k++)
-      if ((S.ne[base + k][0] * q0 + S.ne[base + k][1] * q1) + S.de[base + k] < 0.0f) return;
+  for (int k = 0; k < 3; k++)
+    if ((K.f[F_H0 + k] * pa + K.f[F_H1 + k] * pb) + K.f[F_HD + k] < 0.0f) return;
+  const int w = K.w();
+  int lo = K.i[I_G0W], hi = K.i[I_G1W];
+  if (K.slab()) {
+    const float base = K.f[F_KA] * (float)a + K.f[F_KB] * (float)b;
+    const float l = fminf(fmaxf(base + K.f[F_ALO], (float)lo - 1.0f), (float)hi + 1.0f);
+    const float h = fmaxf(fminf(base + K.f[F_AHI], (float)hi + 1.0f), (float)lo - 1.0f);
+    lo = max(lo, (int)floorf(l));
+    hi = min(hi, (int)floorf(h));
   }
-  int lo = S.g0[w], hi = S.g1[w];
-  if (S.slab) {
-    const double du = (double)u;
-    const double nu = (double)S.n[ua], nvv = (double)S.n[va], nw = (double)S.n[w];
-    const double A = (nu * (double)pa + nvv * (double)pb) - S.nv0;
-    const double L0 = -(A + S.nc) / nw;
-    const double H0 = -(A + S.nm) / nw;
-    double l = fmin(L0, H0) / du - 2.0, h = fmax(L0, H0) / du + 2.0;
-    l = fmin(fmax(l, (double)lo - 1.0), (double)hi + 1.0);
-    h = fmax(fmin(h, (double)hi + 1.0), (double)lo - 1.0);
-    const int li = (int)floor(l), hi2 = (int)floor(h);
-    lo = max(lo, li);
-    hi = min(hi, hi2);
+  if (lo > hi) return;
+  const float tua = K.f[F_NUA] * pa, tva = K.f[F_NVA] * pb;
+  const float Ka = (w == 2) ? (tua + tva) : tua;
+  const float Kb = (w == 2) ? 0.0f : tva;
+  const float cA = (w == 2) ? pb : pa;
+  const float cB = (w == 2) ? pa : pb;
+  float KA[3], KB[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    KA[k] = K.f[F_AC + k] * cA;
+    KB[k] = K.f[F_BC + k] * cB;
   }
+  const int ua = (w == 0) ? 1 : 0;
+  const int va = (w == 2) ? 1 : 2;
   const uint64_t cbase = (spread3((uint32_t)a) << ua) | (spread3((uint32_t)b) << va);
+  const float nw = K.f[F_NW], d1 = K.f[F_D1], d2 = K.f[F_D2];
   for (int c = lo; c <= hi; c++) {
     const float pc = (float)c * u;
-    const float px = (w == 0) ? pc : pa;
-    const float py = (w == 1) ? pc : ((w == 0) ? pa : pb);
-    const float pz = (w == 2) ? pc : pb;
-    const float t = (S.n[0] * px + S.n[1] * py) + S.n[2] * pz;
-    if ((t + S.d1) * (t + S.d2) > 0.0f) continue;
+    const float t = (nw * pc + Ka) + Kb;
+    if ((t + d1) * (t + d2) > 0.0f) continue;
     bool ok = true;
-    if (w != 2) {
 #pragma unroll
-      for (int k = 0; k < 3; k++)
-        if ((S.ne[k][0] * px + S.ne[k][1] * py) + S.de[k] < 0.0f) ok = false;
-    }
-    if (w != 0) {
-#pragma unroll
-      for (int k = 0; k < 3; k++)
-        if ((S.ne[3 + k][0] * py + S.ne[3 + k][1] * pz) + S.de[3 + k] < 0.0f) ok = false;
-    }
-    if (w != 1) {
-#pragma unroll
-      for (int k = 0; k < 3; k++)
-        if ((S.ne[6 + k][0] * pz + S.ne[6 + k][1] * px) + S.de[6 + k] < 0.0f) ok = false;
+    for (int k = 0; k < 3; k++) {
+      if ((K.f[F_AV + k] * pc + KA[k]) + K.f[F_AD + k] < 0.0f) ok = false;
+      if ((K.f[F_BV + k] * pc + KB[k]) + K.f[F_BD + k] < 0.0f) ok = false;
     }
     if (!ok) continue;
-    act(cbase | (spread3((uint32_t)c) << w), tri);
+    act.add(cbase | (spread3((uint32_t)c) << w), (uint32_t)K.i[I_TRI]);
   }
 }
 
-// Set a voxel's bit in the sparse brick grid. New bricks are taken from the pool with
+// Set voxels' bits in the sparse brick grid. A thread collects its bits per 64-bit word
+// in a small register buffer and sets each word with one atomicOr when the buffer is
+// full or the thread is done (mostly at a point where the whole wave has converged).
+// The brick table lookup is cached per thread. New bricks are taken from the pool with
 // one atomic per distinct brick per wave; a lost install race leaves a zeroed, unused
 // slot behind. An exhausted pool raises the overflow counter and the pass is redone
 // with a larger pool.
 struct MarkAction {
+  static constexpr int K = 4;
   uint32_t* table;
   ull* pool;
   ull* ctr;
   uint64_t pool_cap;
   uint64_t mstart;
-  __device__ void operator()(uint64_t code, uint32_t) const {
+  uint64_t wid[K];
+  ull bits[K];
+  int nb = 0;
+  uint32_t cbrick = 0xFFFFFFFFu, cslot = 0;
+
+  __device__ __forceinline__ void add(uint64_t code, uint32_t) {
     const uint64_t rel = code - mstart;
-    const uint32_t b = (uint32_t)(rel >> kBrickBits);
-    uint32_t s = __hip_atomic_load(&table[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    bool need = (s == 0);
+    const uint64_t wd = rel >> 6;
+    const ull bit = 1ull << (rel & 63);
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      if (k < nb && wid[k] == wd) {
+        bits[k] |= bit;
+        hit = true;
+      }
+    if (hit) return;
+    if (nb == K) flush();
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      if (k == nb) {
+        wid[k] = wd;
+        bits[k] = bit;
+      }
+    nb++;
+  }
+
+  __device__ __forceinline__ uint32_t slot_of(uint32_t b, bool have) {
+    uint32_t s = 1u;
+    if (have) {
+      if (b == cbrick)
+        s = cslot;
+      else
+        s = __hip_atomic_load(&table[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    bool need = have && (s == 0);
     while (__any(need)) {
       const ull m = __ballot(need);
       const int leader = __ffsll(m) - 1;
@@ -248,9 +373,24 @@ struct MarkAction {
         need = false;
       }
     }
-    if (s == 0xFFFFFFFFu) return;
-    const uint32_t wi = (uint32_t)(rel >> 6) & (kBrickWords - 1);
-    atomicOr(&pool[(uint64_t)(s - 1) * kBrickWords + wi], 1ull << (rel & 63));
+    if (have && s != 0xFFFFFFFFu) {
+      cbrick = b;
+      cslot = s;
+    }
+    return s;
+  }
+
+  __device__ void flush() {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const bool have = k < nb;
+      const uint32_t b = (uint32_t)(wid[k] >> (kBrickBits - 6));
+      const uint32_t s = slot_of(b, have);
+      if (have && s != 0xFFFFFFFFu)
+        atomicOr(&pool[(uint64_t)(s - 1) * kBrickWords + (uint32_t)(wid[k] & (kBrickWords - 1))],
+                 bits[k]);
+    }
+    nb = 0;
   }
 };
 
@@ -262,7 +402,7 @@ struct WinnerAction {
   uint32_t* winner;
   ull* ctr;
   uint64_t mstart;
-  __device__ void operator()(uint64_t code, uint32_t tri) const {
+  __device__ __forceinline__ void add(uint64_t code, uint32_t tri) {
     const uint64_t rel = code - mstart;
     const uint32_t b = (uint32_t)(rel >> kBrickBits);
     const uint32_t s = table[b];
@@ -272,46 +412,95 @@ struct WinnerAction {
     // both passes evaluate the same predicates, so the voxel is marked; a voxel that is
     // not (which would be a bug) is counted and skipped rather than written
     if (s == 0 || !(pool[slot] & bit)) {
-      atomicAdd(&ctr[kCtrOverflow], 4ull);
+      atomicAdd(&ctr[kCtrError], 1ull);
       return;
     }
     const uint32_t rank = wprefix[slot] + (uint32_t)__popcll(pool[slot] & (bit - 1ull));
     atomicMin(&winner[rank], tri);
   }
+  __device__ __forceinline__ void flush() {}
 };
 
-template <typename Act>
-__global__ __launch_bounds__(256) void k_vox_small(VoxParams P, Act act, uint32_t* large_tri,
-                                                   uint32_t* large_cols, ull* ctr,
-                                                   int collect) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
-  const uint32_t t = P.list ? P.list[i] : i;
-  Setup S;
-  setup_tri(P.tris + (uint64_t)t * P.stride, P, S);
-  const uint64_t ncols = tri_columns(S);
-  const uint64_t work = S.slab ? ncols : ncols * (uint64_t)(S.g1[S.w] - S.g0[S.w] + 1);
-  if (work > (uint64_t)P.small_cols) {
-    if (collect) {
-      const ull idx = atomicAdd(&ctr[kCtrLargeCount], 1ull);
-      large_tri[idx] = t;
-      large_cols[idx] = (uint32_t)ncols;
-    }
-    return;
-  }
-  const int w = S.w;
-  const int ua = (w == 0) ? 1 : 0;
-  const int va = (w == 2) ? 1 : 2;
-  for (int b = S.g0[va]; b <= S.g1[va]; b++)
-    for (int a = S.g0[ua]; a <= S.g1[ua]; a++) walk_column(S, P, a, b, t, act);
-}
+// Small tier: a block of 256 threads takes 256 consecutive triangles of the partition
+// list. Each thread builds its triangle's walk constants into LDS; a block scan lays out
+// the (triangle, column) items; then the threads share the items, one column each, so
+// the work per lane is a few voxel tests regardless of triangle size. Triangles with more
+// than small_cols columns (or non-slab triangles with more than small_cols * 8 tests) go
+// to the column-parallel tier.
+constexpr int kTile = 256;
 
 template <typename Act>
-__global__ __launch_bounds__(256) void k_vox_large(VoxParams P, Act act,
+__global__ __launch_bounds__(kTile) void k_vox_tile(VoxParams P, Act act0, uint32_t* large_tri,
+                                                    uint32_t* large_cols, ull* ctr, int collect) {
+  __shared__ uint32_t sk[kWalkWords][kTile];
+  __shared__ uint32_t soff[kTile + 1];
+  __shared__ uint32_t wsum[kTile / kWave];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t i = blockIdx.x * kTile + tid;
+  uint32_t items = 0;
+  if (i < P.n) {
+    const uint32_t t = P.list ? P.list[i] : i;
+    Setup S;
+    setup_tri(P.tris + (uint64_t)t * P.stride, P, S);
+    Walk K;
+    make_walk(S, P, t, K);
+    const uint64_t ncols = K.columns();
+    const uint64_t lim = (uint64_t)P.small_cols;
+    if (K.slab() ? (ncols > lim) : (K.work() > lim * 8)) {
+      if (collect) {
+        const ull idx = atomicAdd(&ctr[kCtrLargeCount], 1ull);
+        large_tri[idx] = t;
+        large_cols[idx] = (uint32_t)ncols;
+      }
+    } else {
+      items = (uint32_t)ncols;
+#pragma unroll
+      for (int f = 0; f < kWalkF; f++) sk[f][tid] = __float_as_uint(K.f[f]);
+#pragma unroll
+      for (int f = 0; f < kWalkI; f++) sk[kWalkF + f][tid] = (uint32_t)K.i[f];
+    }
+  }
+  uint32_t total;
+  const uint32_t inc = block_incl_scan<uint32_t, kTile>(items, wsum, total);
+  soff[tid] = inc - items;
+  if (tid == kTile - 1) soff[kTile] = total;
+  __syncthreads();
+  Act act = act0;
+  for (uint32_t it = tid; it < total; it += kTile) {
+    // owner: last j with soff[j] <= it (triangles without items have empty ranges)
+    uint32_t lo = 0, hi = kTile - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (soff[mid] <= it)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    Walk K;
+#pragma unroll
+    for (int f = 0; f < kWalkF; f++) K.f[f] = __uint_as_float(sk[f][lo]);
+#pragma unroll
+    for (int f = 0; f < kWalkI; f++) K.i[f] = (int)sk[kWalkF + f][lo];
+    const uint32_t local = it - soff[lo];
+    const uint32_t eu = (uint32_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
+    const int a = K.i[I_G0U] + (int)(local % eu);
+    const int b = K.i[I_G0V] + (int)(local / eu);
+    walk_column(K, P, a, b, act);
+  }
+  act.flush();
+}
+
+// Column-parallel tier: one thread per column of the deferred triangles; the counts live
+// on the device (ctr[kCtrLargeCount] triangles, ctr[kCtrCols] columns), the grid strides.
+template <typename Act>
+__global__ __launch_bounds__(256) void k_vox_large(VoxParams P, Act act0,
                                                    const uint32_t* __restrict__ large_tri,
                                                    const uint64_t* __restrict__ large_off,
-                                                   uint32_t n_large, uint64_t total) {
+                                                   const ull* __restrict__ ctr) {
+  const uint32_t n_large = (uint32_t)ctr[kCtrLargeCount];
+  const uint64_t total = ctr[kCtrCols];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  Act act = act0;
   for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
     // last j with large_off[j] <= g
     uint32_t lo = 0, hi = n_large - 1;
@@ -326,14 +515,14 @@ __global__ __launch_bounds__(256) void k_vox_large(VoxParams P, Act act,
     const uint64_t local = g - large_off[lo];
     Setup S;
     setup_tri(P.tris + (uint64_t)t * P.stride, P, S);
-    const int w = S.w;
-    const int ua = (w == 0) ? 1 : 0;
-    const int va = (w == 2) ? 1 : 2;
-    const uint64_t eu = (uint64_t)(S.g1[ua] - S.g0[ua] + 1);
-    const int a = S.g0[ua] + (int)(local % eu);
-    const int b = S.g0[va] + (int)(local / eu);
-    walk_column(S, P, a, b, t, act);
+    Walk K;
+    make_walk(S, P, t, K);
+    const uint64_t eu = (uint64_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
+    const int a = K.i[I_G0U] + (int)(local % eu);
+    const int b = K.i[I_G0V] + (int)(local / eu);
+    walk_column(K, P, a, b, act);
   }
+  act.flush();
 }
 
 // ---------------------------------------------------------------- compaction
@@ -351,27 +540,36 @@ struct OccStore {
   }
 };
 
-// one wave per occupied brick
+// Occupied-brick kernels: one wave per brick (lane = word), striding over the
+// ctr[kCtrNumOcc] bricks.
+__device__ __forceinline__ uint64_t wave_global() {
+  return ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+}
+__device__ __forceinline__ uint64_t waves_total() {
+  return (uint64_t)gridDim.x * blockDim.x / kWave;
+}
+
 __global__ __launch_bounds__(256) void k_brick_count(const uint32_t* __restrict__ occ,
-                                                     uint32_t n_occ,
+                                                     const ull* __restrict__ ctr,
                                                      const uint32_t* __restrict__ table,
                                                      const ull* __restrict__ pool,
                                                      uint64_t* __restrict__ bcount) {
-  const uint32_t wv = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  if (wv >= n_occ) return;
-  const uint32_t s = table[occ[wv]];
-  const uint32_t c = (uint32_t)__popcll(pool[(uint64_t)(s - 1) * kBrickWords + lane_id()]);
-  const uint32_t tot = wave_incl_scan<uint32_t>(c);
-  if (lane_id() == kWave - 1) bcount[wv] = tot;
+  const uint64_t n_occ = ctr[kCtrNumOcc];
+  for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
+    const uint32_t s = table[occ[wv]];
+    const uint32_t c = (uint32_t)__popcll(pool[(uint64_t)(s - 1) * kBrickWords + lane_id()]);
+    const uint32_t tot = wave_incl_scan<uint32_t>(c);
+    if (lane_id() == kWave - 1) bcount[wv] = tot;
+  }
 }
 
 struct CountStore {
   uint64_t* off;
   ull* total;
-  uint64_t n;
+  const ull* n_dev;  // actual element count (device)
   __device__ void operator()(uint64_t i, uint64_t ex, uint64_t v) const {
     off[i] = ex;
-    if (i == n - 1) *total = (ull)(ex + v);
+    if (i == *n_dev - 1) *total = (ull)(ex + v);
   }
 };
 struct ColLoad {
@@ -380,40 +578,43 @@ struct ColLoad {
 };
 
 __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__ occ,
-                                                    uint32_t n_occ,
+                                                    const ull* __restrict__ ctr,
                                                     const uint32_t* __restrict__ table,
                                                     const ull* __restrict__ pool,
                                                     const uint64_t* __restrict__ boff,
                                                     uint64_t mstart, uint64_t* __restrict__ codes,
                                                     uint32_t* __restrict__ wprefix) {
-  const uint32_t wv = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  if (wv >= n_occ) return;
-  const uint32_t b = occ[wv];
-  const uint32_t s = table[b];
-  const uint64_t slot = (uint64_t)(s - 1) * kBrickWords + lane_id();
-  ull word = pool[slot];
-  const uint32_t c = (uint32_t)__popcll(word);
-  const uint32_t inc = wave_incl_scan<uint32_t>(c);
-  uint64_t pos = boff[wv] + (inc - c);
-  if (wprefix) wprefix[slot] = (uint32_t)pos;
-  const uint64_t cb = mstart + ((uint64_t)b << kBrickBits) + ((uint64_t)lane_id() << 6);
-  while (word) {
-    const int bit = __ffsll(word) - 1;
-    codes[pos++] = cb + (uint64_t)bit;
-    word &= word - 1;
+  const uint64_t n_occ = ctr[kCtrNumOcc];
+  for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
+    const uint32_t b = occ[wv];
+    const uint32_t s = table[b];
+    const uint64_t slot = (uint64_t)(s - 1) * kBrickWords + lane_id();
+    ull word = pool[slot];
+    const uint32_t c = (uint32_t)__popcll(word);
+    const uint32_t inc = wave_incl_scan<uint32_t>(c);
+    uint64_t pos = boff[wv] + (inc - c);
+    if (wprefix) wprefix[slot] = (uint32_t)pos;
+    const uint64_t cb = mstart + ((uint64_t)b << kBrickBits) + ((uint64_t)lane_id() << 6);
+    while (word) {
+      const int bit = __ffsll(word) - 1;
+      codes[pos++] = cb + (uint64_t)bit;
+      word &= word - 1;
+    }
   }
 }
 
 // restore the all-zero state of the table and the pool for the occupied bricks
 __global__ __launch_bounds__(256) void k_brick_clear(const uint32_t* __restrict__ occ,
-                                                     uint32_t n_occ, uint32_t* __restrict__ table,
+                                                     const ull* __restrict__ ctr,
+                                                     uint32_t* __restrict__ table,
                                                      ull* __restrict__ pool) {
-  const uint32_t wv = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  if (wv >= n_occ) return;
-  const uint32_t b = occ[wv];
-  const uint32_t s = table[b];
-  pool[(uint64_t)(s - 1) * kBrickWords + lane_id()] = 0ull;
-  if (lane_id() == 0) table[b] = 0u;
+  const uint64_t n_occ = ctr[kCtrNumOcc];
+  for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
+    const uint32_t b = occ[wv];
+    const uint32_t s = table[b];
+    pool[(uint64_t)(s - 1) * kBrickWords + lane_id()] = 0ull;
+    if (lane_id() == 0) table[b] = 0u;
+  }
 }
 
 // ---------------------------------------------------------------- payload records
@@ -428,7 +629,7 @@ __global__ __launch_bounds__(256) void k_payload_records(
   const uint32_t t = winner[i];
   float col0, col1, col2, nr0, nr1, nr2;
   if (t == 0xFFFFFFFFu) {  // cannot happen: both passes evaluate the same predicates
-    atomicAdd(&ctr[kCtrOverflow], 2ull);
+    atomicAdd(&ctr[kCtrError], 1ull);
     col0 = col1 = col2 = nr0 = nr1 = nr2 = 0.0f;
   } else {
     const float* p = tris + (uint64_t)t * 21;
@@ -698,6 +899,10 @@ hipError_t voxelize_partition(Context& c, uint64_t part, uint64_t morton_start,
   P.unit = unitlength;
   P.unit_div = 1.0f / unitlength;  // voxelizer.cpp:164, IEEE on the host as on the reference
   P.coord_limit = 4.0f * (float)gridsize * unitlength;
+  // The reference's FP32 plane test (t + d1) * (t + d2) differs from its exact value by
+  // less than ~3e-6 * coord_limit; along the column axis (|n_w| >= 1/2 here) that is
+  // below 2.1e-5 * G voxels. The margin is 0.5 + 4e-5 * G voxels, at least twice that.
+  P.slab_margin = 0.5 + 4e-5 * (double)gridsize;
   P.small_cols = c.tune.small_cols;
 
   const uint64_t span = morton_end - morton_start;
@@ -722,38 +927,52 @@ hipError_t voxelize_partition(Context& c, uint64_t part, uint64_t morton_start,
   SVO_TRY(ensure(c.large_tri, n * 4));
   SVO_TRY(ensure(c.large_cols, n * 4));
   SVO_TRY(ensure(c.large_off, n * 8 + 8));
+  SVO_TRY(ensure(c.occ_list, nbricks * 4));
+  SVO_TRY(ensure_scan_scratch(c, std::max<uint64_t>(n, nbricks), 8));
   ull* ctr = c.counters.as<ull>();
   ull hc[kNumCounters];
   const unsigned nblk = (unsigned)((n + 255) / 256);
-  uint64_t n_large = 0, total_cols = 0;
+  const unsigned large_grid = 2048;
+  auto brick_grid = [&](uint64_t bricks) {
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((bricks * kWave + 255) / 256, 8192));
+  };
 
+  // ---- phase A: mark, compact counts; one synchronization at its end
   for (int attempt = 0;; attempt++) {
-    SVO_TRY(fill_u64((uint64_t*)ctr, 0, 8, c.stream));
+    const uint64_t bc_bound = std::min<uint64_t>(nbricks, c.pool_cap);
+    SVO_TRY(ensure(c.bcount, bc_bound * 8 + 8));
+    SVO_TRY(fill_u64((uint64_t*)ctr, 0, 7, c.stream));  // counters 0..6
     MarkAction mark{c.table.as<uint32_t>(), c.pool.as<ull>(), ctr, c.pool_cap, morton_start};
     ktimer_begin(c, "vox_small_mark");
-    hipLaunchKernelGGL((k_vox_small<MarkAction>), dim3(nblk), dim3(256), 0, c.stream, P, mark,
+    hipLaunchKernelGGL((k_vox_tile<MarkAction>), dim3(nblk), dim3(kTile), 0, c.stream, P, mark,
                        c.large_tri.as<uint32_t>(), c.large_cols.as<uint32_t>(), ctr, 1);
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
+    SVO_TRY((scan_exclusive<uint64_t>(
+        ColLoad{c.large_cols.as<uint32_t>()},
+        CountStore{c.large_off.as<uint64_t>(), ctr + kCtrCols, ctr + kCtrLargeCount}, n,
+        c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrLargeCount)));
+    ktimer_begin(c, "vox_large_mark");
+    hipLaunchKernelGGL((k_vox_large<MarkAction>), dim3(large_grid), dim3(256), 0, c.stream, P,
+                       mark, c.large_tri.as<uint32_t>(), c.large_off.as<uint64_t>(), ctr);
+    ktimer_end(c);
+    SVO_TRY(hipGetLastError());
+    // occupied bricks in Morton order, their voxel counts and offsets
+    SVO_TRY((scan_exclusive<uint32_t>(OccLoad{c.table.as<uint32_t>()},
+                                      OccStore{c.occ_list.as<uint32_t>(), ctr, nbricks}, nbricks,
+                                      c.scan_scratch.as<uint32_t>(), c.stream)));
+    ktimer_begin(c, "brick_count");
+    hipLaunchKernelGGL(k_brick_count, dim3(brick_grid(bc_bound)), dim3(256), 0, c.stream,
+                       c.occ_list.as<uint32_t>(), ctr, c.table.as<uint32_t>(), c.pool.as<ull>(),
+                       c.bcount.as<uint64_t>());
+    ktimer_end(c);
+    SVO_TRY(hipGetLastError());
+    SVO_TRY((scan_exclusive<uint64_t>(
+        ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
+        CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox, ctr + kCtrNumOcc}, bc_bound,
+        c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrNumOcc)));
     SVO_TRY(read_counters(c, hc));
-    n_large = hc[kCtrLargeCount];
-    total_cols = 0;
-    if (n_large) {
-      SVO_TRY(ensure_scan_scratch(c, n_large, 8));
-      CountStore st{c.large_off.as<uint64_t>(), ctr + kCtrCols, n_large};
-      SVO_TRY((scan_exclusive<uint64_t>(ColLoad{c.large_cols.as<uint32_t>()}, st, n_large,
-                                        c.scan_scratch.as<uint64_t>(), c.stream)));
-      SVO_TRY(read_counters(c, hc));
-      total_cols = hc[kCtrCols];
-      const uint64_t blocks = std::min<uint64_t>((total_cols + 255) / 256, 65536);
-      ktimer_begin(c, "vox_large_mark");
-      hipLaunchKernelGGL((k_vox_large<MarkAction>), dim3((unsigned)blocks), dim3(256), 0,
-                         c.stream, P, mark, c.large_tri.as<uint32_t>(),
-                         c.large_off.as<uint64_t>(), (uint32_t)n_large, total_cols);
-      ktimer_end(c);
-      SVO_TRY(hipGetLastError());
-      SVO_TRY(read_counters(c, hc));
-    }
+    ktimer_flush(c);
     if (!hc[kCtrOverflow]) break;
     // pool exhausted: clear everything, grow, redo
     const uint64_t want = std::max<uint64_t>(c.pool_cap * 2, hc[kCtrPoolCount] + 4096);
@@ -767,36 +986,17 @@ hipError_t voxelize_partition(Context& c, uint64_t part, uint64_t morton_start,
       return hipErrorOutOfMemory;
     }
   }
-
-  // compaction: occupied bricks in Morton order -> sorted unique codes
-  SVO_TRY(ensure(c.occ_list, nbricks * 4));
-  SVO_TRY(ensure_scan_scratch(c, nbricks, 4));
-  SVO_TRY((scan_exclusive<uint32_t>(OccLoad{c.table.as<uint32_t>()},
-                                    OccStore{c.occ_list.as<uint32_t>(), ctr, nbricks}, nbricks,
-                                    c.scan_scratch.as<uint32_t>(), c.stream)));
-  SVO_TRY(read_counters(c, hc));
   const uint64_t n_occ = hc[kCtrNumOcc];
-  uint64_t nvox = 0;
+  const uint64_t nvox = hc[kCtrNumVox];
+  const uint64_t n_large = hc[kCtrLargeCount];
+
+  // ---- phase B: emit sorted codes, payload, clear; no synchronization
   if (n_occ) {
-    SVO_TRY(ensure(c.bcount, n_occ * 8));
-    SVO_TRY(ensure(c.wprefix, c.pool_cap * kBrickWords * 4));
-    const unsigned wblk = (unsigned)((n_occ * kWave + 255) / 256);
-    ktimer_begin(c, "brick_count");
-    hipLaunchKernelGGL(k_brick_count, dim3(wblk), dim3(256), 0, c.stream,
-                       c.occ_list.as<uint32_t>(), (uint32_t)n_occ, c.table.as<uint32_t>(),
-                       c.pool.as<ull>(), c.bcount.as<uint64_t>());
-    ktimer_end(c);
-    SVO_TRY(hipGetLastError());
-    SVO_TRY(ensure_scan_scratch(c, n_occ, 8));
-    SVO_TRY((scan_exclusive<uint64_t>(ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
-                                      CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox, n_occ}, n_occ,
-                                      c.scan_scratch.as<uint64_t>(), c.stream)));
-    SVO_TRY(read_counters(c, hc));
-    nvox = hc[kCtrNumVox];
+    if (c.payload) SVO_TRY(ensure(c.wprefix, c.pool_cap * kBrickWords * 4));
     SVO_TRY(ensure_keep(c.codes, (c.n_vox + nvox) * 8, c.n_vox * 8, c.stream));
     ktimer_begin(c, "brick_emit");
-    hipLaunchKernelGGL(k_brick_emit, dim3(wblk), dim3(256), 0, c.stream, c.occ_list.as<uint32_t>(),
-                       (uint32_t)n_occ, c.table.as<uint32_t>(), c.pool.as<ull>(),
+    hipLaunchKernelGGL(k_brick_emit, dim3(brick_grid(n_occ)), dim3(256), 0, c.stream,
+                       c.occ_list.as<uint32_t>(), ctr, c.table.as<uint32_t>(), c.pool.as<ull>(),
                        c.bcount.as<uint64_t>(), morton_start, c.codes.as<uint64_t>() + c.n_vox,
                        c.payload ? c.wprefix.as<uint32_t>() : (uint32_t*)nullptr);
     ktimer_end(c);
@@ -812,16 +1012,15 @@ hipError_t voxelize_partition(Context& c, uint64_t part, uint64_t morton_start,
       WinnerAction win{c.table.as<uint32_t>(), c.pool.as<ull>(), c.wprefix.as<uint32_t>(),
                        c.winner.as<uint32_t>(), ctr, morton_start};
       ktimer_begin(c, "vox_small_winner");
-      hipLaunchKernelGGL((k_vox_small<WinnerAction>), dim3(nblk), dim3(256), 0, c.stream, P,
+      hipLaunchKernelGGL((k_vox_tile<WinnerAction>), dim3(nblk), dim3(kTile), 0, c.stream, P,
                          win, c.large_tri.as<uint32_t>(), c.large_cols.as<uint32_t>(), ctr, 0);
       ktimer_end(c);
       SVO_TRY(hipGetLastError());
       if (n_large) {
-        const uint64_t blocks = std::min<uint64_t>((total_cols + 255) / 256, 65536);
         ktimer_begin(c, "vox_large_winner");
-        hipLaunchKernelGGL((k_vox_large<WinnerAction>), dim3((unsigned)blocks), dim3(256), 0,
+        hipLaunchKernelGGL((k_vox_large<WinnerAction>), dim3(large_grid), dim3(256), 0,
                            c.stream, P, win, c.large_tri.as<uint32_t>(),
-                           c.large_off.as<uint64_t>(), (uint32_t)n_large, total_cols);
+                           c.large_off.as<uint64_t>(), ctr);
         ktimer_end(c);
         SVO_TRY(hipGetLastError());
       }
@@ -834,22 +1033,16 @@ hipError_t voxelize_partition(Context& c, uint64_t part, uint64_t morton_start,
                          (float*)((char*)c.records.p + (1 + c.n_vox) * 32), ctr);
       ktimer_end(c);
       SVO_TRY(hipGetLastError());
-      SVO_TRY(read_counters(c, hc));
-      if (hc[kCtrOverflow]) {
-        set_error("payload pass found a voxel without a covering triangle");
-        return hipErrorUnknown;
-      }
     }
     ktimer_begin(c, "brick_clear");
-    hipLaunchKernelGGL(k_brick_clear, dim3(wblk), dim3(256), 0, c.stream,
-                       c.occ_list.as<uint32_t>(), (uint32_t)n_occ, c.table.as<uint32_t>(),
-                       c.pool.as<ull>());
+    hipLaunchKernelGGL(k_brick_clear, dim3(brick_grid(n_occ)), dim3(256), 0, c.stream,
+                       c.occ_list.as<uint32_t>(), ctr, c.table.as<uint32_t>(), c.pool.as<ull>());
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
   }
   c.n_vox += nvox;
   *nfilled = nvox;
-  return hipStreamSynchronize(c.stream);
+  return hipSuccess;
 }
 
 }  // namespace svo
