@@ -98,9 +98,10 @@ def main():
     n_data = int(info.n_data)
 
     # dominant kernel: the instrumented kernel with the most device time
-    names = ["part_hitbox", "vox_small_mark", "vox_large_mark", "brick_count", "brick_emit",
+    names = ["part_hitbox", "vox_small_touch", "vox_large_touch", "vox_small_mark",
+             "vox_large_mark", "brick_count", "brick_emit",
              "vox_small_winner", "vox_large_winner", "payload_records", "brick_clear",
-             "emit_children"]
+             "emit_children", "emit_leaves"]
     ktimes = {}
     for nm in names:
         n, ms = ctx.kernel_timing(nm)

@@ -81,9 +81,11 @@ struct ParentStore {
   Arena A;
   LevelMeta M;
   const uint64_t* leaves;
+  uint32_t* leaf_parent;  // leaf level only: parent index of every leaf
   int l, D;
   __device__ void operator()(uint64_t i, uint32_t ex, uint32_t head) const {
     const bool leaf = (l + 1 == D);
+    if (leaf) leaf_parent[i] = ex + head - 1;
     if (!leaf && M.off[l + 1] + M.n[l + 1] > A.cap) {
       M.flag[0] = 1;
       return;
@@ -139,20 +141,38 @@ __global__ __launch_bounds__(256) void k_emit_children(Arena A, LevelMeta M, int
     M.flag[1] = 1;
     return;
   }
+  if (child_is_leaf) return;  // leaves are written by k_emit_leaves
   uint64_t run = st;
   for (uint32_t r = 0; r < cnt; r++) {
     const uint64_t c = (uint64_t)f + r;
-    if (child_is_leaf) {
-      // OctreeBuilder.cpp:137-139 (binary: data 1) / :157-159 (payload: its data index)
-      put_node(nodes, cb + r, payload ? (1ull + c) : 1ull, 0ull, ~0ull);
-    } else {
-      const uint64_t sz = A.size[co + c];
-      const uint32_t cm = A.mask[co + c];
-      A.start[co + c] = run;
-      put_node(nodes, cb + r, 0ull, run + sz - 1 - (uint32_t)__popc(cm), offsets_of(cm));
-      run += sz - 1;
-    }
+    const uint64_t sz = A.size[co + c];
+    const uint32_t cm = A.mask[co + c];
+    A.start[co + c] = run;
+    put_node(nodes, cb + r, 0ull, run + sz - 1 - (uint32_t)__popc(cm), offsets_of(cm));
+    run += sz - 1;
   }
+}
+
+// One thread per leaf: position = children_base(parent) + rank among its siblings.
+// Consecutive leaves land in consecutive records, so the stores coalesce.
+// OctreeBuilder.cpp:137-139 (binary: data 1) / :157-159 (payload: its data index).
+__global__ __launch_bounds__(256) void k_emit_leaves(Arena A, LevelMeta M, int D, int payload,
+                                                     uint64_t node_cap,
+                                                     const uint32_t* __restrict__ leaf_parent,
+                                                     uint64_t* __restrict__ nodes) {
+  if (M.flag[0]) return;
+  const uint64_t nl = M.n[D];
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nl) return;
+  const uint64_t po = M.off[D - 1] + leaf_parent[i];
+  const uint32_t cnt = (uint32_t)__popc((uint32_t)A.mask[po]);
+  const uint64_t cb = A.start[po] + A.size[po] - 1 - cnt;
+  const uint64_t pos = cb + (i - A.first[po]);
+  if (pos >= node_cap) {
+    M.flag[1] = 1;
+    return;
+  }
+  put_node(nodes, pos, payload ? (1ull + i) : 1ull, 0ull, ~0ull);
 }
 
 __global__ void k_emit_root(Arena A, LevelMeta M, uint64_t node_cap, uint64_t* nodes) {
@@ -227,6 +247,7 @@ hipError_t build_octree(Context& c, uint64_t gridsize, int levels) {
     SVO_TRY(ensure(c.lvl_first, cap * 4));
     SVO_TRY(ensure(c.lvl_start, cap * 8));
     SVO_TRY(ensure(c.lvl_aux, (2 * 64 + 8) * 8));
+    SVO_TRY(ensure(c.lvl_leafp, V * 4));
     const uint64_t node_cap = cap + V + 1;
     SVO_TRY(ensure(c.nodes, node_cap * 24));
     SVO_TRY(ensure(c.scan_scratch, scan_scratch_elems(V) * 4 + 256));
@@ -241,7 +262,7 @@ hipError_t build_octree(Context& c, uint64_t gridsize, int levels) {
     // ---- bottom-up: levels D-1 .. 0
     for (int l = D - 1; l >= 0; l--) {
       HeadLoad ld{A, M, c.codes.as<uint64_t>(), l, D};
-      ParentStore st{A, M, c.codes.as<uint64_t>(), l, D};
+      ParentStore st{A, M, c.codes.as<uint64_t>(), c.lvl_leafp.as<uint32_t>(), l, D};
       SVO_TRY((scan_exclusive<uint32_t>(ld, st, bound[l + 1], c.scan_scratch.as<uint32_t>(),
                                         c.stream, M.n + (l + 1))));
     }
@@ -249,13 +270,19 @@ hipError_t build_octree(Context& c, uint64_t gridsize, int levels) {
     hipLaunchKernelGGL(k_emit_root, dim3(1), dim3(1), 0, c.stream, A, M, node_cap,
                        c.nodes.as<uint64_t>());
     SVO_TRY(hipGetLastError());
-    for (int l = 0; l < D; l++) {
+    for (int l = 0; l + 1 < D; l++) {
       ktimer_begin(c, "emit_children");
       hipLaunchKernelGGL(k_emit_children, dim3((unsigned)((bound[l] + 255) / 256)), dim3(256), 0,
                          c.stream, A, M, l, D, c.payload, node_cap, c.nodes.as<uint64_t>());
       ktimer_end(c);
       SVO_TRY(hipGetLastError());
     }
+    ktimer_begin(c, "emit_leaves");
+    hipLaunchKernelGGL(k_emit_leaves, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, c.stream, A,
+                       M, D, c.payload, node_cap, c.lvl_leafp.as<uint32_t>(),
+                       c.nodes.as<uint64_t>());
+    ktimer_end(c);
+    SVO_TRY(hipGetLastError());
     unsigned long long flags[3] = {0, 0, 0};
     SVO_TRY(hipMemcpyAsync(flags, M.flag, 24, hipMemcpyDeviceToHost, c.stream));
     SVO_TRY(hipStreamSynchronize(c.stream));

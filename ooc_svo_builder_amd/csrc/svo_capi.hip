@@ -213,11 +213,11 @@ void svo_ctx_destroy(svo_ctx* ctx) {
   Context& c = ctx->c;
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
-  DevBuf* bufs[] = {&c.tris, &c.hitbox, &c.part_lists, &c.table, &c.pool, &c.counters,
+  DevBuf* bufs[] = {&c.tris, &c.hitbox, &c.part_lists, &c.table, &c.pool, &c.pool8, &c.counters,
                     &c.large_tri, &c.large_cols, &c.large_off, &c.occ_list, &c.bcount,
                     &c.wprefix, &c.winner, &c.scan_scratch, &c.codes, &c.records, &c.nodes,
                     &c.data, &c.lvl_code, &c.lvl_size, &c.lvl_mask, &c.lvl_first,
-                    &c.lvl_start, &c.lvl_aux};
+                    &c.lvl_start, &c.lvl_aux, &c.lvl_leafp};
   for (DevBuf* b : bufs) release(*b);
   ktimer_flush(c);
   for (hipEvent_t e : c.evpool) (void)hipEventDestroy(e);

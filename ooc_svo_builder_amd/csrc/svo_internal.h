@@ -11,9 +11,11 @@
 namespace svo {
 
 // Sparse occupancy grid: the partition's Morton range is cut into aligned bricks of
-// 2^kBrickBits voxels (16^3). A dense table maps brick -> pool slot (0 = empty); a
-// brick's bits live in the pool as 64 words, in Morton order.
+// 2^kBrickBits voxels (16^3). A dense table maps brick -> pool slot (0 = empty). A slot
+// holds the brick's voxels as bytes (marked by plain stores) and, after compaction, as
+// 64 bit words, both in Morton order.
 constexpr int kBrickBits = 12;
+constexpr int kBrickVox = 1 << kBrickBits;
 constexpr int kBrickWords = 1 << (kBrickBits - 6);
 
 // Device counters block (one 256-byte line of u64s).
@@ -76,7 +78,7 @@ struct Context {
   uint64_t n_parts = 0;
 
   // voxelizer scratch (kept zeroed between partitions: table, pool)
-  DevBuf table, pool;
+  DevBuf table, pool, pool8;  // pool: bit words per slot; pool8: bytes per slot
   uint64_t table_entries = 0;
   uint64_t pool_cap = 0;  // bricks
   DevBuf counters;
@@ -94,7 +96,7 @@ struct Context {
   DevBuf nodes, data;
   uint64_t n_nodes = 0, n_data = 0;
   int hazard = 0;
-  DevBuf lvl_code, lvl_size, lvl_mask, lvl_first, lvl_start, lvl_aux;
+  DevBuf lvl_code, lvl_size, lvl_mask, lvl_first, lvl_start, lvl_aux, lvl_leafp;
 
   StageTimes times;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

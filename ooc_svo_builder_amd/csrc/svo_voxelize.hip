@@ -252,18 +252,16 @@ __device__ __forceinline__ void make_walk(const Setup& S, const VoxParams& P, ui
   K.i[I_TRI] = (int)tri;
 }
 
-// Test the voxels of column (a, b) and hand every accepted one to act.add(code, tri).
-template <typename Act>
-__device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, int a, int b,
-                                            Act& act) {
-  const float u = P.unit;
-  const float pa = (float)a * u;
-  const float pb = (float)b * u;
+// The voxels of column (a, b) that can pass: false when the column's projection test
+// fails, else [lo, hi] along w (the bbox, narrowed to the plane slab). Both passes of
+// the voxelizer call this with the same inputs, so they see the same ranges.
+__device__ __forceinline__ bool column_range(const Walk& K, float pa, float pb, int a, int b,
+                                             int& lo, int& hi) {
 #pragma unroll
   for (int k = 0; k < 3; k++)
-    if ((K.f[F_H0 + k] * pa + K.f[F_H1 + k] * pb) + K.f[F_HD + k] < 0.0f) return;
-  const int w = K.w();
-  int lo = K.i[I_G0W], hi = K.i[I_G1W];
+    if ((K.f[F_H0 + k] * pa + K.f[F_H1 + k] * pb) + K.f[F_HD + k] < 0.0f) return false;
+  lo = K.i[I_G0W];
+  hi = K.i[I_G1W];
   if (K.slab()) {
     const float base = K.f[F_KA] * (float)a + K.f[F_KB] * (float)b;
     const float l = fminf(fmaxf(base + K.f[F_ALO], (float)lo - 1.0f), (float)hi + 1.0f);
@@ -271,7 +269,34 @@ __device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, i
     lo = max(lo, (int)floorf(l));
     hi = min(hi, (int)floorf(h));
   }
-  if (lo > hi) return;
+  return lo <= hi;
+}
+
+// Pre-pass: flag every brick that a column's candidate voxels fall in (a superset of the
+// bricks that will hold marked voxels).
+template <typename Act>
+__device__ __forceinline__ void touch_column(const Walk& K, const VoxParams& P, int a, int b,
+                                             Act& act) {
+  const float u = P.unit;
+  int lo, hi;
+  if (!column_range(K, (float)a * u, (float)b * u, a, b, lo, hi)) return;
+  const int w = K.w();
+  const int ua = (w == 0) ? 1 : 0;
+  const int va = (w == 2) ? 1 : 2;
+  const uint64_t bbase = (spread3((uint32_t)(a >> 4)) << ua) | (spread3((uint32_t)(b >> 4)) << va);
+  for (int cb = lo >> 4; cb <= (hi >> 4); cb++) act.touch(bbase | (spread3((uint32_t)cb) << w));
+}
+
+// Test the voxels of column (a, b) and hand every accepted one to act.add(code, tri).
+template <typename Act>
+__device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, int a, int b,
+                                            Act& act) {
+  const float u = P.unit;
+  const float pa = (float)a * u;
+  const float pb = (float)b * u;
+  int lo, hi;
+  if (!column_range(K, pa, pb, a, b, lo, hi)) return;
+  const int w = K.w();
   const float tua = K.f[F_NUA] * pa, tva = K.f[F_NVA] * pb;
   const float Ka = (w == 2) ? (tua + tva) : tua;
   const float Kb = (w == 2) ? 0.0f : tva;
@@ -302,102 +327,57 @@ __device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, i
   }
 }
 
-// Set voxels' bits in the sparse brick grid. A thread collects its bits per 64-bit word
-// in a small register buffer and sets each word with one atomicOr when the buffer is
-// full or the thread is done (mostly at a point where the whole wave has converged).
-// The brick table lookup is cached per thread. New bricks are taken from the pool with
-// one atomic per distinct brick per wave; a lost install race leaves a zeroed, unused
-// slot behind. An exhausted pool raises the overflow counter and the pass is redone
-// with a larger pool.
-struct MarkAction {
-  static constexpr int K = 4;
+// Brick allocation without atomics: the touch pass stores 1 into the table entry of
+// every brick a column's candidate voxels fall in; a scan over the table then numbers the
+// flagged bricks in Morton order (table entry = slot + 1) and lists them.
+struct TouchAction {
+  static constexpr bool kTouch = true;
   uint32_t* table;
-  ull* pool;
+  uint64_t brick0;  // first brick of the partition (morton_start >> kBrickBits)
+  __device__ __forceinline__ void touch(uint64_t brick_code) { table[brick_code - brick0] = 1u; }
+  __device__ __forceinline__ void add(uint64_t, uint32_t) {}
+  __device__ __forceinline__ void flush() {}
+};
+
+// Mark voxels: a brick slot holds one byte per voxel (Morton order) and a mark is a plain
+// byte store of 1. Concurrent marks of one voxel all store the same value, so no atomic
+// read-modify-write is needed however many triangles cover it (about three per voxel on
+// a terrain). The brick's slot comes from the table (assigned by the touch pass and the
+// table scan), cached per thread.
+struct MarkAction {
+  static constexpr bool kTouch = false;
+  const uint32_t* table;
+  uint8_t* pool8;
   ull* ctr;
   uint64_t pool_cap;
   uint64_t mstart;
-  uint64_t wid[K];
-  ull bits[K];
-  int nb = 0;
   uint32_t cbrick = 0xFFFFFFFFu, cslot = 0;
-
+  __device__ __forceinline__ void touch(uint64_t) {}
   __device__ __forceinline__ void add(uint64_t code, uint32_t) {
     const uint64_t rel = code - mstart;
-    const uint64_t wd = rel >> 6;
-    const ull bit = 1ull << (rel & 63);
-    bool hit = false;
-#pragma unroll
-    for (int k = 0; k < K; k++)
-      if (k < nb && wid[k] == wd) {
-        bits[k] |= bit;
-        hit = true;
-      }
-    if (hit) return;
-    if (nb == K) flush();
-#pragma unroll
-    for (int k = 0; k < K; k++)
-      if (k == nb) {
-        wid[k] = wd;
-        bits[k] = bit;
-      }
-    nb++;
-  }
-
-  __device__ __forceinline__ uint32_t slot_of(uint32_t b, bool have) {
-    uint32_t s = 1u;
-    if (have) {
-      if (b == cbrick)
-        s = cslot;
-      else
-        s = __hip_atomic_load(&table[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    bool need = have && (s == 0);
-    while (__any(need)) {
-      const ull m = __ballot(need);
-      const int leader = __ffsll(m) - 1;
-      const uint32_t lb = (uint32_t)__shfl((int)b, leader);
-      uint32_t got = 0;
-      if (lane_id() == leader) {
-        const ull mine = atomicAdd(&ctr[kCtrPoolCount], 1ull) + 1ull;
-        if (mine > pool_cap) {
-          atomicOr(&ctr[kCtrOverflow], 1ull);
-          got = 0xFFFFFFFFu;
-        } else {
-          const uint32_t prev = atomicCAS(&table[lb], 0u, (uint32_t)mine);
-          got = prev ? prev : (uint32_t)mine;
-        }
-      }
-      got = (uint32_t)__shfl((int)got, leader);
-      if (need && b == lb) {
-        s = got;
-        need = false;
-      }
-    }
-    if (have && s != 0xFFFFFFFFu) {
+    const uint32_t b = (uint32_t)(rel >> kBrickBits);
+    if (b != cbrick) {
       cbrick = b;
-      cslot = s;
+      cslot = table[b];
     }
-    return s;
-  }
-
-  __device__ void flush() {
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      const bool have = k < nb;
-      const uint32_t b = (uint32_t)(wid[k] >> (kBrickBits - 6));
-      const uint32_t s = slot_of(b, have);
-      if (have && s != 0xFFFFFFFFu)
-        atomicOr(&pool[(uint64_t)(s - 1) * kBrickWords + (uint32_t)(wid[k] & (kBrickWords - 1))],
-                 bits[k]);
+    const uint32_t s = cslot;
+    if (s == 0 || s > pool_cap) {
+      // s == 0 cannot happen (the touch pass covers every candidate voxel); s > pool_cap
+      // means the pool is too small and the pass is redone with a larger one
+      atomicAdd(&ctr[s == 0 ? kCtrError : kCtrOverflow], 1ull);
+      return;
     }
-    nb = 0;
+    pool8[(uint64_t)(s - 1) * kBrickVox + (rel & (kBrickVox - 1))] = 1;
   }
+  __device__ __forceinline__ void flush() {}
 };
 
 // Second payload pass: lowest triangle index per voxel rank.
 struct WinnerAction {
+  static constexpr bool kTouch = false;
+  __device__ __forceinline__ void touch(uint64_t) {}
   const uint32_t* table;
-  const ull* pool;
+  const ull* pool;  // bit form of the bricks, written by the compaction
   const uint32_t* wprefix;
   uint32_t* winner;
   ull* ctr;
@@ -435,6 +415,7 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(VoxParams P, Act act0, uint3
   __shared__ uint32_t sk[kWalkWords][kTile];
   __shared__ uint32_t soff[kTile + 1];
   __shared__ uint32_t wsum[kTile / kWave];
+  Act act = act0;
   const uint32_t tid = threadIdx.x;
   const uint32_t i = blockIdx.x * kTile + tid;
   uint32_t items = 0;
@@ -465,7 +446,6 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(VoxParams P, Act act0, uint3
   soff[tid] = inc - items;
   if (tid == kTile - 1) soff[kTile] = total;
   __syncthreads();
-  Act act = act0;
   for (uint32_t it = tid; it < total; it += kTile) {
     // owner: last j with soff[j] <= it (triangles without items have empty ranges)
     uint32_t lo = 0, hi = kTile - 1;
@@ -485,7 +465,10 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(VoxParams P, Act act0, uint3
     const uint32_t eu = (uint32_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
     const int a = K.i[I_G0U] + (int)(local % eu);
     const int b = K.i[I_G0V] + (int)(local / eu);
-    walk_column(K, P, a, b, act);
+    if constexpr (Act::kTouch)
+      touch_column(K, P, a, b, act);
+    else
+      walk_column(K, P, a, b, act);
   }
   act.flush();
 }
@@ -520,7 +503,10 @@ __global__ __launch_bounds__(256) void k_vox_large(VoxParams P, Act act0,
     const uint64_t eu = (uint64_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
     const int a = K.i[I_G0U] + (int)(local % eu);
     const int b = K.i[I_G0V] + (int)(local / eu);
-    walk_column(K, P, a, b, act);
+    if constexpr (Act::kTouch)
+      touch_column(K, P, a, b, act);
+    else
+      walk_column(K, P, a, b, act);
   }
   act.flush();
 }
@@ -532,10 +518,14 @@ struct OccLoad {
 };
 struct OccStore {
   uint32_t* occ;
+  uint32_t* table;  // flagged entries become slot + 1 (slots in Morton order)
   ull* ctr;
   uint64_t n;
   __device__ void operator()(uint64_t i, uint32_t ex, uint32_t v) const {
-    if (v) occ[ex] = (uint32_t)i;
+    if (v) {
+      occ[ex] = (uint32_t)i;
+      table[i] = ex + 1u;
+    }
     if (i == n - 1) ctr[kCtrNumOcc] = (ull)(ex + v);
   }
 };
@@ -549,15 +539,34 @@ __device__ __forceinline__ uint64_t waves_total() {
   return (uint64_t)gridDim.x * blockDim.x / kWave;
 }
 
+// Pack each occupied brick's bytes into 64 bit words (lane l = voxels 64l..64l+63) and
+// count its voxels.
 __global__ __launch_bounds__(256) void k_brick_count(const uint32_t* __restrict__ occ,
                                                      const ull* __restrict__ ctr,
+                                                     uint64_t pool_cap,
                                                      const uint32_t* __restrict__ table,
-                                                     const ull* __restrict__ pool,
+                                                     const uint8_t* __restrict__ pool8,
+                                                     ull* __restrict__ pool,
                                                      uint64_t* __restrict__ bcount) {
-  const uint64_t n_occ = ctr[kCtrNumOcc];
+  // more flagged bricks than slots: the pass is redone with a larger pool
+  const uint64_t n_occ = min((uint64_t)ctr[kCtrNumOcc], pool_cap);
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
     const uint32_t s = table[occ[wv]];
-    const uint32_t c = (uint32_t)__popcll(pool[(uint64_t)(s - 1) * kBrickWords + lane_id()]);
+    const uint4* src = (const uint4*)(pool8 + (uint64_t)(s - 1) * kBrickVox + lane_id() * 64);
+    ull m = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint4 v = src[q];
+      const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int d = 0; d < 4; d++) {
+        const uint32_t x = w4[d];  // four bytes, each 0 or 1
+        const uint32_t nib = (x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u);
+        m |= (ull)nib << (16 * q + 4 * d);
+      }
+    }
+    pool[(uint64_t)(s - 1) * kBrickWords + lane_id()] = m;
+    const uint32_t c = (uint32_t)__popcll(m);
     const uint32_t tot = wave_incl_scan<uint32_t>(c);
     if (lane_id() == kWave - 1) bcount[wv] = tot;
   }
@@ -603,16 +612,18 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
   }
 }
 
-// restore the all-zero state of the table and the pool for the occupied bricks
+// restore the all-zero state of the table and the byte pool for the occupied bricks
 __global__ __launch_bounds__(256) void k_brick_clear(const uint32_t* __restrict__ occ,
                                                      const ull* __restrict__ ctr,
                                                      uint32_t* __restrict__ table,
-                                                     ull* __restrict__ pool) {
+                                                     uint8_t* __restrict__ pool8) {
   const uint64_t n_occ = ctr[kCtrNumOcc];
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
     const uint32_t b = occ[wv];
     const uint32_t s = table[b];
-    pool[(uint64_t)(s - 1) * kBrickWords + lane_id()] = 0ull;
+    uint4* dst = (uint4*)(pool8 + (uint64_t)(s - 1) * kBrickVox + lane_id() * 64);
+#pragma unroll
+    for (int q = 0; q < 4; q++) dst[q] = make_uint4(0u, 0u, 0u, 0u);
     if (lane_id() == 0) table[b] = 0u;
   }
 }
@@ -919,9 +930,10 @@ hipError_t voxelize_partition(Context& c, uint64_t part, uint64_t morton_start,
   }
   if (c.pool_cap == 0) {
     // lost install races leave unused slots behind, so the pool is not capped at nbricks
-    const uint64_t cap = std::max<uint64_t>(4096, n / 8);
+    const uint64_t cap = std::max<uint64_t>(1024, n / 64);
     SVO_TRY(ensure(c.pool, cap * kBrickWords * 8));
-    SVO_TRY(fill_u64(c.pool.as<uint64_t>(), 0, cap * kBrickWords, c.stream));
+    SVO_TRY(ensure(c.pool8, cap * kBrickVox));
+    SVO_TRY(fill_u64(c.pool8.as<uint64_t>(), 0, cap * kBrickVox / 8, c.stream));
     c.pool_cap = cap;
   }
   SVO_TRY(ensure(c.large_tri, n * 4));
@@ -942,9 +954,10 @@ hipError_t voxelize_partition(Context& c, uint64_t part, uint64_t morton_start,
     const uint64_t bc_bound = std::min<uint64_t>(nbricks, c.pool_cap);
     SVO_TRY(ensure(c.bcount, bc_bound * 8 + 8));
     SVO_TRY(fill_u64((uint64_t*)ctr, 0, 7, c.stream));  // counters 0..6
-    MarkAction mark{c.table.as<uint32_t>(), c.pool.as<ull>(), ctr, c.pool_cap, morton_start};
-    ktimer_begin(c, "vox_small_mark");
-    hipLaunchKernelGGL((k_vox_tile<MarkAction>), dim3(nblk), dim3(kTile), 0, c.stream, P, mark,
+    // touch pass: flag the bricks of every candidate voxel, collect the large triangles
+    TouchAction touch{c.table.as<uint32_t>(), morton_start >> kBrickBits};
+    ktimer_begin(c, "vox_small_touch");
+    hipLaunchKernelGGL((k_vox_tile<TouchAction>), dim3(nblk), dim3(kTile), 0, c.stream, P, touch,
                        c.large_tri.as<uint32_t>(), c.large_cols.as<uint32_t>(), ctr, 1);
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
@@ -952,19 +965,33 @@ hipError_t voxelize_partition(Context& c, uint64_t part, uint64_t morton_start,
         ColLoad{c.large_cols.as<uint32_t>()},
         CountStore{c.large_off.as<uint64_t>(), ctr + kCtrCols, ctr + kCtrLargeCount}, n,
         c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrLargeCount)));
+    ktimer_begin(c, "vox_large_touch");
+    hipLaunchKernelGGL((k_vox_large<TouchAction>), dim3(large_grid), dim3(256), 0, c.stream, P,
+                       touch, c.large_tri.as<uint32_t>(), c.large_off.as<uint64_t>(), ctr);
+    ktimer_end(c);
+    SVO_TRY(hipGetLastError());
+    // number the flagged bricks in Morton order: table entry = slot + 1, occ = brick list
+    SVO_TRY((scan_exclusive<uint32_t>(
+        OccLoad{c.table.as<uint32_t>()},
+        OccStore{c.occ_list.as<uint32_t>(), c.table.as<uint32_t>(), ctr, nbricks}, nbricks,
+        c.scan_scratch.as<uint32_t>(), c.stream)));
+    // mark pass: byte stores into the slots
+    MarkAction mark{c.table.as<uint32_t>(), c.pool8.as<uint8_t>(), ctr, c.pool_cap, morton_start};
+    ktimer_begin(c, "vox_small_mark");
+    hipLaunchKernelGGL((k_vox_tile<MarkAction>), dim3(nblk), dim3(kTile), 0, c.stream, P, mark,
+                       c.large_tri.as<uint32_t>(), c.large_cols.as<uint32_t>(), ctr, 0);
+    ktimer_end(c);
+    SVO_TRY(hipGetLastError());
     ktimer_begin(c, "vox_large_mark");
     hipLaunchKernelGGL((k_vox_large<MarkAction>), dim3(large_grid), dim3(256), 0, c.stream, P,
                        mark, c.large_tri.as<uint32_t>(), c.large_off.as<uint64_t>(), ctr);
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
-    // occupied bricks in Morton order, their voxel counts and offsets
-    SVO_TRY((scan_exclusive<uint32_t>(OccLoad{c.table.as<uint32_t>()},
-                                      OccStore{c.occ_list.as<uint32_t>(), ctr, nbricks}, nbricks,
-                                      c.scan_scratch.as<uint32_t>(), c.stream)));
+    // per-brick voxel counts (bytes packed to bits) and their offsets
     ktimer_begin(c, "brick_count");
     hipLaunchKernelGGL(k_brick_count, dim3(brick_grid(bc_bound)), dim3(256), 0, c.stream,
-                       c.occ_list.as<uint32_t>(), ctr, c.table.as<uint32_t>(), c.pool.as<ull>(),
-                       c.bcount.as<uint64_t>());
+                       c.occ_list.as<uint32_t>(), ctr, c.pool_cap, c.table.as<uint32_t>(),
+                       c.pool8.as<uint8_t>(), c.pool.as<ull>(), c.bcount.as<uint64_t>());
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
     SVO_TRY((scan_exclusive<uint64_t>(
@@ -973,15 +1000,17 @@ hipError_t voxelize_partition(Context& c, uint64_t part, uint64_t morton_start,
         c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrNumOcc)));
     SVO_TRY(read_counters(c, hc));
     ktimer_flush(c);
-    if (!hc[kCtrOverflow]) break;
-    // pool exhausted: clear everything, grow, redo
-    const uint64_t want = std::max<uint64_t>(c.pool_cap * 2, hc[kCtrPoolCount] + 4096);
+    if (!hc[kCtrOverflow] && hc[kCtrNumOcc] <= c.pool_cap) break;
+    // pool too small for the flagged bricks: clear everything, grow, redo
+    const uint64_t want = std::max<uint64_t>(c.pool_cap * 2, hc[kCtrNumOcc] + 1024);
     SVO_TRY(fill_u32(c.table.as<uint32_t>(), 0, nbricks, c.stream));
     release(c.pool);
+    release(c.pool8);
     SVO_TRY(ensure(c.pool, want * kBrickWords * 8));
-    SVO_TRY(fill_u64(c.pool.as<uint64_t>(), 0, want * kBrickWords, c.stream));
+    SVO_TRY(ensure(c.pool8, want * kBrickVox));
+    SVO_TRY(fill_u64(c.pool8.as<uint64_t>(), 0, want * kBrickVox / 8, c.stream));
     c.pool_cap = want;
-    if (attempt > 24 || want > (1ull << 31) / kBrickWords) {
+    if (attempt > 8 || want > (1ull << 26)) {
       set_error("brick pool could not be sized");
       return hipErrorOutOfMemory;
     }
@@ -1036,7 +1065,8 @@ hipError_t voxelize_partition(Context& c, uint64_t part, uint64_t morton_start,
     }
     ktimer_begin(c, "brick_clear");
     hipLaunchKernelGGL(k_brick_clear, dim3(brick_grid(n_occ)), dim3(256), 0, c.stream,
-                       c.occ_list.as<uint32_t>(), ctr, c.table.as<uint32_t>(), c.pool.as<ull>());
+                       c.occ_list.as<uint32_t>(), ctr, c.table.as<uint32_t>(),
+                       c.pool8.as<uint8_t>());
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
   }
