@@ -9,6 +9,16 @@ partition). Prints one JSON line.
 
 The CPU baseline is the single-threaded CPU restatement of the reference algorithm
 (oracle/), timed on the same host on the same workload.
+
+Two more sections report the batched and served forms on a 512-mesh mix (icosphere /
+terrain 50/50, 1k-2M triangles log-uniform, gridsize 64-1024, 30% payload; meshes
+resident in HBM):
+  * batched: the whole mix in one svo_batch_run against a plain per-mesh loop of
+    svo_run, every job's node and data bytes compared between the two;
+  * served: open-loop Poisson arrivals (50-2000 jobs/s, plus 2% large jobs: a 2049^2
+    terrain, 8,388,608 triangles, payload, gridsize 2048) into svo_server with a 200 GB
+    admission budget; p50/p99/p999 latency and jobs/s, next to the plain loop's latency
+    under the same arrivals (a FIFO queue served at the measured per-job plain times).
 """
 import argparse
 import json
@@ -48,6 +58,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="terrain2048", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched section")
+    ap.add_argument("--no-served", action="store_true", help="skip the served section")
+    ap.add_argument("--mix-jobs", type=int, default=512, help="meshes in the job mix")
+    ap.add_argument("--served-seconds", type=float, default=2.0,
+                    help="arrival window per rate of the served sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -170,8 +185,197 @@ def main():
                                "gpu_speedup": value / (N / cpu_s),
                                "bit_exact_vs_gpu": bool(same)}
     ctx.close()
+
+    mix = None
+    if not (args.no_batched and args.no_served):
+        try:
+            mix = MixBench(args.mix_jobs)
+        except Exception as e:  # the headline line is printed regardless
+            out["batched"] = out["served"] = {"error": repr(e)}
+    if mix is not None and not args.no_batched:
+        try:
+            out["batched"] = mix.batched()
+        except Exception as e:
+            out["batched"] = {"error": repr(e)}
+    if mix is not None and not args.no_served:
+        try:
+            out["served"] = mix.served(args.served_seconds)
+        except Exception as e:
+            out["served"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def _pct(a, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(a, dtype=np.float64), q)) if len(a) else None
+
+
+class MixBench:
+    """The 512-mesh mix on the device, its plain per-job results kept for comparison."""
+
+    def __init__(self, n_jobs):
+        import numpy as np
+        import torch
+
+        from ooc_svo_builder_amd import SvoContext, generators
+        self.np, self.torch = np, torch
+        workers = max(1, min(16, os.cpu_count() or 1))
+        t = time.perf_counter()
+        jobs = generators.job_mix(n_jobs, seed=4, workers=workers)
+        _log("mix of %d meshes generated in %.1f s" % (n_jobs, time.perf_counter() - t))
+        dev = torch.device("cuda:0")
+        self.jobs = [dict(j, records=torch.from_numpy(j["records"]).to(dev)) for j in jobs]
+        self.n_tris = int(sum(j["records"].shape[0] for j in jobs))
+        del jobs
+        torch.cuda.synchronize()
+        self.ctx = SvoContext(0)
+        # plain per-job loop: warm-up, timed pass, then a pass that keeps every output
+        for j in self.jobs[:8]:
+            self._plain(j)
+        torch.cuda.synchronize()
+        self.plain_ms = []
+        t0 = time.perf_counter()
+        for j in self.jobs:
+            t = time.perf_counter()
+            self._plain(j)
+            self.plain_ms.append((time.perf_counter() - t) * 1e3)
+        self.plain_s = time.perf_counter() - t0
+        _log("plain loop: %d jobs in %.3f s" % (len(self.jobs), self.plain_s))
+        self.want = []
+        for j in self.jobs:
+            info = self._plain(j)
+            nodes = torch.empty(info.n_nodes * 24, dtype=torch.uint8, device=dev)
+            data = torch.empty(info.n_data * 32, dtype=torch.uint8, device=dev)
+            self.ctx.copy_nodes_to(nodes.data_ptr(), 0, info.n_nodes)
+            self.ctx.copy_data_to(data.data_ptr(), 0, info.n_data)
+            self.want.append((info.n_nodes, info.n_data, info.nfilled, nodes, data))
+
+    def _plain(self, j):
+        return self.ctx.run(j["records"], j["payload"], j["gridsize"])
+
+    def batched(self):
+        torch = self.torch
+        from ooc_svo_builder_amd import SvoContext
+        b = SvoContext(0)
+        res, info = b.batch_run(self.jobs)  # warm-up (allocations)
+        reps = 3
+        times = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            res, info = b.batch_run(self.jobs)
+            times.append(time.perf_counter() - t)
+        best = min(times)
+        same = 0
+        dev = torch.device("cuda:0")
+        for r, w in zip(res, self.want):
+            nn, nd, nf, nodes, data = w
+            if (r.n_nodes, r.n_data, r.nfilled) != (nn, nd, nf):
+                continue
+            got_n = torch.empty_like(nodes)
+            got_d = torch.empty_like(data)
+            b.copy_nodes_to(got_n.data_ptr(), r.node_offset, nn)
+            b.copy_data_to(got_d.data_ptr(), r.data_offset, nd)
+            same += int(torch.equal(got_n, nodes) and torch.equal(got_d, data))
+        b.close()
+        n = len(self.jobs)
+        _log("batched: %d jobs in %.3f s, %d/%d identical" % (n, best, same, n))
+        return {"jobs": n, "triangles": self.n_tris, "seconds": best,
+                "jobs_per_s": n / best, "triangles_per_s": self.n_tris / best,
+                "stage_ms": {"partition": info.ms_partition, "voxelize": info.ms_voxelize,
+                             "build": info.ms_build},
+                "plain_loop_seconds": self.plain_s, "plain_jobs_per_s": n / self.plain_s,
+                "speedup_vs_plain": self.plain_s / best,
+                "identical_jobs": same, "identical": same == n,
+                "voxels": int(info.nfilled), "nodes": int(info.n_nodes)}
+
+    def served(self, seconds):
+        np, torch = self.np, self.torch
+        from ooc_svo_builder_amd import SvoServer, generators
+        large = dict(records=torch.from_numpy(generators.terrain(n=2049, seed=50, payload=True))
+                     .to("cuda:0"), payload=True, gridsize=2048)
+        torch.cuda.synchronize()
+        # plain service time of the large job (the plain loop's per-job times otherwise)
+        self._plain(large)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        self._plain(large)
+        large_ms = (time.perf_counter() - t) * 1e3
+        rng = np.random.default_rng(5)
+        rates = (50, 100, 200, 500, 1000, 2000)
+        rows = []
+        srv = SvoServer(0, budget_mb=200 * 1024)
+        # warm the slots (allocations) with one pass over a few jobs
+        for tk in [srv.submit(j) for j in self.jobs[:16] + [large]]:
+            srv.wait(tk)
+            srv.release(tk)
+        for lam in rates:
+            n = max(1, int(lam * seconds))
+            gaps = rng.exponential(1.0 / lam, size=n)
+            arrive = np.cumsum(gaps)
+            pick = [(-1 if rng.uniform() < 0.02 else int(rng.integers(0, len(self.jobs))))
+                    for _ in range(n)]
+            import threading
+            lat = []
+            done_t = []
+            tickets = []
+            cv = threading.Condition()
+
+            def collector():
+                k = 0
+                while k < n:
+                    with cv:
+                        while len(tickets) <= k:
+                            cv.wait()
+                        tk = tickets[k]
+                    r, ms = srv.wait(tk)
+                    lat.append(ms)
+                    done_t.append(time.perf_counter())
+                    srv.release(tk)
+                    k += 1
+            st0 = srv.stats()
+            th = threading.Thread(target=collector)
+            th.start()
+            t0 = time.perf_counter()
+            for i in range(n):
+                while time.perf_counter() - t0 < arrive[i]:
+                    pass
+                tk = srv.submit(large if pick[i] < 0 else self.jobs[pick[i]])
+                with cv:
+                    tickets.append(tk)
+                    cv.notify()
+            th.join()
+            span = max(done_t) - t0
+            st1 = srv.stats()
+            # the plain loop under the same arrivals: FIFO, measured per-job service times
+            svc = np.array([large_ms if p < 0 else self.plain_ms[p] for p in pick]) / 1e3
+            fin = np.empty(n)
+            tcur = 0.0
+            for i in range(n):
+                tcur = max(tcur, arrive[i]) + svc[i]
+                fin[i] = tcur
+            plat = (fin - arrive) * 1e3
+            row = {"rate_jobs_per_s": lam, "jobs": n, "large_jobs": int(sum(p < 0 for p in pick)),
+                   "jobs_per_s": n / span, "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
+                   "p999_ms": _pct(lat, 99.9),
+                   "mean_batch_jobs": (st1.completed - st0.completed) /
+                   max(1, st1.batches - st0.batches),
+                   "plain_jobs_per_s": n / fin[-1], "plain_p50_ms": _pct(plat, 50),
+                   "plain_p99_ms": _pct(plat, 99), "plain_p999_ms": _pct(plat, 99.9)}
+            rows.append(row)
+            _log("served %d/s: %.1f jobs/s p50 %.2f p99 %.2f ms (plain p99 %.1f ms)" % (
+                lam, row["jobs_per_s"], row["p50_ms"], row["p99_ms"], row["plain_p99_ms"]))
+        srv.close()
+        cap_plain = len(self.jobs) / self.plain_s
+        return {"budget_gb": 200, "slots": 2, "window_s": seconds, "sweep": rows,
+                "plain_capacity_jobs_per_s": cap_plain,
+                "large_job_plain_ms": large_ms,
+                "plain_latency_model": "FIFO queue served at the measured per-job svo_run times"}
 
 
 if __name__ == "__main__":

@@ -55,6 +55,13 @@ int svo_ctx_set_stream(svo_ctx* ctx, void* hip_stream);
 /* The hipStream_t the context launches on. */
 void* svo_ctx_stream(svo_ctx* ctx);
 
+/* Voxelizer tuning (no reference counterpart): small_cols = the column count above which
+ * a triangle goes to the column-parallel tier (<= 0 keeps the current value);
+ * dense_limit_mb = largest partition voxelized in a dense byte grid, larger ones use
+ * touched bricks only (UINT64_MAX keeps the current value; the default is a third of
+ * the free HBM, at most 64 GiB). Neither changes the output. */
+int svo_ctx_set_tuning(svo_ctx* ctx, int small_cols, uint64_t dense_limit_mb);
+
 /* partitioner.cpp:12-28: number of Morton-range partitions (a power of 8) such that the
  * gridsize^3 byte grid fits memory_limit_mb. Pure integer rule, no device work. */
 uint64_t svo_estimate_partitions(uint64_t gridsize, uint64_t memory_limit_mb);
@@ -78,8 +85,10 @@ int svo_partition_indices(svo_ctx* ctx, uint64_t partition, uint32_t* host_out);
 /* voxelizer.cpp:138-307 followed by main.cpp:354-385: voxelize one partition (its list
  * from svo_partition, or the whole mesh when no partitioning was done) over Morton
  * codes [morton_start, morton_end) and append its voxels, in Morton order, to the
- * context's voxel stream. color_mode applies to payload meshes. nfilled_out receives
- * the voxel count of this partition (main.cpp:348). */
+ * context's voxel stream. The range must be an aligned Morton cube (a partition: its
+ * length a power of 8 dividing morton_start), as main.cpp:329-333 passes. color_mode
+ * applies to payload meshes. nfilled_out receives the voxel count of this partition
+ * (main.cpp:348). */
 int svo_voxelize(svo_ctx* ctx, uint64_t partition, uint64_t morton_start,
                  uint64_t morton_end, float unitlength, uint64_t gridsize, int color_mode,
                  uint64_t* nfilled_out);
@@ -95,12 +104,14 @@ int svo_add_voxels(svo_ctx* ctx, const uint64_t* codes, const void* records32, u
  * n_nodes/n_data are the .octree header counts (octree_io.h:74-83). hazard_out is set
  * to 1 when the reference's output would be undefined for this voxel set (last voxel
  * at the second-to-last Morton code, OctreeBuilder.cpp:36-42, or an empty 1^3 grid);
- * the completed tree is produced then. generate_levels (-levels) is not supported yet
- * and returns SVO_ERR_UNSUPPORTED. */
+ * the completed tree is produced then. generate_levels != 0 (-levels) also writes a data
+ * record for every internal node, the average of its children's (OctreeBuilder.cpp:82-99),
+ * with every data record in post-order (n_data grows by the internal node count). */
 int svo_build(svo_ctx* ctx, uint64_t gridsize, int generate_levels, uint64_t* n_nodes_out,
               uint64_t* n_data_out, int* hazard_out);
 
-/* Copy records [first, first+count) of the built octree to host memory: nodes are 24 B
+/* Copy records [first, first+count) of the built octree to 'host_dst' (host memory, or
+ * device memory: the copy direction follows the pointer): nodes are 24 B
  * (u64 data, u64 children_base, i8 offsets[8]), data are 32 B (u64 morton, f32 color[3],
  * f32 normal[3]) — the .octreenodes / .octreedata layouts (octree_io.h:49-66). */
 int svo_copy_nodes(svo_ctx* ctx, void* host_dst, uint64_t first, uint64_t count);
@@ -130,6 +141,90 @@ int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payloa
             int src_on_device, const float bbox[6], uint64_t gridsize,
             uint64_t memory_limit_mb, int color_mode, int generate_levels,
             svo_run_info* info);
+
+/* Batched form (no reference counterpart: main.cpp builds one mesh per process).
+ * Runs main.cpp:281-399 for n_jobs meshes at once: every job's partitions are voxelized
+ * in the same persistent launches (one device work queue of 256-triangle tiles over all
+ * meshes) and all octrees are built by the same level passes, so the launch count does
+ * not grow with the number of meshes. Each job's output is byte-identical to svo_run on
+ * that job alone. Results: the node and data records of job j are records
+ * [node_offset, node_offset + n_nodes) and [data_offset, data_offset + n_data) of the
+ * batch's arrays (svo_copy_nodes / svo_copy_data / svo_nodes_device / svo_data_device).
+ * Limits: n_jobs <= 65536, gridsize <= 32768, fewer than 2^32 triangle references and
+ * voxels in the whole batch. generate_levels must be 0 (-levels is supported by the
+ * single-mesh calls). 'records' of a job is a host
+ * pointer, or a device pointer when src_on_device != 0 (used in place, no copy). */
+typedef struct svo_job {
+  const float* records;
+  uint64_t n_triangles;
+  int payload;
+  int src_on_device;
+  float bbox[6]; /* .tri header bbox: min xyz, max xyz */
+  uint64_t gridsize;
+  uint64_t memory_limit_mb;
+  int color_mode;
+  int generate_levels;
+} svo_job;
+typedef struct svo_job_result {
+  uint64_t n_partitions;
+  uint64_t nfilled;
+  uint64_t n_nodes;
+  uint64_t n_data;
+  uint64_t node_offset;
+  uint64_t data_offset;
+  int hazard;
+} svo_job_result;
+/* info (optional) receives batch totals and the device time of each stage. */
+int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_result* results,
+                  svo_run_info* info);
+
+/* Served form (no reference counterpart). A server owns 'slots' contexts, each with its
+ * own stream and a worker thread. Submitted jobs queue in arrival order; a free slot
+ * takes, from the head of the queue, as many jobs as fit the admission budget (estimated
+ * HBM footprint of each job's working set and of its held results) and max_batch_jobs,
+ * and runs them as one batch (svo_batch_run). Jobs that arrive while batches are on the
+ * GPU are admitted by the next free slot, so batches of different slots overlap on the
+ * device. A job's node and data records stay in HBM, owned by its ticket, until
+ * svo_server_release. Every result is byte-identical to svo_run on that job alone.
+ * All svo_server_* calls are thread-safe. A host 'records' pointer must stay valid until
+ * the job is done (the records are read when its batch starts). */
+typedef struct svo_server svo_server;
+/* admission_budget_mb = 0 selects 200 GiB; max_batch_jobs = 0 selects 512; slots = 0
+ * selects 2. */
+int svo_server_create(int device, uint64_t admission_budget_mb, uint32_t max_batch_jobs,
+                      uint32_t slots, svo_server** out);
+/* Waits for running batches, then frees every ticket and context. */
+void svo_server_destroy(svo_server* srv);
+/* Queue a job; returns at once with its ticket (> 0). */
+int svo_server_submit(svo_server* srv, const svo_job* job, uint64_t* ticket_out);
+/* state_out: 0 queued, 1 running, 2 done, 3 failed. */
+#define SVO_JOB_QUEUED 0
+#define SVO_JOB_RUNNING 1
+#define SVO_JOB_DONE 2
+#define SVO_JOB_FAILED 3
+int svo_server_poll(svo_server* srv, uint64_t ticket, int* state_out);
+/* Block until the job is done or failed. Returns the job's own status (SVO_OK or the
+ * batch's error code, message in svo_last_error()); result_out (optional) receives its
+ * counts (node_offset = data_offset = 0: the records are the ticket's own arrays) and
+ * latency_ms (optional) the time from submission to completion. */
+int svo_server_wait(svo_server* srv, uint64_t ticket, svo_job_result* result_out,
+                    double* latency_ms);
+/* The ticket's node / data records: copies (dst in host or device memory) and device
+ * pointers (valid until svo_server_release). */
+int svo_server_copy_nodes(svo_server* srv, uint64_t ticket, void* dst, uint64_t first,
+                          uint64_t count);
+int svo_server_copy_data(svo_server* srv, uint64_t ticket, void* dst, uint64_t first,
+                         uint64_t count);
+const void* svo_server_nodes_device(svo_server* srv, uint64_t ticket);
+const void* svo_server_data_device(svo_server* srv, uint64_t ticket);
+/* Free a finished ticket and its records (returns its HBM to the admission budget). */
+int svo_server_release(svo_server* srv, uint64_t ticket);
+typedef struct svo_server_stats {
+  uint64_t submitted, completed, failed, batches;
+  uint64_t max_batch_jobs_seen;
+  uint64_t peak_admitted_bytes;
+} svo_server_stats;
+int svo_server_stats_get(svo_server* srv, svo_server_stats* out);
 
 /* Per-kernel device timing with HIP events on the context's stream, the counterpart of
  * the reference's stage timers (globals.h:8-29, main.cpp:45-56; those truncate to whole

@@ -8,4 +8,4 @@ synthetic workload generators.
 from .formats import (DATA_DTYPE, NODE_DTYPE, Octree, read_octree, read_tri,  # noqa: F401
                       write_tri)
 from .lib import (CLI_BINARY, CLI_PAYLOAD, COLOR_MODES, LIB_PATH, SIGNATURES,  # noqa: F401
-                  SvoContext, SvoError, estimate_partitions, load_library)
+                  SvoContext, SvoError, SvoServer, estimate_partitions, load_library)

@@ -12,9 +12,11 @@
 //   children_base(P)   = start(P) + size(P) - 1 - popcount(mask(P))
 //   position(child r)  = children_base(P) + r,   root at n_nodes - 1 = size(root) - 1.
 // Bottom-up passes build each level (parent = code >> 3, child mask, subtree size) with
-// one scan per level; top-down passes give every parent thread its <= 8 children's
-// starts and write their 24-byte records (Node.h:16-18, octree_io.h:62-66).
+// one scan per level; top-down passes give every node its start and position and write
+// its 24-byte record (Node.h:16-18, octree_io.h:62-66), one thread per node, level by
+// level from the root.
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "svo_device.h"
@@ -45,14 +47,29 @@ __device__ __forceinline__ void put_node(uint64_t* nodes, uint64_t pos, uint64_t
   r[2] = offs;
 }
 
-// Level arrays live in arenas (code, size, mask, first child, start), level l at element
-// offset off[l]; the per-level counts n[l] and offsets are kept on the device so that the
-// passes run back to back without host round trips. Level D (the leaves) is the voxel
-// code stream itself, with implicit size 1.
+// Jobs. The voxel stream may hold several jobs (meshes), job-major, each tagged in the
+// code bits selected by tag_mask (0 for a single job). Levels are indexed by height above
+// the leaves, h = 0 (leaves) .. H: a node of job j at height h has depth D_j - h, so the
+// root of job j is its only node at height D_j, and one pass per height serves all jobs.
+// parent(c) keeps the tag and drops the lowest Morton digit.
+struct Tags {
+  uint64_t tag_mask;  // bits of the job index (shifted by kJobShift)
+  __device__ __forceinline__ uint32_t job(uint64_t c) const {
+    return (uint32_t)((c & tag_mask) >> kJobShift);
+  }
+  __device__ __forceinline__ uint64_t parent(uint64_t c) const {
+    return (c & tag_mask) | ((c & ~tag_mask) >> 3);
+  }
+};
+
+// Level arrays live in arenas (code, size, mask, first child, start, parent index),
+// height h >= 1 at element offset off[h]; the per-height counts n[h] and offsets are kept
+// on the device so that the passes run back to back without host round trips. Height 0
+// (the leaves) is the voxel code stream itself, with implicit size 1.
 struct LevelMeta {
-  unsigned long long* n;    // n[l], l = 0..D
-  unsigned long long* off;  // off[l], l = 0..D-1
-  unsigned long long* flag; // [0] arena overflow, [1] node range error, [2] root size
+  unsigned long long* n;    // n[h], h = 0..H
+  unsigned long long* off;  // off[h], h = 1..H+1
+  unsigned long long* flag; // [0] arena overflow, [1] node range error, [2] total nodes
 };
 
 struct Arena {
@@ -61,135 +78,341 @@ struct Arena {
   uint8_t* mask;
   uint32_t* first;
   uint64_t* start;
+  uint32_t* pidx;  // index of the node's parent within the parent level
   uint64_t cap;
+  // -levels only (nullptr otherwise): internal nodes in the subtree (the node included),
+  // the node's averaged colour and normal, and the first data-record rank of its subtree
+  uint64_t* isize;
+  float* cache;  // 6 floats per node
+  uint64_t* pstart;
 };
 
-// Parent level l from child level l+1: head = first child of a parent group.
+// -levels data record of an internal node (OctreeBuilder.cpp:82-99): the caches of its
+// 8 children summed in child order (a null child adds zeros), divided by the number of
+// non-null children, the normal then normalized (glm::normalize = v * (1 / sqrt(dot))).
+// Children j in [i, je) of the child level, in child order; leaf caches are the voxels'
+// own records (payload) or zero (geometry-only builds: leaves carry no data).
+__device__ __forceinline__ void average_children(const Arena& A, uint64_t coff, bool leaf,
+                                                 const float* __restrict__ leaf_rec,
+                                                 int payload, const uint64_t* code, uint64_t i,
+                                                 uint64_t je, uint64_t& isz, float* out) {
+  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
+  float notnull = 0.0f;
+  isz = 1;
+  uint64_t j = i;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, b0 = 0.0f, b1 = 0.0f, b2 = 0.0f;
+    if (j < je && (int)(code[j] & 7) == k) {
+      notnull += 1.0f;
+      const float* r = nullptr;
+      if (!leaf) {
+        isz += A.isize[coff + j];
+        r = A.cache + (coff + j) * 6;
+      } else if (payload) {
+        r = leaf_rec + j * 8 + 2;
+      }
+      if (r) {
+        a0 = r[0];
+        a1 = r[1];
+        a2 = r[2];
+        b0 = r[3];
+        b1 = r[4];
+        b2 = r[5];
+      }
+      j++;
+    }
+    c0 = c0 + a0;
+    c1 = c1 + a1;
+    c2 = c2 + a2;
+    n0 = n0 + b0;
+    n1 = n1 + b1;
+    n2 = n2 + b2;
+  }
+  out[0] = c0 / notnull;
+  out[1] = c1 / notnull;
+  out[2] = c2 / notnull;
+  const float t0 = n0 / notnull, t1 = n1 / notnull, t2 = n2 / notnull;
+  const float inv = 1.0f / sqrtf((t0 * t0 + t1 * t1) + t2 * t2);
+  out[3] = t0 * inv;
+  out[4] = t1 * inv;
+  out[5] = t2 * inv;
+}
+
+// Parent height h+1 from child height h: head = first child of a parent group; a job's
+// root (height D_j) has no parent.
 struct HeadLoad {
   Arena A;
   LevelMeta M;
+  Tags T;
+  const JobDev* jobs;
   const uint64_t* leaves;
-  int l, D;
+  int h;
   __device__ uint32_t operator()(uint64_t i) const {
     // an overflowed child level is not read (its tail lies beyond the arena)
-    if (l + 1 < D && M.off[l + 1] + M.n[l + 1] > A.cap) return 0u;
-    const uint64_t* code = (l + 1 == D) ? leaves : A.code + M.off[l + 1];
-    return (i == 0 || (code[i] >> 3) != (code[i - 1] >> 3)) ? 1u : 0u;
+    if (h > 0 && M.off[h] + M.n[h] > A.cap) return 0u;
+    const uint64_t* code = (h == 0) ? leaves : A.code + M.off[h];
+    const uint64_t c = code[i];
+    if (jobs[T.job(c)].depth <= h) return 0u;
+    return (i == 0 || T.parent(c) != T.parent(code[i - 1])) ? 1u : 0u;
   }
 };
 struct ParentStore {
   Arena A;
   LevelMeta M;
+  Tags T;
+  const JobDev* jobs;
   const uint64_t* leaves;
-  uint32_t* leaf_parent;  // leaf level only: parent index of every leaf
-  int l, D;
+  uint32_t* leaf_parent;  // height 0 only: parent index of every leaf
+  unsigned long long* root_size;  // per job: nodes in its tree
+  unsigned long long* root_isize; // per job: internal nodes in its tree (-levels)
+  const float* leaf_rec;          // -levels, payload: voxel i's record at leaf_rec + 8 i
+  int h;
   __device__ void operator()(uint64_t i, uint32_t ex, uint32_t head) const {
-    const bool leaf = (l + 1 == D);
-    if (leaf) leaf_parent[i] = ex + head - 1;
-    if (!leaf && M.off[l + 1] + M.n[l + 1] > A.cap) {
+    const bool leaf = (h == 0);
+    if (leaf)
+      leaf_parent[i] = ex + head - 1;
+    else if (M.off[h] + i < A.cap)
+      A.pidx[M.off[h] + i] = ex + head - 1;
+    if (!leaf && M.off[h] + M.n[h] > A.cap) {
       M.flag[0] = 1;
       return;
     }
-    const uint64_t* code = leaf ? leaves : A.code + M.off[l + 1];
-    const uint64_t* csize = leaf ? nullptr : A.size + M.off[l + 1];
-    const uint64_t nch = M.n[l + 1];
-    const uint64_t base = M.off[l];
+    const uint64_t* code = leaf ? leaves : A.code + M.off[h];
+    const uint64_t* csize = leaf ? nullptr : A.size + M.off[h];
+    const uint64_t nch = M.n[h];
+    const uint64_t base = M.off[h + 1];
     if (head) {
       if (base + ex >= A.cap) {
         M.flag[0] = 1;
       } else {
-        const uint64_t pc = code[i] >> 3;
+        const uint64_t pc = T.parent(code[i]);
         uint32_t m = 0;
-        uint64_t sz = 1;
+        uint64_t sz = 1, je = i;
         for (uint64_t j = i; j < nch && j < i + 8; j++) {
           const uint64_t cj = code[j];
-          if ((cj >> 3) != pc) break;
+          if (T.parent(cj) != pc) break;
           m |= 1u << (cj & 7);
           sz += csize ? csize[j] : 1ull;
+          je = j + 1;
         }
         A.code[base + ex] = pc;
         A.size[base + ex] = sz;
         A.mask[base + ex] = (uint8_t)m;
         A.first[base + ex] = (uint32_t)i;
+        const uint32_t jb = T.job(pc);
+        uint64_t isz = 1;
+        if (A.isize) {
+          float avg[6];
+          average_children(A, leaf ? 0 : M.off[h], leaf, leaf_rec, jobs[jb].payload, code, i,
+                           je, isz, avg);
+          A.isize[base + ex] = isz;
+#pragma unroll
+          for (int q = 0; q < 6; q++) A.cache[(base + ex) * 6 + q] = avg[q];
+        }
+        if (jobs[jb].depth == h + 1) {
+          root_size[jb] = sz;
+          root_isize[jb] = isz;
+        }
       }
     }
     if (i == nch - 1) {
       const uint64_t np = (uint64_t)ex + head;
-      M.n[l] = np;
-      if (l > 0) M.off[l - 1] = base + np;
+      M.n[h + 1] = np;
+      M.off[h + 2] = base + np;
     }
   }
 };
 
-// One thread per parent at level l writes its children's records (level l+1).
-__global__ __launch_bounds__(256) void k_emit_children(Arena A, LevelMeta M, int l, int D,
-                                                       int payload, uint64_t node_cap,
-                                                       uint64_t* __restrict__ nodes) {
+// Per job, after the bottom-up passes: node count (a single null root for an empty
+// grid, finalizeTree pads it; the single leaf of a 1^3 grid), node base (exclusive scan
+// over the jobs), data count (-levels adds one record per internal node), the records
+// the builder writes itself (OctreeBuilder.cpp:25-29: the null record, plus the white
+// record of geometry-only builds), and the hazard flag. One block.
+__global__ __launch_bounds__(256) void k_job_bases(JobDev* jobs, uint32_t n_jobs,
+                                                   const unsigned long long* root_size,
+                                                   const unsigned long long* root_isize,
+                                                   const uint64_t* __restrict__ codes,
+                                                   Tags T, LevelMeta M, uint64_t node_cap,
+                                                   uint64_t* __restrict__ nodes,
+                                                   const float* __restrict__ in_rec,
+                                                   float* __restrict__ records) {
+  __shared__ uint64_t wsum[256 / kWave];
+  __shared__ uint64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t j0 = 0; j0 < n_jobs; j0 += 256) {
+    const uint32_t j = j0 + threadIdx.x;
+    uint64_t nn = 0;
+    if (j < n_jobs) {
+      const JobDev& J = jobs[j];
+      nn = (J.nvox == 0 || J.depth == 0) ? 1ull : root_size[j];
+    }
+    uint64_t tot;
+    const uint64_t inc = block_incl_scan<uint64_t, 256>(nn, wsum, tot);
+    const uint64_t nb = carry + inc - nn;
+    if (j < n_jobs) {
+      JobDev& J = jobs[j];
+      J.n_nodes = nn;
+      J.nbase = nb;
+      const bool trivial = (J.nvox == 0 || J.depth == 0);
+      J.n_data = (J.payload ? 1 + J.nvox : 2) + ((J.levels && !trivial) ? root_isize[j] : 0);
+      const uint64_t G3 = J.gridsize * J.gridsize * J.gridsize;
+      uint64_t hz = 0;
+      if (J.nvox == 0) {
+        hz = (J.gridsize == 1) ? 1 : 0;
+      } else {
+        const uint64_t last = codes[J.voff + J.nvox - 1] & ~T.tag_mask;
+        // OctreeBuilder.cpp:36-42 leaves the root undefined when the last voxel is the
+        // second-to-last code; the completed tree is built and the hazard reported
+        hz = (G3 >= 2 && last == G3 - 2) ? 1 : 0;
+      }
+      J.hazard = hz;
+      if (trivial) {
+        if (nb < node_cap) {
+          uint64_t* r = nodes + 3 * nb;
+          r[0] = J.nvox ? 1ull : 0ull;
+          r[1] = 0;
+          r[2] = ~0ull;
+        } else {
+          M.flag[1] = 1;
+        }
+      }
+      float* d = records + J.dbase * 8;
+      if (J.levels && J.payload && J.depth == 0 && J.nvox && in_rec != records) {
+        // 1^3 grid: the single voxel's record keeps its place after the null record
+        for (int k = 0; k < 8; k++) d[8 + k] = in_rec[(J.dbase + 1) * 8 + k];
+      }
+      for (int k = 0; k < 8; k++) d[k] = 0.0f;
+      if (!J.payload) {
+        // data[1] = {0, colour (1,1,1), normal (0,0,0)}
+        for (int k = 0; k < 8; k++) d[8 + k] = 0.0f;
+        d[8 + 2] = d[8 + 3] = d[8 + 4] = 1.0f;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) M.flag[2] = carry;
+}
+
+// -levels: the data records of a subtree are written in post-order (a node completes
+// after its children, OctreeBuilder.cpp:58-99, 149-168), so node P's record is at
+// base + pstart(P) + S(P) - 1, with S = nodes that write a record (every node for
+// payload builds, internal nodes only for geometry-only ones) and
+// pstart(child r) = pstart(P) + sum_{r' < r} S(child r').
+__device__ __forceinline__ uint64_t levels_weight(const Arena& A, uint64_t k, int payload) {
+  return payload ? A.size[k] : A.isize[k];
+}
+
+__device__ __forceinline__ void put_data(float* rec, uint64_t idx, const float* avg) {
+  float* r = rec + idx * 8;
+  *(uint64_t*)r = 0ull;  // internal nodes' records carry morton 0
+#pragma unroll
+  for (int q = 0; q < 6; q++) r[2 + q] = canon_nan(avg[q]);
+}
+
+// One thread per node at height h >= 1: a job's root starts at 0 and sits last; any
+// other node's start is its parent's start plus the (size - 1) of its preceding
+// siblings, and its position is the parent's children_base plus its rank. It writes its
+// own record (children_base relative to its job's node array) and start; with -levels
+// also its data record. Consecutive nodes land in consecutive records, so the stores
+// coalesce.
+__global__ __launch_bounds__(256) void k_emit_level(Arena A, LevelMeta M, Tags T,
+                                                    const JobDev* __restrict__ jobs, int h,
+                                                    uint64_t node_cap,
+                                                    uint64_t* __restrict__ nodes,
+                                                    float* __restrict__ out_rec) {
   if (M.flag[0]) return;  // arena overflow: the build is redone with a larger arena
-  const uint64_t np = M.n[l];
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  const bool child_is_leaf = (l + 1 == D);
-  const uint64_t po = M.off[l] + p;
-  const uint64_t co = child_is_leaf ? 0 : M.off[l + 1];
-  const uint32_t m = A.mask[po];
-  const uint32_t cnt = (uint32_t)__popc(m);
-  const uint64_t st = A.start[po];
-  const uint64_t cb = st + A.size[po] - 1 - cnt;
-  const uint32_t f = A.first[po];
-  if (cb + cnt > node_cap) {
+  const uint64_t nc = M.n[h];
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  const uint64_t co = M.off[h];
+  const uint64_t code = A.code[co + i];
+  const JobDev& J = jobs[T.job(code)];
+  const bool lv = A.isize && J.levels;
+  const uint64_t sz = A.size[co + i];
+  const uint32_t cm = A.mask[co + i];
+  uint64_t st, pos, pst = 0;
+  if (J.depth == h) {
+    st = 0;
+    pos = sz - 1;
+  } else {
+    const uint64_t po = M.off[h + 1] + A.pidx[co + i];
+    const uint32_t f = A.first[po];
+    const uint32_t pcnt = (uint32_t)__popc((uint32_t)A.mask[po]);
+    const uint64_t ps = A.start[po];
+    const uint64_t cb = ps + A.size[po] - 1 - pcnt;
+    st = ps;
+    for (uint64_t j = f; j < i; j++) st += A.size[co + j] - 1;
+    if (lv) {
+      pst = A.pstart[po];
+      for (uint64_t j = f; j < i; j++) pst += levels_weight(A, co + j, J.payload);
+    }
+    pos = cb + (i - f);
+  }
+  const uint64_t gpos = J.nbase + pos;
+  if (pos >= J.n_nodes || gpos >= node_cap) {
     M.flag[1] = 1;
     return;
   }
-  if (child_is_leaf) return;  // leaves are written by k_emit_leaves
-  uint64_t run = st;
-  for (uint32_t r = 0; r < cnt; r++) {
-    const uint64_t c = (uint64_t)f + r;
-    const uint64_t sz = A.size[co + c];
-    const uint32_t cm = A.mask[co + c];
-    A.start[co + c] = run;
-    put_node(nodes, cb + r, 0ull, run + sz - 1 - (uint32_t)__popc(cm), offsets_of(cm));
-    run += sz - 1;
+  A.start[co + i] = st;
+  uint64_t data = 0;
+  if (lv) {
+    A.pstart[co + i] = pst;
+    const uint64_t idx = (J.payload ? 1 : 2) + pst + levels_weight(A, co + i, J.payload) - 1;
+    if (idx >= J.n_data) {
+      M.flag[1] = 1;
+      return;
+    }
+    data = idx;
+    put_data(out_rec, J.dbase + idx, A.cache + (co + i) * 6);
   }
+  put_node(nodes, gpos, data, st + sz - 1 - (uint32_t)__popc(cm), offsets_of(cm));
 }
 
 // One thread per leaf: position = children_base(parent) + rank among its siblings.
-// Consecutive leaves land in consecutive records, so the stores coalesce.
-// OctreeBuilder.cpp:137-139 (binary: data 1) / :157-159 (payload: its data index).
-__global__ __launch_bounds__(256) void k_emit_leaves(Arena A, LevelMeta M, int D, int payload,
+// OctreeBuilder.cpp:137-139 (binary: data 1) / :157-159 (payload: its data index; with
+// -levels its record moves to its post-order place).
+__global__ __launch_bounds__(256) void k_emit_leaves(Arena A, LevelMeta M, Tags T,
+                                                     const JobDev* __restrict__ jobs,
                                                      uint64_t node_cap,
+                                                     const uint64_t* __restrict__ codes,
                                                      const uint32_t* __restrict__ leaf_parent,
-                                                     uint64_t* __restrict__ nodes) {
+                                                     uint64_t* __restrict__ nodes,
+                                                     const float* __restrict__ in_rec,
+                                                     float* __restrict__ out_rec) {
   if (M.flag[0]) return;
-  const uint64_t nl = M.n[D];
+  const uint64_t nl = M.n[0];
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nl) return;
-  const uint64_t po = M.off[D - 1] + leaf_parent[i];
+  const JobDev& J = jobs[T.job(codes[i])];
+  if (J.depth == 0) return;  // 1^3 grid: its leaf is the root, written by k_job_bases
+  const uint64_t po = M.off[1] + leaf_parent[i];
   const uint32_t cnt = (uint32_t)__popc((uint32_t)A.mask[po]);
   const uint64_t cb = A.start[po] + A.size[po] - 1 - cnt;
-  const uint64_t pos = cb + (i - A.first[po]);
-  if (pos >= node_cap) {
+  const uint64_t rank = i - A.first[po];
+  const uint64_t pos = cb + rank;
+  const uint64_t gpos = J.nbase + pos;
+  if (pos >= J.n_nodes || gpos >= node_cap) {
     M.flag[1] = 1;
     return;
   }
-  put_node(nodes, pos, payload ? (1ull + i) : 1ull, 0ull, ~0ull);
-}
-
-__global__ void k_emit_root(Arena A, LevelMeta M, uint64_t node_cap, uint64_t* nodes) {
-  if (M.flag[0] || M.n[0] != 1) {  // arena overflow, or not a single root
-    M.flag[0] = 1;
-    return;
+  uint64_t data = J.payload ? (1ull + (i - J.voff)) : 1ull;
+  if (A.isize && J.levels && J.payload) {
+    const uint64_t idx = 1 + A.pstart[po] + rank;
+    if (idx >= J.n_data) {
+      M.flag[1] = 1;
+      return;
+    }
+    const float4* src = (const float4*)(in_rec + (J.dbase + data) * 8);
+    float4* dst = (float4*)(out_rec + (J.dbase + idx) * 8);
+    dst[0] = src[0];
+    dst[1] = src[1];
+    data = idx;
   }
-  const uint64_t o = M.off[0];
-  const uint32_t m = A.mask[o];
-  const uint64_t sz = A.size[o];
-  A.start[o] = 0;
-  M.flag[2] = sz;
-  if (sz > node_cap) {
-    M.flag[1] = 1;
-    return;
-  }
-  put_node(nodes, sz - 1, 0ull, sz - 1 - (uint32_t)__popc(m), offsets_of(m));
+  put_node(nodes, gpos, data, 0ull, ~0ull);
 }
 
 #define SVO_TRY(x)                   \
@@ -198,47 +421,69 @@ __global__ void k_emit_root(Arena A, LevelMeta M, uint64_t node_cap, uint64_t* n
     if (_e != hipSuccess) return _e; \
   } while (0)
 
-hipError_t build_octree(Context& c, uint64_t gridsize, int levels) {
-  if (levels) {
-    set_error("-levels is not implemented on the GPU path");
+hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
+  if (levels && (tagged || n_jobs != 1)) {
+    set_error("-levels is supported for single builds only");
     return hipErrorNotSupported;
   }
-  int D = 0;
-  while ((1ull << D) < gridsize) D++;
   const uint64_t V = c.n_vox;
-  const uint64_t total_codes = gridsize * gridsize * gridsize;
-  c.hazard = 0;
   if (V >= (1ull << 32)) {
-    set_error("more than 2^32 voxels in one octree");
+    set_error("more than 2^32 voxels in one build");
     return hipErrorInvalidValue;
   }
-
-  // data records (octree_io.h:49-53, OctreeBuilder.cpp:25-29)
-  if (!c.payload) {
-    static const float white[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0};
-    SVO_TRY(ensure(c.data, 64));
-    SVO_TRY(hipMemcpyAsync(c.data.p, white, 64, hipMemcpyHostToDevice, c.stream));
-    c.n_data = 2;
-  } else {
-    SVO_TRY(ensure_keep(c.records, (1 + V) * 32, (1 + V) * 32, c.stream));
-    SVO_TRY(fill_u64(c.records.as<uint64_t>(), 0, 4, c.stream));
-    c.n_data = 1 + V;
+  if (n_jobs == 0 || n_jobs > kMaxJobs || (!tagged && n_jobs != 1)) {
+    set_error("invalid job count");
+    return hipErrorInvalidValue;
   }
-
-  if (V == 0 || D == 0) {
-    // empty grid: one null root (finalizeTree pads the whole grid); 1^3 grid with its
-    // single voxel: the root is that leaf
-    if (V == 0 && gridsize == 1) c.hazard = 1;
-    const uint64_t rec[3] = {V ? 1ull : 0ull, 0, ~0ull};
-    SVO_TRY(ensure(c.nodes, 24));
-    SVO_TRY(hipMemcpyAsync(c.nodes.p, rec, 24, hipMemcpyHostToDevice, c.stream));
-    c.n_nodes = 1;
-    return hipStreamSynchronize(c.stream);
+  // host view of the jobs: depth, voxel range, data range
+  std::vector<JobDev> jd(n_jobs);
+  uint64_t n_records = 0;
+  int H = 0;
+  std::vector<uint64_t> bound;
+  for (uint32_t j = 0; j < n_jobs; j++) {
+    Job& J = c.jobs[j];
+    JobDev& d = jd[j];
+    memset(&d, 0, sizeof(d));
+    int D = 0;
+    while ((1ull << D) < J.gridsize) D++;
+    if (tagged && D > 16) {
+      set_error("a batched job's gridsize is at most 65536");
+      return hipErrorInvalidValue;
+    }
+    d.tris = J.tris;
+    d.unit_div = J.unit_vox != 0.0f ? 1.0f / J.unit_vox : 0.0f;
+    d.color_mode = J.color_mode;
+    d.payload = J.payload;
+    d.depth = D;
+    d.levels = levels ? 1 : 0;
+    d.gridsize = J.gridsize;
+    d.voff = J.voff;
+    d.nvox = J.nvox;
+    J.dbase = n_records;
+    J.n_data = J.payload ? 1 + J.nvox : 2;
+    d.dbase = J.dbase;
+    n_records += J.n_data;
+    if (J.nvox) H = std::max(H, D);
   }
+  // per-height upper bounds (host) size the launch grids
+  bound.assign(H + 2, 0);
+  bound[0] = V;
+  for (int h = 1; h <= H; h++) {
+    uint64_t b = 0;
+    for (uint32_t j = 0; j < n_jobs; j++) {
+      if (!jd[j].nvox || jd[j].depth < h) continue;
+      const int e = jd[j].depth - h;
+      b += (3 * e >= 63) ? jd[j].nvox : std::min<uint64_t>(jd[j].nvox, 1ull << (3 * e));
+    }
+    bound[h] = std::min(bound[h - 1], b);
+  }
+  SVO_TRY(ensure(c.jobdev, n_jobs * sizeof(JobDev)));
+  SVO_TRY(upload_async(c, c.jobdev.p, jd.data(), n_jobs * sizeof(JobDev)));
+  // data records: payload records were written by the voxelizer at dbase + 1 ..
+  SVO_TRY(ensure_keep(c.records, n_records * 32, n_records * 32, c.stream));
+  JobDev* d_jobs = c.jobdev.as<JobDev>();
+  const Tags T{tagged ? ~kCodeMask : 0ull};
 
-  // per-level upper bounds (host) size the launch grids
-  std::vector<uint64_t> bound(D + 1);
-  for (int l = 0; l <= D; l++) bound[l] = (3 * l >= 63) ? V : std::min<uint64_t>(V, 1ull << (3 * l));
   uint64_t cap = std::max<uint64_t>(V / 2 + 1024, 4096);
   for (int attempt = 0;; attempt++) {
     SVO_TRY(ensure(c.lvl_code, cap * 8));
@@ -246,48 +491,70 @@ hipError_t build_octree(Context& c, uint64_t gridsize, int levels) {
     SVO_TRY(ensure(c.lvl_mask, cap));
     SVO_TRY(ensure(c.lvl_first, cap * 4));
     SVO_TRY(ensure(c.lvl_start, cap * 8));
-    SVO_TRY(ensure(c.lvl_aux, (2 * 64 + 8) * 8));
-    SVO_TRY(ensure(c.lvl_leafp, V * 4));
-    const uint64_t node_cap = cap + V + 1;
+    SVO_TRY(ensure(c.lvl_pidx, cap * 4));
+    SVO_TRY(ensure(c.lvl_aux, (2 * 64 + 8) * 8 + (uint64_t)n_jobs * 16));
+    if (levels) {
+      SVO_TRY(ensure(c.lvl_isize, cap * 8));
+      SVO_TRY(ensure(c.lvl_cache, cap * 24));
+      SVO_TRY(ensure(c.lvl_pstart, cap * 8));
+      // null (+ white) record, one per voxel (payload), one per internal node (<= cap)
+      SVO_TRY(ensure(c.records2, (n_records + cap) * 32));
+    }
+    float* out_rec = levels ? c.records2.as<float>() : c.records.as<float>();
+    SVO_TRY(ensure(c.lvl_leafp, std::max<uint64_t>(V, 1) * 4));
+    const uint64_t node_cap = cap + V + n_jobs;
     SVO_TRY(ensure(c.nodes, node_cap * 24));
-    SVO_TRY(ensure(c.scan_scratch, scan_scratch_elems(V) * 4 + 256));
+    SVO_TRY(ensure(c.scan_scratch, scan_scratch_elems(std::max<uint64_t>(V, 1)) * 4 + 256));
     Arena A{c.lvl_code.as<uint64_t>(), c.lvl_size.as<uint64_t>(), c.lvl_mask.as<uint8_t>(),
-            c.lvl_first.as<uint32_t>(), c.lvl_start.as<uint64_t>(), cap};
+            c.lvl_first.as<uint32_t>(), c.lvl_start.as<uint64_t>(), c.lvl_pidx.as<uint32_t>(),
+            cap, levels ? c.lvl_isize.as<uint64_t>() : nullptr,
+            levels ? c.lvl_cache.as<float>() : nullptr,
+            levels ? c.lvl_pstart.as<uint64_t>() : nullptr};
     unsigned long long* meta = c.lvl_aux.as<unsigned long long>();
     LevelMeta M{meta, meta + 64, meta + 128};
-    // n[D] = V, off[D-1] = 0, flags = 0
-    SVO_TRY(fill_u64((uint64_t*)meta, 0, 2 * 64 + 8, c.stream));
-    SVO_TRY(hipMemcpyAsync(meta + D, &V, 8, hipMemcpyHostToDevice, c.stream));
+    unsigned long long* root_size = meta + 136;
+    unsigned long long* root_isize = root_size + n_jobs;
+    // n[0] = V, off[1] = 0, flags = 0, root sizes = 0
+    SVO_TRY(fill_u64((uint64_t*)meta, 0, 2 * 64 + 8 + 2 * (uint64_t)n_jobs, c.stream));
+    SVO_TRY(fill_u64((uint64_t*)meta, V, 1, c.stream));
 
-    // ---- bottom-up: levels D-1 .. 0
-    for (int l = D - 1; l >= 0; l--) {
-      HeadLoad ld{A, M, c.codes.as<uint64_t>(), l, D};
-      ParentStore st{A, M, c.codes.as<uint64_t>(), c.lvl_leafp.as<uint32_t>(), l, D};
-      SVO_TRY((scan_exclusive<uint32_t>(ld, st, bound[l + 1], c.scan_scratch.as<uint32_t>(),
-                                        c.stream, M.n + (l + 1))));
+    // ---- bottom-up: heights 1 .. H
+    for (int h = 0; h < H; h++) {
+      HeadLoad ld{A, M, T, d_jobs, c.codes.as<uint64_t>(), h};
+      ParentStore st{A, M, T, d_jobs, c.codes.as<uint64_t>(), c.lvl_leafp.as<uint32_t>(),
+                     root_size, root_isize, c.records.as<float>() + 8, h};
+      SVO_TRY((scan_exclusive<uint32_t>(ld, st, bound[h], c.scan_scratch.as<uint32_t>(),
+                                        c.stream, M.n + h)));
     }
-    // ---- top-down
-    hipLaunchKernelGGL(k_emit_root, dim3(1), dim3(1), 0, c.stream, A, M, node_cap,
-                       c.nodes.as<uint64_t>());
+    hipLaunchKernelGGL(k_job_bases, dim3(1), dim3(256), 0, c.stream, d_jobs, n_jobs, root_size,
+                       root_isize, c.codes.as<uint64_t>(), T, M, node_cap,
+                       c.nodes.as<uint64_t>(), c.records.as<float>(), out_rec);
     SVO_TRY(hipGetLastError());
-    for (int l = 0; l + 1 < D; l++) {
+    // ---- top-down: heights H .. 1, then the leaves
+    for (int h = H; h >= 1; h--) {
       ktimer_begin(c, "emit_children");
-      hipLaunchKernelGGL(k_emit_children, dim3((unsigned)((bound[l] + 255) / 256)), dim3(256), 0,
-                         c.stream, A, M, l, D, c.payload, node_cap, c.nodes.as<uint64_t>());
+      hipLaunchKernelGGL(k_emit_level, dim3((unsigned)std::max<uint64_t>((bound[h] + 255) / 256, 1)),
+                         dim3(256), 0, c.stream, A, M, T, (const JobDev*)d_jobs, h, node_cap,
+                         c.nodes.as<uint64_t>(), out_rec);
       ktimer_end(c);
       SVO_TRY(hipGetLastError());
     }
-    ktimer_begin(c, "emit_leaves");
-    hipLaunchKernelGGL(k_emit_leaves, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, c.stream, A,
-                       M, D, c.payload, node_cap, c.lvl_leafp.as<uint32_t>(),
-                       c.nodes.as<uint64_t>());
-    ktimer_end(c);
-    SVO_TRY(hipGetLastError());
+    if (V) {
+      ktimer_begin(c, "emit_leaves");
+      hipLaunchKernelGGL(k_emit_leaves, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, c.stream,
+                         A, M, T, (const JobDev*)d_jobs, node_cap, c.codes.as<uint64_t>(),
+                         c.lvl_leafp.as<uint32_t>(), c.nodes.as<uint64_t>(),
+                         c.records.as<float>(), out_rec);
+      ktimer_end(c);
+      SVO_TRY(hipGetLastError());
+    }
     unsigned long long flags[3] = {0, 0, 0};
     SVO_TRY(hipMemcpyAsync(flags, M.flag, 24, hipMemcpyDeviceToHost, c.stream));
+    SVO_TRY(hipMemcpyAsync(jd.data(), d_jobs, n_jobs * sizeof(JobDev), hipMemcpyDeviceToHost,
+                           c.stream));
     SVO_TRY(hipStreamSynchronize(c.stream));
     ktimer_flush(c);
-    if (flags[1]) {
+    if (flags[1] && !flags[0]) {
       set_error("internal: node record out of range");
       return hipErrorUnknown;
     }
@@ -301,13 +568,17 @@ hipError_t build_octree(Context& c, uint64_t gridsize, int levels) {
     }
     cap *= 4;
   }
-  {
-    uint64_t last = 0;
-    SVO_TRY(hipMemcpy(&last, c.codes.as<uint64_t>() + V - 1, 8, hipMemcpyDeviceToHost));
-    // OctreeBuilder.cpp:36-42 leaves the root undefined when the last voxel is the
-    // second-to-last code; the completed tree is built and the hazard reported
-    if (total_codes >= 2 && last == total_codes - 2) c.hazard = 1;
+  n_records = 0;
+  for (uint32_t j = 0; j < n_jobs; j++) {
+    Job& J = c.jobs[j];
+    J.n_nodes = jd[j].n_nodes;
+    J.nbase = jd[j].nbase;
+    J.hazard = (int)jd[j].hazard;
+    J.n_data = jd[j].n_data;
+    n_records += J.n_data;
   }
+  c.n_data = n_records;
+  c.data_in_records2 = levels != 0;  // the post-order records are the output
   return hipSuccess;
 }
 

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <sstream>
@@ -154,6 +155,32 @@ static int fail(hipError_t e, const char* where) {
 
 static int is_pow2_u64(uint64_t x) { return x != 0 && !(x & (x - 1)); }
 
+// Copies on the context's own stream (never the null stream, which would wait for every
+// other context's work), direction from the pointers.
+static hipError_t copy_sync(Context& c, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return hipSuccess;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, c.stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(c.stream);
+}
+static int ctz64(uint64_t x) { return x ? __builtin_ctzll(x) : 64; }
+
+// The plain flow's single job: the mesh loaded by svo_mesh_load, an empty voxel stream.
+static void reset_plain(Context& c) {
+  c.jobs.assign(1, Job{});
+  Job& J = c.jobs[0];
+  J.tris = c.tris.as<float>();
+  J.n_tris = c.n_tris;
+  J.payload = c.payload;
+  c.part_used = 0;
+  c.n_vox = 0;
+  c.nfilled = 0;
+  c.n_nodes = c.n_data = 0;
+  c.hazard = 0;
+  c.batch_state = false;
+  c.data_in_records2 = false;
+}
+
 // The 6-significant-digit text round trip a bbox value takes through the .trip header
 // (trip_tools.h:112 writes it, main.cpp:306-307 re-parses it).
 static float text_roundtrip(float v) {
@@ -204,6 +231,16 @@ int svo_ctx_create(int device, svo_ctx** out) {
     svo_ctx_destroy(ctx);
     return fail(e, "svo_ctx_create");
   }
+  {
+    // dense voxel grids up to a third of the free HBM (at most 64 GiB): 288 GB of HBM3E
+    // holds the 2048^3 grid (8 GiB) and a 4096^3 partition (64 GiB) next to everything else
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+      ctx->c.dense_limit = std::min<uint64_t>((uint64_t)free_b / 3, 64ull << 30);
+    const char* env = getenv("SVO_DENSE_LIMIT_MB");
+    if (env) ctx->c.dense_limit = (uint64_t)strtoull(env, nullptr, 10) << 20;
+  }
+  reset_plain(ctx->c);
   *out = ctx;
   return SVO_OK;
 }
@@ -213,12 +250,16 @@ void svo_ctx_destroy(svo_ctx* ctx) {
   Context& c = ctx->c;
   (void)hipSetDevice(c.device);
   if (c.stream) (void)hipStreamSynchronize(c.stream);
-  DevBuf* bufs[] = {&c.tris, &c.hitbox, &c.part_lists, &c.table, &c.pool, &c.pool8, &c.counters,
+  DevBuf* bufs[] = {&c.tris, &c.jobdev, &c.batch_tris, &c.hitbox, &c.part_lists, &c.unitdev,
+                    &c.table, &c.pool, &c.pool8, &c.grid8, &c.counters, &c.large_unit,
                     &c.large_tri, &c.large_cols, &c.large_off, &c.occ_list, &c.bcount,
                     &c.wprefix, &c.winner, &c.scan_scratch, &c.codes, &c.records, &c.nodes,
-                    &c.data, &c.lvl_code, &c.lvl_size, &c.lvl_mask, &c.lvl_first,
-                    &c.lvl_start, &c.lvl_aux, &c.lvl_leafp};
+                    &c.lvl_code, &c.lvl_size, &c.lvl_mask, &c.lvl_first, &c.lvl_start,
+                    &c.lvl_aux, &c.lvl_leafp, &c.lvl_pidx, &c.lvl_isize, &c.lvl_cache,
+                    &c.lvl_pstart, &c.records2};
   for (DevBuf* b : bufs) release(*b);
+  if (c.stage) (void)hipHostFree(c.stage);
+  c.stage = nullptr;
   ktimer_flush(c);
   for (hipEvent_t e : c.evpool) (void)hipEventDestroy(e);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
@@ -243,6 +284,13 @@ int svo_ctx_set_stream(svo_ctx* ctx, void* hip_stream) {
 }
 
 void* svo_ctx_stream(svo_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
+
+int svo_ctx_set_tuning(svo_ctx* ctx, int small_cols, uint64_t dense_limit_mb) {
+  CHECK_CTX();
+  if (small_cols > 0) ctx->c.tune.small_cols = small_cols;
+  if (dense_limit_mb != UINT64_MAX) ctx->c.dense_limit = dense_limit_mb << 20;
+  return SVO_OK;
+}
 
 uint64_t svo_estimate_partitions(uint64_t gridsize, uint64_t memory_limit_mb) {
   const uint64_t required = (gridsize * gridsize * gridsize) / 1024 / 1024;
@@ -273,10 +321,7 @@ int svo_mesh_load(svo_ctx* ctx, const float* records, uint64_t n_triangles, int 
   TRYC(hipStreamSynchronize(c.stream), "svo_mesh_load");
   c.n_tris = n_triangles;
   c.payload = payload ? 1 : 0;
-  c.n_parts = 0;
-  c.n_vox = 0;
-  c.nfilled = 0;
-  c.n_nodes = c.n_data = 0;
+  reset_plain(c);
   return SVO_OK;
 }
 
@@ -284,6 +329,7 @@ int svo_partition(svo_ctx* ctx, float unitlength, uint64_t gridsize, uint64_t n_
                   uint64_t* counts_out) {
   CHECK_CTX();
   Context& c = ctx->c;
+  if (c.batch_state) reset_plain(c);
   if (!is_pow2_u64(gridsize) || n_partitions == 0) {
     g_err = "gridsize must be a power of 2 and n_partitions > 0";
     return SVO_ERR_INVALID;
@@ -294,26 +340,28 @@ int svo_partition(svo_ctx* ctx, float unitlength, uint64_t gridsize, uint64_t n_
     g_err = "n_partitions must be a power of 8 not above gridsize^3";
     return SVO_ERR_INVALID;
   }
-  TRYC(partition_mesh(c, unitlength, gridsize, n_partitions), "svo_partition");
+  c.part_used = 0;
+  TRYC(partition_job(c, 0, unitlength, gridsize, n_partitions), "svo_partition");
   if (counts_out)
-    for (uint64_t j = 0; j < n_partitions; j++) counts_out[j] = c.part_cnt[j];
+    for (uint64_t j = 0; j < n_partitions; j++) counts_out[j] = c.jobs[0].part_cnt[j];
   return SVO_OK;
 }
 
 int svo_partition_indices(svo_ctx* ctx, uint64_t partition, uint32_t* host_out) {
   CHECK_CTX();
   Context& c = ctx->c;
-  if (partition >= c.n_parts) {
+  const Job& J = c.jobs[0];
+  if (partition >= J.n_parts) {
     g_err = "partition index out of range";
     return SVO_ERR_INVALID;
   }
-  if (c.n_parts == 1) {
+  if (J.n_parts == 1) {
     for (uint64_t i = 0; i < c.n_tris; i++) host_out[i] = (uint32_t)i;
     return SVO_OK;
   }
-  if (c.part_cnt[partition])
-    TRYC(hipMemcpy(host_out, c.part_lists.as<uint32_t>() + c.part_off[partition],
-                   c.part_cnt[partition] * 4, hipMemcpyDeviceToHost),
+  if (J.part_cnt[partition])
+    TRYC(copy_sync(c, host_out, c.part_lists.as<uint32_t>() + J.part_off[partition],
+                   J.part_cnt[partition] * 4),
          "svo_partition_indices");
   return SVO_OK;
 }
@@ -322,20 +370,39 @@ int svo_voxelize(svo_ctx* ctx, uint64_t partition, uint64_t morton_start, uint64
                  float unitlength, uint64_t gridsize, int color_mode, uint64_t* nfilled_out) {
   CHECK_CTX();
   Context& c = ctx->c;
+  if (c.batch_state) reset_plain(c);
   if (!is_pow2_u64(gridsize) || gridsize > (1u << 15) || morton_end <= morton_start ||
       morton_end > gridsize * gridsize * gridsize || color_mode < 0 || color_mode > 3) {
     g_err = "invalid voxelize arguments";
     return SVO_ERR_INVALID;
   }
-  if (c.n_parts > 1 && partition >= c.n_parts) {
+  // the reference's partitions are aligned Morton cubes (main.cpp:329-333)
+  const uint64_t span = morton_end - morton_start;
+  if (!is_pow2_u64(span) || (ctz64(span) % 3) != 0 || morton_start % span != 0) {
+    g_err = "the Morton range must be an aligned cube (a partition)";
+    return SVO_ERR_INVALID;
+  }
+  Job& J = c.jobs[0];
+  if (J.n_parts > 1 && partition >= J.n_parts) {
     g_err = "partition index out of range";
     return SVO_ERR_INVALID;
   }
-  if (c.n_parts <= 1) partition = 0;
+  if (J.n_parts <= 1) partition = 0;
+  J.gridsize = gridsize;
+  J.unit_vox = unitlength;
+  J.color_mode = color_mode;
+  VoxUnit u;
+  u.job = 0;
+  u.tris = c.tris.as<float>();
+  u.list = (J.n_parts > 1) ? c.part_lists.as<uint32_t>() + J.part_off[partition] : nullptr;
+  u.n = (J.n_parts > 1) ? J.part_cnt[partition] : c.n_tris;
+  u.stride = c.payload ? 21 : 9;
+  u.mstart = morton_start;
+  u.mend = morton_end;
+  u.unit = unitlength;
+  u.gridsize = gridsize;
   uint64_t nf = 0;
-  TRYC(voxelize_partition(c, partition, morton_start, morton_end, unitlength, gridsize,
-                          color_mode, &nf),
-       "svo_voxelize");
+  TRYC(voxelize_units(c, std::vector<VoxUnit>{u}, false, &nf), "svo_voxelize");
   c.nfilled += nf;
   if (nfilled_out) *nfilled_out = nf;
   return SVO_OK;
@@ -344,6 +411,7 @@ int svo_voxelize(svo_ctx* ctx, uint64_t partition, uint64_t morton_start, uint64
 int svo_add_voxels(svo_ctx* ctx, const uint64_t* codes, const void* records32, uint64_t n) {
   CHECK_CTX();
   Context& c = ctx->c;
+  if (c.batch_state) reset_plain(c);
   if (n == 0) return SVO_OK;
   if (!codes || (c.payload && !records32)) {
     g_err = "null codes or records";
@@ -351,7 +419,7 @@ int svo_add_voxels(svo_ctx* ctx, const uint64_t* codes, const void* records32, u
   }
   uint64_t last = 0;
   if (c.n_vox)
-    TRYC(hipMemcpy(&last, c.codes.as<uint64_t>() + c.n_vox - 1, 8, hipMemcpyDeviceToHost),
+    TRYC(copy_sync(c, &last, c.codes.as<uint64_t>() + c.n_vox - 1, 8),
          "svo_add_voxels");
   for (uint64_t i = 0; i < n; i++) {
     if ((c.n_vox || i) && codes[i] <= last) {
@@ -384,10 +452,16 @@ int svo_build(svo_ctx* ctx, uint64_t gridsize, int generate_levels, uint64_t* n_
     g_err = "gridsize must be a power of 2";
     return SVO_ERR_INVALID;
   }
-  hipError_t e = build_octree(c, gridsize, generate_levels);
+  if (c.batch_state) reset_plain(c);
+  Job& J = c.jobs[0];
+  J.gridsize = gridsize;
+  J.voff = 0;
+  J.nvox = c.n_vox;
+  hipError_t e = build_jobs(c, 1, false, generate_levels);
   if (e != hipSuccess) return fail(e, "svo_build");
+  c.hazard = J.hazard;
   unsigned long long err = 0;
-  TRYC(hipMemcpy(&err, c.counters.as<unsigned long long>() + kCtrError, 8, hipMemcpyDeviceToHost),
+  TRYC(copy_sync(c, &err, c.counters.as<unsigned long long>() + kCtrError, 8),
        "svo_build");
   if (err) {
     g_err = "internal: " + std::to_string(err) +
@@ -408,8 +482,7 @@ int svo_copy_nodes(svo_ctx* ctx, void* host_dst, uint64_t first, uint64_t count)
     return SVO_ERR_INVALID;
   }
   if (count)
-    TRYC(hipMemcpy(host_dst, (const char*)c.nodes.p + first * 24, count * 24,
-                   hipMemcpyDeviceToHost),
+    TRYC(copy_sync(c, host_dst, (const char*)c.nodes.p + first * 24, count * 24),
          "svo_copy_nodes");
   return SVO_OK;
 }
@@ -421,17 +494,14 @@ int svo_copy_data(svo_ctx* ctx, void* host_dst, uint64_t first, uint64_t count) 
     g_err = "data range out of bounds";
     return SVO_ERR_INVALID;
   }
-  const void* src = c.payload ? c.records.p : c.data.p;
   if (count)
-    TRYC(hipMemcpy(host_dst, (const char*)src + first * 32, count * 32, hipMemcpyDeviceToHost),
+    TRYC(copy_sync(c, host_dst, (const char*)c.data_out().p + first * 32, count * 32),
          "svo_copy_data");
   return SVO_OK;
 }
 
 const void* svo_nodes_device(svo_ctx* ctx) { return ctx ? ctx->c.nodes.p : nullptr; }
-const void* svo_data_device(svo_ctx* ctx) {
-  return ctx ? (ctx->c.payload ? ctx->c.records.p : ctx->c.data.p) : nullptr;
-}
+const void* svo_data_device(svo_ctx* ctx) { return ctx ? ctx->c.data_out().p : nullptr; }
 
 uint64_t svo_voxel_count(svo_ctx* ctx) { return ctx ? ctx->c.n_vox : 0; }
 
@@ -443,7 +513,7 @@ int svo_copy_codes(svo_ctx* ctx, uint64_t* host_dst, uint64_t first, uint64_t co
     return SVO_ERR_INVALID;
   }
   if (count)
-    TRYC(hipMemcpy(host_dst, c.codes.as<uint64_t>() + first, count * 8, hipMemcpyDeviceToHost),
+    TRYC(copy_sync(c, host_dst, c.codes.as<uint64_t>() + first, count * 8),
          "svo_copy_codes");
   return SVO_OK;
 }
@@ -492,10 +562,7 @@ int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payloa
       g_err = "payload flag differs from the loaded mesh";
       return SVO_ERR_INVALID;
     }
-    c.n_parts = 0;
-    c.n_vox = 0;
-    c.nfilled = 0;
-    c.n_nodes = c.n_data = 0;
+    reset_plain(c);
     TRYC(fill_u64(c.counters.as<uint64_t>(), 0, kNumCounters, c.stream), "svo_run");
   }
   const uint64_t np = svo_estimate_partitions(gridsize, memory_limit_mb);
@@ -535,6 +602,135 @@ int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payloa
     info->n_nodes = nn;
     info->n_data = nd;
     info->hazard = hz;
+    info->ms_partition = ms_p;
+    info->ms_voxelize = ms_v;
+    info->ms_build = ms_b;
+  }
+  return SVO_OK;
+}
+
+static float elapsed(Context& c) {
+  float ms = 0;
+  if (hipEventRecord(c.ev1, c.stream) != hipSuccess || hipEventSynchronize(c.ev1) != hipSuccess ||
+      hipEventElapsedTime(&ms, c.ev0, c.ev1) != hipSuccess)
+    return 0.0f;
+  return ms;
+}
+
+int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_result* results,
+                  svo_run_info* info) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (!jobs || !results || n_jobs == 0 || n_jobs > kMaxJobs) {
+    g_err = "invalid batch arguments";
+    return SVO_ERR_INVALID;
+  }
+  uint64_t host_bytes = 0;
+  for (uint32_t j = 0; j < n_jobs; j++) {
+    const svo_job& q = jobs[j];
+    if (!is_pow2_u64(q.gridsize) || q.gridsize > (1u << 15) || q.memory_limit_mb <= 1 ||
+        q.color_mode < 0 || q.color_mode > 3 || (q.n_triangles && !q.records)) {
+      g_err = "invalid arguments for batch job " + std::to_string(j);
+      return SVO_ERR_INVALID;
+    }
+    if (q.generate_levels) {
+      g_err = "-levels is not implemented on the GPU path";
+      return SVO_ERR_UNSUPPORTED;
+    }
+    if (!q.src_on_device) host_bytes += (q.n_triangles * (q.payload ? 84 : 36) + 255) & ~255ull;
+  }
+  // meshes in host memory are uploaded into one buffer; device meshes are used in place
+  reset_plain(c);
+  c.batch_state = true;
+  if (host_bytes) TRYC(ensure(c.batch_tris, host_bytes), "svo_batch_run");
+  c.jobs.assign(n_jobs, Job{});
+  uint64_t hoff = 0;
+  for (uint32_t j = 0; j < n_jobs; j++) {
+    const svo_job& q = jobs[j];
+    Job& J = c.jobs[j];
+    const uint64_t bytes = q.n_triangles * (q.payload ? 84 : 36);
+    if (q.src_on_device || !bytes) {
+      J.tris = q.records;
+    } else {
+      J.tris = (const float*)((char*)c.batch_tris.p + hoff);
+      TRYC(hipMemcpyAsync((void*)J.tris, q.records, bytes, hipMemcpyHostToDevice, c.stream),
+           "svo_batch_run");
+      hoff += (bytes + 255) & ~255ull;
+    }
+    J.n_tris = q.n_triangles;
+    J.payload = q.payload ? 1 : 0;
+    J.color_mode = q.color_mode;
+    J.gridsize = q.gridsize;
+    // main.cpp:306-311: the voxelizer's unitlength comes from the .trip text bbox
+    J.unit_vox = (text_roundtrip(q.bbox[3]) - text_roundtrip(q.bbox[0])) / (float)q.gridsize;
+  }
+  TRYC(fill_u64(c.counters.as<uint64_t>(), 0, kNumCounters, c.stream), "svo_batch_run");
+  float ms_p = 0, ms_v = 0, ms_b = 0;
+  TRYC(hipEventRecord(c.ev0, c.stream), "svo_batch_run");
+  uint64_t total_parts = 0;
+  for (uint32_t j = 0; j < n_jobs; j++) {
+    const svo_job& q = jobs[j];
+    const uint64_t np = svo_estimate_partitions(q.gridsize, q.memory_limit_mb);
+    const float u_part = (q.bbox[3] - q.bbox[0]) / (float)q.gridsize;  // partitioner.cpp:45
+    TRYC(partition_job(c, j, u_part, q.gridsize, np), "svo_batch_run");
+    total_parts += np;
+  }
+  ms_p = elapsed(c);
+  std::vector<VoxUnit> units;
+  for (uint32_t j = 0; j < n_jobs; j++) {
+    const Job& J = c.jobs[j];
+    const uint64_t G = J.gridsize;
+    const uint64_t mp = (G * G * G) / J.n_parts;
+    for (uint64_t i = 0; i < J.n_parts; i++) {
+      if (J.part_cnt[i] == 0) continue;  // main.cpp:330
+      VoxUnit u;
+      u.job = j;
+      u.tris = J.tris;
+      u.list = (J.n_parts > 1) ? c.part_lists.as<uint32_t>() + J.part_off[i] : nullptr;
+      u.n = J.part_cnt[i];
+      u.stride = J.payload ? 21 : 9;
+      u.mstart = i * mp;
+      u.mend = (i + 1) * mp;
+      u.unit = J.unit_vox;
+      u.gridsize = G;
+      units.push_back(u);
+    }
+  }
+  TRYC(hipEventRecord(c.ev0, c.stream), "svo_batch_run");
+  uint64_t nf = 0;
+  TRYC(voxelize_units(c, units, true, &nf), "svo_batch_run");
+  c.nfilled = nf;
+  ms_v = elapsed(c);
+  TRYC(hipEventRecord(c.ev0, c.stream), "svo_batch_run");
+  hipError_t e = build_jobs(c, n_jobs, true, 0);
+  if (e != hipSuccess) return fail(e, "svo_batch_run");
+  ms_b = elapsed(c);
+  unsigned long long err = 0;
+  TRYC(copy_sync(c, &err, c.counters.as<unsigned long long>() + kCtrError, 8),
+       "svo_batch_run");
+  if (err) {
+    g_err = "internal: " + std::to_string(err) + " voxelizer consistency failures";
+    return SVO_ERR_DEVICE;
+  }
+  int any_hazard = 0;
+  for (uint32_t j = 0; j < n_jobs; j++) {
+    const Job& J = c.jobs[j];
+    svo_job_result& r = results[j];
+    r.n_partitions = J.n_parts;
+    r.nfilled = J.nvox;
+    r.n_nodes = J.n_nodes;
+    r.n_data = J.n_data;
+    r.node_offset = J.nbase;
+    r.data_offset = J.dbase;
+    r.hazard = J.hazard;
+    any_hazard |= J.hazard;
+  }
+  if (info) {
+    info->n_partitions = total_parts;
+    info->nfilled = nf;
+    info->n_nodes = c.n_nodes;
+    info->n_data = c.n_data;
+    info->hazard = any_hazard;
     info->ms_partition = ms_p;
     info->ms_voxelize = ms_v;
     info->ms_build = ms_b;

@@ -24,12 +24,25 @@ enum Counter : int {
   kCtrOverflow = 1,   // pool exhausted during a pass
   kCtrLargeCount = 2, // triangles deferred to the column-parallel tier
   kCtrNumOcc = 3,     // occupied bricks after compaction
-  kCtrNumVox = 4,     // voxels of the current partition
+  kCtrNumVox = 4,     // voxels of the current pass
   kCtrLevelN = 5,     // node count of the level being built
   kCtrCols = 6,       // columns of the large tier
-  kCtrError = 7,      // sticky: internal consistency failures (cleared per job)
+  kCtrQueueA = 7,     // tile queues of the persistent voxelize launches
+  kCtrQueueB = 8,
+  kCtrQueueC = 9,
+  kCtrError = 15,     // sticky: internal consistency failures (cleared per job)
+  kCtrPartTotal = 16, // partition list length
   kNumCounters = 32
 };
+constexpr int kCtrPassClear = 15;  // counters 0..14 are cleared at the start of a pass
+
+// Voxel codes of a batch carry their job index in bits kJobShift and up, so that one
+// sorted stream holds the voxels of many meshes (job-major, Morton order within a job).
+// A job's gridsize is then at most 2^16 (codes below 2^48); a single job uses tag 0
+// and any gridsize.
+constexpr int kJobShift = 48;
+constexpr uint64_t kCodeMask = (1ull << kJobShift) - 1;
+constexpr uint32_t kMaxJobs = 1u << 16;
 
 struct DevBuf {
   void* p = nullptr;
@@ -61,42 +74,103 @@ struct PendingTimer {
   hipEvent_t a, b;
 };
 
+// One mesh to voxelize and build. The plain flow (svo_mesh_load, svo_partition,
+// svo_voxelize, svo_build) uses job 0; a batch uses one job per mesh.
+struct Job {
+  const float* tris = nullptr;  // device records (9 or 21 floats per triangle)
+  uint64_t n_tris = 0;
+  int payload = 0;
+  int color_mode = 0;
+  uint64_t gridsize = 0;
+  float unit_vox = 0.0f;  // voxelizer unitlength (main.cpp:311)
+  // partitions: lists in Context::part_lists at part_off[i], part_cnt[i] triangles
+  uint64_t n_parts = 0;
+  std::vector<uint64_t> part_off, part_cnt;
+  // results
+  uint64_t voff = 0, nvox = 0;    // range of the voxel stream (Context::codes)
+  uint64_t dbase = 0, n_data = 0; // range of the data records (Context::records)
+  uint64_t nbase = 0, n_nodes = 0;// range of the node records (Context::nodes)
+  int hazard = 0;
+};
+
+// Device view of a job for the payload records and the build.
+struct JobDev {
+  const float* tris;
+  float unit_div;
+  int color_mode;
+  int payload;
+  int depth;  // log2 gridsize
+  int levels; // -levels: internal nodes carry averaged data records
+  uint64_t gridsize;
+  uint64_t voff, nvox, dbase;
+  uint64_t n_nodes, nbase, hazard, n_data;  // written by the build
+};
+
+// Host description of one voxelize unit (one partition of one job).
+struct VoxUnit {
+  uint32_t job;
+  const float* tris;
+  const uint32_t* list;  // nullptr = all triangles of the job
+  uint64_t n;
+  int stride;
+  uint64_t mstart, mend;
+  float unit;
+  uint64_t gridsize;
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   VoxTuning tune;
 
-  // mesh (device-resident records, 9 or 21 floats each)
+  // mesh of the plain flow (device-resident records, 9 or 21 floats each)
   DevBuf tris;
   uint64_t n_tris = 0;
   int payload = 0;
 
-  // partitioning
-  DevBuf hitbox, part_lists;
-  std::vector<uint64_t> part_off, part_cnt;
-  uint64_t n_parts = 0;
+  // jobs: job 0 is the plain flow's mesh; after a batch, the batch's meshes
+  std::vector<Job> jobs;
+  bool batch_state = false;  // the outputs are a batch's
+  DevBuf jobdev;  // JobDev[jobs.size()]
+  DevBuf batch_tris;  // records of a batch's meshes loaded from host memory
 
-  // voxelizer scratch (kept zeroed between partitions: table, pool)
+  // partitioning (lists of all jobs' partitions, concatenated)
+  DevBuf hitbox, part_lists;
+  uint64_t part_used = 0;
+
+  // voxelizer scratch (kept zeroed between passes: table, grid8, pool8)
+  DevBuf unitdev;  // VoxParams[n] | ubase[n+1] | tile_off[n+1] | uvoff[n+1]
   DevBuf table, pool, pool8;  // pool: bit words per slot; pool8: bytes per slot
+  DevBuf grid8;               // dense byte grids of a pass's units (when they fit)
   uint64_t table_entries = 0;
-  uint64_t pool_cap = 0;  // bricks
+  uint64_t grid_zeroed = 0;     // bytes of grid8 known to be zero
+  uint64_t pool_cap = 0;        // slots of pool8 (and of pool) on the touch path
+  uint64_t dense_limit = 0;     // largest dense grid in bytes (set from free HBM)
   DevBuf counters;
-  DevBuf large_tri, large_cols, large_off;
+  DevBuf large_unit, large_tri, large_cols, large_off;
   DevBuf occ_list, bcount, wprefix, winner;
   DevBuf scan_scratch;
 
-  // accumulated voxel stream across partitions (sorted by construction)
-  DevBuf codes;    // u64 Morton codes
-  DevBuf records;  // payload: 32 B records, slot 0 reserved for the null record
+  // voxel stream across partitions and jobs (sorted by construction)
+  DevBuf codes;    // u64 Morton codes (job tag in the high bits for batches)
+  DevBuf records;  // 32 B data records of every job (each job: null record first)
   uint64_t n_vox = 0;
   uint64_t nfilled = 0;
 
   // build output
-  DevBuf nodes, data;
+  DevBuf nodes;
   uint64_t n_nodes = 0, n_data = 0;
   int hazard = 0;
-  DevBuf lvl_code, lvl_size, lvl_mask, lvl_first, lvl_start, lvl_aux, lvl_leafp;
+  DevBuf lvl_code, lvl_size, lvl_mask, lvl_first, lvl_start, lvl_aux, lvl_leafp, lvl_pidx;
+  DevBuf lvl_isize, lvl_cache, lvl_pstart;  // -levels: internal counts, averages, data starts
+  DevBuf records2;                          // -levels: data records in post-order
+  bool data_in_records2 = false;            // the built data records are in records2
+  const DevBuf& data_out() const { return data_in_records2 ? records2 : records; }
+
+  // pinned host staging for small uploads (bump allocated; wraps after a stream sync)
+  void* stage = nullptr;
+  size_t stage_cap = 0, stage_used = 0;
 
   StageTimes times;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -119,12 +193,24 @@ hipError_t ensure(DevBuf& b, size_t bytes);
 hipError_t ensure_keep(DevBuf& b, size_t bytes, size_t used, hipStream_t s);
 void release(DevBuf& b);
 
+// Copy 'bytes' from host memory to device memory 'dst' asynchronously on c.stream
+// through the pinned staging buffer (the source may be reused at once).
+hipError_t upload_async(Context& c, void* dst, const void* src, size_t bytes);
+
 // Stage drivers (return hipSuccess or an error; message in last error).
-hipError_t partition_mesh(Context& c, float unitlength, uint64_t gridsize, uint64_t n_parts);
-hipError_t voxelize_partition(Context& c, uint64_t part, uint64_t morton_start,
-                              uint64_t morton_end, float unitlength, uint64_t gridsize,
-                              int color_mode, uint64_t* nfilled);
-hipError_t build_octree(Context& c, uint64_t gridsize, int levels);
+// Partition job j's mesh into n_parts Morton-range partitions; lists appended to
+// c.part_lists (reset c.part_used first to start over).
+hipError_t partition_job(Context& c, uint32_t j, float unitlength, uint64_t gridsize,
+                         uint64_t n_parts);
+// Voxelize the units in the same launches and append their voxels to the stream.
+// batch = false: every unit belongs to job 0, whose voxels accumulate over calls;
+// batch = true: the units hold the whole of their jobs, whose ranges are laid out here.
+hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& units, bool batch,
+                          uint64_t* nfilled);
+// Build the octrees of jobs [0, n_jobs) from the voxel stream; 'tagged' = the stream
+// carries job tags (a batch). levels (-levels, OctreeBuilder.cpp:82-99) is supported for
+// a single job.
+hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels);
 
 hipError_t fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s);

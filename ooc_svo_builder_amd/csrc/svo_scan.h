@@ -1,7 +1,7 @@
 // Device-wide exclusive scan (reduce-then-scan), parameterised by load/store functors so
 // that producers and consumers fuse into the two passes: a Load functor yields element
 // i's value, a Store functor receives (i, exclusive prefix, value). Tiles are 256
-// threads x 8 items in striped order, so every load of a wave is one contiguous span.
+// threads x 8 consecutive items per thread.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,20 +17,21 @@ constexpr uint64_t kScanTile = (uint64_t)kScanBlock * kScanItems;
 
 // n_dev (optional) holds the actual element count on the device; n is then an upper
 // bound used only to size the grid, and elements at or beyond *n_dev are treated as
-// absent (loads give 0, stores are skipped).
+// absent (loads give 0, stores are skipped). Each thread owns kScanItems consecutive
+// elements: all its loads are issued before the single block-wide scan.
 template <typename T, typename Load>
 __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(Load ld, uint64_t n,
                                                             const unsigned long long* n_dev,
                                                             T* sums) {
   __shared__ T sm[kScanBlock / kWave];
   if (n_dev) n = min(n, (uint64_t)*n_dev);
-  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  T v[kScanItems];
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) v[k] = (base + k < n) ? ld(base + k) : (T)0;
   T acc = 0;
 #pragma unroll
-  for (int k = 0; k < kScanItems; k++) {
-    uint64_t i = base + (uint64_t)k * kScanBlock + threadIdx.x;
-    if (i < n) acc += ld(i);
-  }
+  for (int k = 0; k < kScanItems; k++) acc += v[k];
   T tot;
   block_incl_scan<T, kScanBlock>(acc, sm, tot);
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
@@ -42,17 +43,21 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_apply(Load ld, Store st, ui
                                                            const T* prefix) {
   __shared__ T sm[kScanBlock / kWave];
   if (n_dev) n = min(n, (uint64_t)*n_dev);
-  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-  if (base >= n && blockIdx.x != 0) return;  // uniform per block
-  T carry = prefix ? prefix[blockIdx.x] : (T)0;
+  if ((uint64_t)blockIdx.x * kScanTile >= n) return;  // uniform per block
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  T v[kScanItems];
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) v[k] = (base + k < n) ? ld(base + k) : (T)0;
+  T acc = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) acc += v[k];
+  T tot;
+  const T inc = block_incl_scan<T, kScanBlock>(acc, sm, tot);
+  T run = (prefix ? prefix[blockIdx.x] : (T)0) + inc - acc;
 #pragma unroll
   for (int k = 0; k < kScanItems; k++) {
-    uint64_t i = base + (uint64_t)k * kScanBlock + threadIdx.x;
-    T v = (i < n) ? ld(i) : (T)0;
-    T tot;
-    T inc = block_incl_scan<T, kScanBlock>(v, sm, tot);
-    if (i < n) st(i, carry + inc - v, v);
-    carry += tot;
+    if (base + k < n) st(base + k, run, v[k]);
+    run += v[k];
   }
 }
 

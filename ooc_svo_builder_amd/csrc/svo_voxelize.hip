@@ -35,17 +35,24 @@ namespace svo {
 
 typedef unsigned long long ull;
 
+// One voxelize unit: one Morton-range partition of one job (a mesh). A batch voxelizes
+// many units in the same launches.
 struct VoxParams {
   const float* tris;
   const uint32_t* list;  // partition's triangle indices, nullptr = identity
   uint32_t n;
   int stride;  // floats per record: 9 binary, 21 payload
   uint64_t mstart;
+  uint64_t bstart;  // mstart rounded down to a brick: origin of the brick table segment
   int pmin[3], pmax[3];
   float unit, unit_div;
   float coord_limit;  // |vertex| beyond this walks full columns
   double slab_margin; // voxels added on each side of the exact plane slab
   int small_cols;
+  uint32_t job;        // tag of the unit's codes (bits kJobShift and up)
+  int dense;           // 1: voxels in grid8; 0: in pool slots flagged by the touch pass
+  uint64_t table_base; // first entry of the unit's segment of the brick table
+  uint8_t* grid8;      // dense byte grid of the unit (nullptr on the touch path)
 };
 
 struct Setup {
@@ -327,35 +334,72 @@ __device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, i
   }
 }
 
+// Global buffers the actions work on; each action binds its unit's segment.
+struct ActArgs {
+  uint32_t* table;    // all units' brick tables, concatenated
+  uint8_t* pool8;     // touch path: 4096 bytes per slot
+  ull* pool;          // 64 bit words per slot (written by the compaction)
+  uint32_t* wprefix;  // per word: batch rank of its first voxel
+  uint32_t* winner;   // per voxel of the batch: lowest covering triangle
+  ull* ctr;
+  uint64_t pool_cap;  // touch path slots
+};
+
 // Brick allocation without atomics: the touch pass stores 1 into the table entry of
 // every brick a column's candidate voxels fall in; a scan over the table then numbers the
 // flagged bricks in Morton order (table entry = slot + 1) and lists them.
 struct TouchAction {
   static constexpr bool kTouch = true;
   uint32_t* table;
-  uint64_t brick0;  // first brick of the partition (morton_start >> kBrickBits)
+  uint64_t brick0;  // first brick of the unit's table segment
+  __device__ static TouchAction make(const ActArgs& A, const VoxParams& P) {
+    return TouchAction{A.table + P.table_base, P.bstart >> kBrickBits};
+  }
   __device__ __forceinline__ void touch(uint64_t brick_code) { table[brick_code - brick0] = 1u; }
   __device__ __forceinline__ void add(uint64_t, uint32_t) {}
   __device__ __forceinline__ void flush() {}
 };
 
-// Mark voxels: a brick slot holds one byte per voxel (Morton order) and a mark is a plain
-// byte store of 1. Concurrent marks of one voxel all store the same value, so no atomic
-// read-modify-write is needed however many triangles cover it (about three per voxel on
-// a terrain). The brick's slot comes from the table (assigned by the touch pass and the
-// table scan), cached per thread.
+// Mark voxels with plain byte stores of 1: concurrent marks of one voxel all store the
+// same value, so no atomic read-modify-write is needed however many triangles cover it
+// (about three per voxel on a terrain).
+//  * dense units: a byte per voxel of the unit's Morton range (the reference's own layout,
+//    voxelizer.cpp:144, kept in HBM); the brick's table entry is flagged with a plain
+//    store when the thread enters a brick. Only flagged bricks are compacted and cleared
+//    afterwards, so the grid is never swept.
+//  * touch units: 4096 bytes per pool slot; the slot comes from the table (touch pass +
+//    table scan), cached per thread.
 struct MarkAction {
   static constexpr bool kTouch = false;
-  const uint32_t* table;
-  uint8_t* pool8;
+  uint32_t* table;
+  uint8_t* bytes;  // dense grid, or the slot pool
   ull* ctr;
   uint64_t pool_cap;
-  uint64_t mstart;
+  uint64_t base;  // code of the first voxel of the unit's first brick
+  int dense;
   uint32_t cbrick = 0xFFFFFFFFu, cslot = 0;
+  __device__ static MarkAction make(const ActArgs& A, const VoxParams& P) {
+    MarkAction m;
+    m.table = A.table + P.table_base;
+    m.bytes = P.dense ? P.grid8 : A.pool8;
+    m.ctr = A.ctr;
+    m.pool_cap = A.pool_cap;
+    m.base = P.bstart;
+    m.dense = P.dense;
+    return m;
+  }
   __device__ __forceinline__ void touch(uint64_t) {}
   __device__ __forceinline__ void add(uint64_t code, uint32_t) {
-    const uint64_t rel = code - mstart;
+    const uint64_t rel = code - base;
     const uint32_t b = (uint32_t)(rel >> kBrickBits);
+    if (dense) {
+      bytes[rel] = 1;
+      if (b != cbrick) {
+        cbrick = b;
+        table[b] = 1u;
+      }
+      return;
+    }
     if (b != cbrick) {
       cbrick = b;
       cslot = table[b];
@@ -367,7 +411,7 @@ struct MarkAction {
       atomicAdd(&ctr[s == 0 ? kCtrError : kCtrOverflow], 1ull);
       return;
     }
-    pool8[(uint64_t)(s - 1) * kBrickVox + (rel & (kBrickVox - 1))] = 1;
+    bytes[(uint64_t)(s - 1) * kBrickVox + (rel & (kBrickVox - 1))] = 1;
   }
   __device__ __forceinline__ void flush() {}
 };
@@ -375,15 +419,18 @@ struct MarkAction {
 // Second payload pass: lowest triangle index per voxel rank.
 struct WinnerAction {
   static constexpr bool kTouch = false;
-  __device__ __forceinline__ void touch(uint64_t) {}
   const uint32_t* table;
   const ull* pool;  // bit form of the bricks, written by the compaction
   const uint32_t* wprefix;
   uint32_t* winner;
   ull* ctr;
-  uint64_t mstart;
+  uint64_t base;  // code of the first voxel of the unit's first brick
+  __device__ static WinnerAction make(const ActArgs& A, const VoxParams& P) {
+    return WinnerAction{A.table + P.table_base, A.pool, A.wprefix, A.winner, A.ctr, P.bstart};
+  }
+  __device__ __forceinline__ void touch(uint64_t) {}
   __device__ __forceinline__ void add(uint64_t code, uint32_t tri) {
-    const uint64_t rel = code - mstart;
+    const uint64_t rel = code - base;
     const uint32_t b = (uint32_t)(rel >> kBrickBits);
     const uint32_t s = table[b];
     const uint32_t wi = (uint32_t)(rel >> 6) & (kBrickWords - 1);
@@ -401,101 +448,129 @@ struct WinnerAction {
   __device__ __forceinline__ void flush() {}
 };
 
-// Small tier: a block of 256 threads takes 256 consecutive triangles of the partition
-// list. Each thread builds its triangle's walk constants into LDS; a block scan lays out
-// the (triangle, column) items; then the threads share the items, one column each, so
-// the work per lane is a few voxel tests regardless of triangle size. Triangles with more
-// than small_cols columns (or non-slab triangles with more than small_cols * 8 tests) go
-// to the column-parallel tier.
+// last k with a[k] <= v, for a non-decreasing a[0..n) with a[0] <= v
+template <typename T>
+__device__ __forceinline__ uint32_t upper_index(const T* __restrict__ a, uint32_t n, T v) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (a[mid] <= v)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Small tier, persistent: blocks take tiles of 256 consecutive triangles of one unit from
+// a device work queue (tile_off[u] = first tile of unit u). For a tile, each thread builds
+// its triangle's walk constants into LDS; a block scan lays out the (triangle, column)
+// items; then the threads share the items, one column each, so the work per lane is a few
+// voxel tests regardless of triangle size. Triangles with more than small_cols columns
+// (or non-slab triangles with more than small_cols * 8 tests) go to the column-parallel
+// tier. Units the pass does not apply to (skip mask) contribute no items.
 constexpr int kTile = 256;
+constexpr int kSkipDense = 1;   // units voxelized in a dense grid (touch pass)
+constexpr int kSkipBinary = 2;  // geometry-only units (payload winner pass)
+
+__device__ __forceinline__ bool skipped(const VoxParams& P, int skip) {
+  return ((skip & kSkipDense) && P.dense) || ((skip & kSkipBinary) && P.stride == 9);
+}
 
 template <typename Act>
-__global__ __launch_bounds__(kTile) void k_vox_tile(VoxParams P, Act act0, uint32_t* large_tri,
-                                                    uint32_t* large_cols, ull* ctr, int collect) {
+__global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict__ units,
+                                                    uint32_t n_units,
+                                                    const uint32_t* __restrict__ tile_off,
+                                                    ActArgs A, ull* queue,
+                                                    uint32_t* large_unit, uint32_t* large_tri,
+                                                    uint32_t* large_cols, int collect,
+                                                    int skip) {
   __shared__ uint32_t sk[kWalkWords][kTile];
   __shared__ uint32_t soff[kTile + 1];
   __shared__ uint32_t wsum[kTile / kWave];
-  Act act = act0;
+  __shared__ uint32_t s_tile;
+  ull* ctr = A.ctr;
+  const uint32_t total_tiles = tile_off[n_units];
   const uint32_t tid = threadIdx.x;
-  const uint32_t i = blockIdx.x * kTile + tid;
-  uint32_t items = 0;
-  if (i < P.n) {
-    const uint32_t t = P.list ? P.list[i] : i;
-    Setup S;
-    setup_tri(P.tris + (uint64_t)t * P.stride, P, S);
-    Walk K;
-    make_walk(S, P, t, K);
-    const uint64_t ncols = K.columns();
-    const uint64_t lim = (uint64_t)P.small_cols;
-    if (K.slab() ? (ncols > lim) : (K.work() > lim * 8)) {
-      if (collect) {
-        const ull idx = atomicAdd(&ctr[kCtrLargeCount], 1ull);
-        large_tri[idx] = t;
-        large_cols[idx] = (uint32_t)ncols;
+  for (;;) {
+    if (tid == 0) s_tile = (uint32_t)atomicAdd(queue, 1ull);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= total_tiles) break;
+    const uint32_t u = upper_index<uint32_t>(tile_off, n_units + 1, tile);
+    const VoxParams P = units[u];
+    const bool active = !skipped(P, skip);
+    const uint32_t i = (tile - tile_off[u]) * kTile + tid;
+    uint32_t items = 0;
+    if (active && i < P.n) {
+      const uint32_t t = P.list ? P.list[i] : i;
+      Setup S;
+      setup_tri(P.tris + (uint64_t)t * P.stride, P, S);
+      Walk K;
+      make_walk(S, P, t, K);
+      const uint64_t ncols = K.columns();
+      const uint64_t lim = (uint64_t)P.small_cols;
+      if (K.slab() ? (ncols > lim) : (K.work() > lim * 8)) {
+        if (collect) {
+          const ull idx = atomicAdd(&ctr[kCtrLargeCount], 1ull);
+          large_unit[idx] = u;
+          large_tri[idx] = t;
+          large_cols[idx] = (uint32_t)ncols;
+        }
+      } else {
+        items = (uint32_t)ncols;
+#pragma unroll
+        for (int f = 0; f < kWalkF; f++) sk[f][tid] = __float_as_uint(K.f[f]);
+#pragma unroll
+        for (int f = 0; f < kWalkI; f++) sk[kWalkF + f][tid] = (uint32_t)K.i[f];
       }
-    } else {
-      items = (uint32_t)ncols;
-#pragma unroll
-      for (int f = 0; f < kWalkF; f++) sk[f][tid] = __float_as_uint(K.f[f]);
-#pragma unroll
-      for (int f = 0; f < kWalkI; f++) sk[kWalkF + f][tid] = (uint32_t)K.i[f];
     }
-  }
-  uint32_t total;
-  const uint32_t inc = block_incl_scan<uint32_t, kTile>(items, wsum, total);
-  soff[tid] = inc - items;
-  if (tid == kTile - 1) soff[kTile] = total;
-  __syncthreads();
-  for (uint32_t it = tid; it < total; it += kTile) {
-    // owner: last j with soff[j] <= it (triangles without items have empty ranges)
-    uint32_t lo = 0, hi = kTile - 1;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) >> 1;
-      if (soff[mid] <= it)
-        lo = mid;
+    uint32_t total;
+    const uint32_t inc = block_incl_scan<uint32_t, kTile>(items, wsum, total);
+    soff[tid] = inc - items;
+    if (tid == kTile - 1) soff[kTile] = total;
+    __syncthreads();
+    Act act = Act::make(A, P);
+    for (uint32_t it = tid; it < total; it += kTile) {
+      // owner: last j with soff[j] <= it (triangles without items have empty ranges)
+      const uint32_t lo = upper_index<uint32_t>(soff, kTile, it);
+      Walk K;
+#pragma unroll
+      for (int f = 0; f < kWalkF; f++) K.f[f] = __uint_as_float(sk[f][lo]);
+#pragma unroll
+      for (int f = 0; f < kWalkI; f++) K.i[f] = (int)sk[kWalkF + f][lo];
+      const uint32_t local = it - soff[lo];
+      const uint32_t eu = (uint32_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
+      const int a = K.i[I_G0U] + (int)(local % eu);
+      const int b = K.i[I_G0V] + (int)(local / eu);
+      if constexpr (Act::kTouch)
+        touch_column(K, P, a, b, act);
       else
-        hi = mid - 1;
+        walk_column(K, P, a, b, act);
     }
-    Walk K;
-#pragma unroll
-    for (int f = 0; f < kWalkF; f++) K.f[f] = __uint_as_float(sk[f][lo]);
-#pragma unroll
-    for (int f = 0; f < kWalkI; f++) K.i[f] = (int)sk[kWalkF + f][lo];
-    const uint32_t local = it - soff[lo];
-    const uint32_t eu = (uint32_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
-    const int a = K.i[I_G0U] + (int)(local % eu);
-    const int b = K.i[I_G0V] + (int)(local / eu);
-    if constexpr (Act::kTouch)
-      touch_column(K, P, a, b, act);
-    else
-      walk_column(K, P, a, b, act);
+    act.flush();
+    __syncthreads();  // LDS (setups, offsets, s_tile) is reused by the next tile
   }
-  act.flush();
 }
 
 // Column-parallel tier: one thread per column of the deferred triangles; the counts live
 // on the device (ctr[kCtrLargeCount] triangles, ctr[kCtrCols] columns), the grid strides.
 template <typename Act>
-__global__ __launch_bounds__(256) void k_vox_large(VoxParams P, Act act0,
+__global__ __launch_bounds__(256) void k_vox_large(const VoxParams* __restrict__ units, ActArgs A,
+                                                   const uint32_t* __restrict__ large_unit,
                                                    const uint32_t* __restrict__ large_tri,
                                                    const uint64_t* __restrict__ large_off,
-                                                   const ull* __restrict__ ctr) {
+                                                   int skip) {
+  const ull* ctr = A.ctr;
   const uint32_t n_large = (uint32_t)ctr[kCtrLargeCount];
   const uint64_t total = ctr[kCtrCols];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  Act act = act0;
   for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
-    // last j with large_off[j] <= g
-    uint32_t lo = 0, hi = n_large - 1;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) >> 1;
-      if (large_off[mid] <= g)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const uint32_t t = large_tri[lo];
-    const uint64_t local = g - large_off[lo];
+    const uint32_t j = upper_index<uint64_t>(large_off, n_large, g);
+    const VoxParams& P = units[large_unit[j]];
+    if (skipped(P, skip)) continue;
+    const uint32_t t = large_tri[j];
+    const uint64_t local = g - large_off[j];
     Setup S;
     setup_tri(P.tris + (uint64_t)t * P.stride, P, S);
     Walk K;
@@ -503,12 +578,13 @@ __global__ __launch_bounds__(256) void k_vox_large(VoxParams P, Act act0,
     const uint64_t eu = (uint64_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
     const int a = K.i[I_G0U] + (int)(local % eu);
     const int b = K.i[I_G0V] + (int)(local / eu);
+    Act act = Act::make(A, P);
     if constexpr (Act::kTouch)
       touch_column(K, P, a, b, act);
     else
       walk_column(K, P, a, b, act);
+    act.flush();
   }
-  act.flush();
 }
 
 // ---------------------------------------------------------------- compaction
@@ -539,33 +615,41 @@ __device__ __forceinline__ uint64_t waves_total() {
   return (uint64_t)gridDim.x * blockDim.x / kWave;
 }
 
-// Pack each occupied brick's bytes into 64 bit words (lane l = voxels 64l..64l+63) and
-// count its voxels.
+// Pack each flagged brick's bytes into 64 bit words (lane l = voxels 64l..64l+63) at the
+// brick's rank, count its voxels, and zero the bytes that were set. The bytes live in the
+// unit's dense grid (at the brick's index) or in the slot pool (at its rank).
+// ubase[u] = first table entry of unit u (n_units + 1 entries).
 __global__ __launch_bounds__(256) void k_brick_count(const uint32_t* __restrict__ occ,
-                                                     const ull* __restrict__ ctr,
-                                                     uint64_t pool_cap,
-                                                     const uint32_t* __restrict__ table,
-                                                     const uint8_t* __restrict__ pool8,
+                                                     const ull* __restrict__ ctr, uint64_t cap,
+                                                     const VoxParams* __restrict__ units,
+                                                     const uint64_t* __restrict__ ubase,
+                                                     uint32_t n_units, uint8_t* __restrict__ pool8,
                                                      ull* __restrict__ pool,
                                                      uint64_t* __restrict__ bcount) {
-  // more flagged bricks than slots: the pass is redone with a larger pool
-  const uint64_t n_occ = min((uint64_t)ctr[kCtrNumOcc], pool_cap);
+  // more flagged bricks than slots (touch path): the pass is redone with a larger pool
+  const uint64_t n_occ = min((uint64_t)ctr[kCtrNumOcc], cap);
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
-    const uint32_t s = table[occ[wv]];
-    const uint4* src = (const uint4*)(pool8 + (uint64_t)(s - 1) * kBrickVox + lane_id() * 64);
+    const uint64_t ti = occ[wv];
+    const uint32_t u = upper_index<uint64_t>(ubase, n_units + 1, ti);
+    uint8_t* base = units[u].dense ? units[u].grid8 + (ti - ubase[u]) * kBrickVox
+                                   : pool8 + wv * kBrickVox;
+    uint4* src = (uint4*)(base + lane_id() * 64);
+    uint4 v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = src[q];
     ull m = 0;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const uint4 v = src[q];
-      const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+      const uint32_t w4[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
 #pragma unroll
       for (int d = 0; d < 4; d++) {
         const uint32_t x = w4[d];  // four bytes, each 0 or 1
         const uint32_t nib = (x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u);
         m |= (ull)nib << (16 * q + 4 * d);
       }
+      if (v[q].x | v[q].y | v[q].z | v[q].w) src[q] = make_uint4(0u, 0u, 0u, 0u);
     }
-    pool[(uint64_t)(s - 1) * kBrickWords + lane_id()] = m;
+    pool[wv * kBrickWords + lane_id()] = m;
     const uint32_t c = (uint32_t)__popcll(m);
     const uint32_t tot = wave_incl_scan<uint32_t>(c);
     if (lane_id() == kWave - 1) bcount[wv] = tot;
@@ -586,24 +670,29 @@ struct ColLoad {
   __device__ uint64_t operator()(uint64_t i) const { return a[i]; }
 };
 
+// Sorted codes of every flagged brick, tagged with the unit's job in bits kJobShift+.
 __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__ occ,
                                                     const ull* __restrict__ ctr,
-                                                    const uint32_t* __restrict__ table,
+                                                    const VoxParams* __restrict__ units,
+                                                    const uint64_t* __restrict__ ubase,
+                                                    uint32_t n_units,
                                                     const ull* __restrict__ pool,
                                                     const uint64_t* __restrict__ boff,
-                                                    uint64_t mstart, uint64_t* __restrict__ codes,
+                                                    uint64_t* __restrict__ codes,
                                                     uint32_t* __restrict__ wprefix) {
   const uint64_t n_occ = ctr[kCtrNumOcc];
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
-    const uint32_t b = occ[wv];
-    const uint32_t s = table[b];
-    const uint64_t slot = (uint64_t)(s - 1) * kBrickWords + lane_id();
+    const uint64_t ti = occ[wv];
+    const uint32_t u = upper_index<uint64_t>(ubase, n_units + 1, ti);
+    const uint64_t slot = wv * kBrickWords + lane_id();
     ull word = pool[slot];
     const uint32_t c = (uint32_t)__popcll(word);
     const uint32_t inc = wave_incl_scan<uint32_t>(c);
     uint64_t pos = boff[wv] + (inc - c);
     if (wprefix) wprefix[slot] = (uint32_t)pos;
-    const uint64_t cb = mstart + ((uint64_t)b << kBrickBits) + ((uint64_t)lane_id() << 6);
+    const uint64_t cb = (units[u].bstart + ((ti - ubase[u]) << kBrickBits) +
+                         ((uint64_t)lane_id() << 6)) |
+                        ((uint64_t)units[u].job << kJobShift);
     while (word) {
       const int bit = __ffsll(word) - 1;
       codes[pos++] = cb + (uint64_t)bit;
@@ -612,38 +701,62 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
   }
 }
 
-// restore the all-zero state of the table and the byte pool for the occupied bricks
+// restore the all-zero state of the table for the flagged bricks (their bytes were
+// cleared by k_brick_count)
 __global__ __launch_bounds__(256) void k_brick_clear(const uint32_t* __restrict__ occ,
                                                      const ull* __restrict__ ctr,
-                                                     uint32_t* __restrict__ table,
-                                                     uint8_t* __restrict__ pool8) {
+                                                     uint32_t* __restrict__ table) {
   const uint64_t n_occ = ctr[kCtrNumOcc];
-  for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
-    const uint32_t b = occ[wv];
-    const uint32_t s = table[b];
-    uint4* dst = (uint4*)(pool8 + (uint64_t)(s - 1) * kBrickVox + lane_id() * 64);
-#pragma unroll
-    for (int q = 0; q < 4; q++) dst[q] = make_uint4(0u, 0u, 0u, 0u);
-    if (lane_id() == 0) table[b] = 0u;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_occ; i += stride)
+    table[occ[i]] = 0u;
+}
+
+// First voxel (batch rank) of each unit: the offset of its first flagged brick.
+__global__ void k_unit_voxels(const uint32_t* __restrict__ occ, const ull* __restrict__ ctr,
+                              const uint64_t* __restrict__ ubase, uint32_t n_units,
+                              const uint64_t* __restrict__ boff, uint64_t* __restrict__ uvoff) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u > n_units) return;
+  const uint64_t n_occ = ctr[kCtrNumOcc];
+  const uint64_t V = ctr[kCtrNumVox];
+  if (u == n_units) {
+    uvoff[u] = V;
+    return;
   }
+  // first occupied brick with table index >= ubase[u]
+  uint64_t lo = 0, hi = n_occ;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if ((uint64_t)occ[mid] < ubase[u])
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  uvoff[u] = (lo < n_occ) ? boff[lo] : V;
 }
 
 // ---------------------------------------------------------------- payload records
 // BarycentricCoords.h:4-32 for the winning triangle; colour modes of main.cpp:373-382.
+// Voxel i of the pass (stream index s0 + i) belongs to the job in its code's tag; its
+// record goes to dbase[job] + 1 + (s0 + i - voff[job]). Voxels of geometry-only jobs
+// have no record here.
 __global__ __launch_bounds__(256) void k_payload_records(
-    const float* __restrict__ tris, const uint64_t* __restrict__ codes,
-    const uint32_t* __restrict__ winner, uint64_t nvox, float unit_div, int color_mode,
-    uint64_t gridsize, float* __restrict__ out, ull* ctr) {
+    const JobDev* __restrict__ jobs, const uint64_t* __restrict__ codes, uint64_t s0,
+    const uint32_t* __restrict__ winner, uint64_t nvox, float* __restrict__ out, ull* ctr) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nvox) return;
-  const uint64_t code = codes[i];
+  const uint64_t tagged = codes[s0 + i];
+  const JobDev& J = jobs[tagged >> kJobShift];
+  if (!J.payload) return;
+  const uint64_t code = tagged & kCodeMask;
   const uint32_t t = winner[i];
   float col0, col1, col2, nr0, nr1, nr2;
   if (t == 0xFFFFFFFFu) {  // cannot happen: both passes evaluate the same predicates
     atomicAdd(&ctr[kCtrError], 1ull);
     col0 = col1 = col2 = nr0 = nr1 = nr2 = 0.0f;
   } else {
-    const float* p = tris + (uint64_t)t * 21;
+    const float* p = J.tris + (uint64_t)t * 21;
     const float v0x = p[0], v0y = p[1], v0z = p[2];
     const float v1x = p[3], v1y = p[4], v1z = p[5];
     const float v2x = p[6], v2y = p[7], v2z = p[8];
@@ -652,6 +765,8 @@ __global__ __launch_bounds__(256) void k_payload_records(
     nr2 = p[11];
     uint32_t x, y, z;
     unmorton3(code, x, y, z);
+    const int color_mode = J.color_mode;
+    const float unit_div = J.unit_div;
     if (color_mode == 0) {
       // n = normalize(cross(e0, e1)) exactly as in the voxelizer
       const float e0x = v1x - v0x, e0y = v1y - v0y, e0z = v1z - v0z;
@@ -692,7 +807,7 @@ __global__ __launch_bounds__(256) void k_payload_records(
     } else if (color_mode == 1) {  // fixed white
       col0 = col1 = col2 = 1.0f;
     } else if (color_mode == 2) {  // mortonToRGB: (z, y, x) / gridsize
-      const float g = (float)gridsize;
+      const float g = (float)J.gridsize;
       col0 = (float)z / g;
       col1 = (float)y / g;
       col2 = (float)x / g;
@@ -704,7 +819,7 @@ __global__ __launch_bounds__(256) void k_payload_records(
       col2 = (c + 1.0f) / 2.0f;
     }
   }
-  float* r = out + i * 8;
+  float* r = out + (J.dbase + 1 + (s0 + i - J.voff)) * 8;
   *(uint64_t*)r = code;
   r[2] = canon_nan(col0);
   r[3] = canon_nan(col1);
@@ -800,6 +915,28 @@ hipError_t fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s) {
     if (_e != hipSuccess) return _e;      \
   } while (0)
 
+hipError_t upload_async(Context& c, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return hipSuccess;
+  const size_t need = (bytes + 255) & ~(size_t)255;
+  if (c.stage_used + need > c.stage_cap) {
+    // every copy from the staging buffer has completed after this
+    SVO_TRY(hipStreamSynchronize(c.stream));
+    c.stage_used = 0;
+    if (need > c.stage_cap) {
+      if (c.stage) (void)hipHostFree(c.stage);
+      c.stage = nullptr;
+      c.stage_cap = 0;
+      const size_t cap = std::max<size_t>(need, 4u << 20);
+      SVO_TRY(hipHostMalloc(&c.stage, cap, hipHostMallocDefault));
+      c.stage_cap = cap;
+    }
+  }
+  void* h = (char*)c.stage + c.stage_used;
+  memcpy(h, src, bytes);
+  c.stage_used += need;
+  return hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, c.stream);
+}
+
 static hipError_t read_counters(Context& c, ull* h) {
   SVO_TRY(hipMemcpyAsync(h, c.counters.p, kNumCounters * sizeof(ull), hipMemcpyDeviceToHost,
                          c.stream));
@@ -810,12 +947,14 @@ static hipError_t ensure_scan_scratch(Context& c, uint64_t n, size_t elem) {
   return ensure(c.scan_scratch, scan_scratch_elems(n) * elem + 256);
 }
 
-hipError_t partition_mesh(Context& c, float unitlength, uint64_t gridsize, uint64_t n_parts) {
-  c.n_parts = n_parts;
-  c.part_off.assign(n_parts, 0);
-  c.part_cnt.assign(n_parts, 0);
+hipError_t partition_job(Context& c, uint32_t jid, float unitlength, uint64_t gridsize,
+                         uint64_t n_parts) {
+  Job& J = c.jobs[jid];
+  J.n_parts = n_parts;
+  J.part_off.assign(n_parts, 0);
+  J.part_cnt.assign(n_parts, 0);
   if (n_parts <= 1) {
-    c.part_cnt[0] = c.n_tris;
+    J.part_cnt[0] = J.n_tris;
     return hipSuccess;
   }
   // partitions are the 8^k aligned Morton cubes; K cubes per axis
@@ -828,169 +967,245 @@ hipError_t partition_mesh(Context& c, float unitlength, uint64_t gridsize, uint6
   }
   const uint64_t side = gridsize / (uint64_t)K;
   // world box of cube c along any axis: [c*side*u, (c+1)*side*u] (partitioner.cpp:56-61)
-  std::vector<float> lo(K), hi(K);
+  std::vector<float> lohi(2 * 256, 0.0f);
   for (int i = 0; i < K; i++) {
-    lo[i] = (float)(uint32_t)(i * side) * unitlength;
-    hi[i] = (float)(uint32_t)((i + 1) * side) * unitlength;
+    lohi[i] = (float)(uint32_t)(i * side) * unitlength;
+    lohi[256 + i] = (float)(uint32_t)((i + 1) * side) * unitlength;
   }
-  const int stride = c.payload ? 21 : 9;
-  SVO_TRY(ensure(c.hitbox, c.n_tris * 8 + 2 * 256 * sizeof(float) + 256));
-  float* d_lo = (float*)((char*)c.hitbox.p + c.n_tris * 8 + 64);
+  const uint64_t n = J.n_tris;
+  const int stride = J.payload ? 21 : 9;
+  SVO_TRY(ensure(c.hitbox, n * 8 + 2 * 256 * sizeof(float) + 256));
+  float* d_lo = (float*)((char*)c.hitbox.p + ((n * 8 + 255) & ~255ull));
   float* d_hi = d_lo + 256;
-  SVO_TRY(hipMemcpyAsync(d_lo, lo.data(), K * 4, hipMemcpyHostToDevice, c.stream));
-  SVO_TRY(hipMemcpyAsync(d_hi, hi.data(), K * 4, hipMemcpyHostToDevice, c.stream));
-  if (c.n_tris) {
+  SVO_TRY(upload_async(c, d_lo, lohi.data(), 2 * 256 * sizeof(float)));
+  if (n) {
     ktimer_begin(c, "part_hitbox");
-    hipLaunchKernelGGL(k_part_hitbox, dim3((unsigned)((c.n_tris + 255) / 256)), dim3(256), 0,
-                       c.stream, c.tris.as<float>(), stride, c.n_tris, d_lo, d_hi, K,
-                       c.hitbox.as<uint64_t>());
+    hipLaunchKernelGGL(k_part_hitbox, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream,
+                       J.tris, stride, n, d_lo, d_hi, K, c.hitbox.as<uint64_t>());
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
   }
   // one order-preserving flag scan per partition; lists concatenated in partition order
-  SVO_TRY(ensure_scan_scratch(c, c.n_tris, 4));
-  std::vector<ull> totals(n_parts, 0);
-  SVO_TRY(ensure(c.part_lists, std::max<uint64_t>(c.n_tris, 1) * 4 * 2));
-  uint64_t used = 0;
-  ull* d_tot = c.counters.as<ull>() + 16;
+  SVO_TRY(ensure_scan_scratch(c, n, 4));
+  ull* d_tot = c.counters.as<ull>() + kCtrPartTotal;
   for (uint64_t j = 0; j < n_parts; j++) {
     // list capacity: grow if the previous partitions filled it
-    if ((used + c.n_tris) * 4 > c.part_lists.cap)
-      SVO_TRY(ensure_keep(c.part_lists, (used + c.n_tris) * 4 * 2, used * 4, c.stream));
+    if ((c.part_used + n) * 4 > c.part_lists.cap)
+      SVO_TRY(ensure_keep(c.part_lists, (c.part_used + n) * 4 * 2 + 256, c.part_used * 4,
+                          c.stream));
     SVO_TRY(fill_u64((uint64_t*)d_tot, 0, 1, c.stream));
-    uint64_t cx = 0, cy = 0, cz = 0;
-    {
-      uint32_t x, y, z;
-      unmorton3(j, x, y, z);
-      cx = x;
-      cy = y;
-      cz = z;
+    uint32_t cx, cy, cz;
+    unmorton3(j, cx, cy, cz);
+    if (n) {
+      PartLoad ld{c.hitbox.as<uint64_t>(), cx, cy, cz};
+      PartStore st{c.part_lists.as<uint32_t>() + c.part_used, d_tot, n};
+      SVO_TRY((scan_exclusive<uint32_t>(ld, st, n, c.scan_scratch.as<uint32_t>(), c.stream)));
     }
-    if (c.n_tris) {
-      PartLoad ld{c.hitbox.as<uint64_t>(), (uint32_t)cx, (uint32_t)cy, (uint32_t)cz};
-      PartStore st{c.part_lists.as<uint32_t>() + used, d_tot, c.n_tris};
-      SVO_TRY((scan_exclusive<uint32_t>(ld, st, c.n_tris, c.scan_scratch.as<uint32_t>(), c.stream)));
-    }
-    SVO_TRY(hipMemcpyAsync(&totals[j], d_tot, 8, hipMemcpyDeviceToHost, c.stream));
+    ull tot = 0;
+    SVO_TRY(hipMemcpyAsync(&tot, d_tot, 8, hipMemcpyDeviceToHost, c.stream));
     SVO_TRY(hipStreamSynchronize(c.stream));
-    c.part_off[j] = used;
-    c.part_cnt[j] = totals[j];
-    used += totals[j];
+    J.part_off[j] = c.part_used;
+    J.part_cnt[j] = tot;
+    c.part_used += tot;
   }
   return hipSuccess;
 }
 
-hipError_t voxelize_partition(Context& c, uint64_t part, uint64_t morton_start,
-                              uint64_t morton_end, float unitlength, uint64_t gridsize,
-                              int color_mode, uint64_t* nfilled) {
-  const uint64_t n = (c.n_parts > 1) ? c.part_cnt[part] : c.n_tris;
-  *nfilled = 0;
-  if (n == 0) return hipSuccess;
-  if (n > 0xFFFFFFF0ull) {
-    set_error("partition holds more than 2^32 triangles");
-    return hipErrorInvalidValue;
-  }
-  VoxParams P;
-  P.tris = c.tris.as<float>();
-  P.list = (c.n_parts > 1) ? c.part_lists.as<uint32_t>() + c.part_off[part] : nullptr;
-  P.n = (uint32_t)n;
-  P.stride = c.payload ? 21 : 9;
-  P.mstart = morton_start;
-  {
-    uint32_t x, y, z;
-    unmorton3(morton_start, x, y, z);
-    P.pmin[0] = (int)x;
-    P.pmin[1] = (int)y;
-    P.pmin[2] = (int)z;
-    unmorton3(morton_end - 1, x, y, z);
-    P.pmax[0] = (int)x;
-    P.pmax[1] = (int)y;
-    P.pmax[2] = (int)z;
-  }
-  P.unit = unitlength;
-  P.unit_div = 1.0f / unitlength;  // voxelizer.cpp:164, IEEE on the host as on the reference
-  P.coord_limit = 4.0f * (float)gridsize * unitlength;
+// The unit's brick-table segment covers its Morton range in bricks.
+static uint64_t unit_bricks(const VoxUnit& u) {
+  return (u.mend - (u.mstart & ~(uint64_t)(kBrickVox - 1)) + kBrickVox - 1) >> kBrickBits;
+}
+
+static void fill_params(const VoxUnit& h, const Context& c, VoxParams& P) {
+  P.tris = h.tris;
+  P.list = h.list;
+  P.n = (uint32_t)h.n;
+  P.stride = h.stride;
+  P.mstart = h.mstart;
+  P.bstart = h.mstart & ~(uint64_t)(kBrickVox - 1);
+  uint32_t x, y, z;
+  unmorton3(h.mstart, x, y, z);
+  P.pmin[0] = (int)x;
+  P.pmin[1] = (int)y;
+  P.pmin[2] = (int)z;
+  unmorton3(h.mend - 1, x, y, z);
+  P.pmax[0] = (int)x;
+  P.pmax[1] = (int)y;
+  P.pmax[2] = (int)z;
+  P.unit = h.unit;
+  P.unit_div = 1.0f / h.unit;  // voxelizer.cpp:164, IEEE on the host as on the reference
+  P.coord_limit = 4.0f * (float)h.gridsize * h.unit;
   // The reference's FP32 plane test (t + d1) * (t + d2) differs from its exact value by
   // less than ~3e-6 * coord_limit; along the column axis (|n_w| >= 1/2 here) that is
   // below 2.1e-5 * G voxels. The margin is 0.5 + 4e-5 * G voxels, at least twice that.
-  P.slab_margin = 0.5 + 4e-5 * (double)gridsize;
+  P.slab_margin = 0.5 + 4e-5 * (double)h.gridsize;
   P.small_cols = c.tune.small_cols;
+  P.job = h.job;
+  P.dense = 0;
+  P.table_base = 0;
+  P.grid8 = nullptr;
+}
 
-  const uint64_t span = morton_end - morton_start;
-  const uint64_t nbricks = (span + (1ull << kBrickBits) - 1) >> kBrickBits;
-  if (nbricks > 0xFFFFFFF0ull) {
-    set_error("partition too large for the brick table");
+hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch,
+                          uint64_t* nfilled) {
+  *nfilled = 0;
+  std::vector<VoxUnit> units;
+  for (const VoxUnit& u : hu)
+    if (u.n) units.push_back(u);  // main.cpp:330 skips empty partitions
+  const uint32_t nu = (uint32_t)units.size();
+  if (nu == 0) {
+    if (batch)
+      for (Job& J : c.jobs) {
+        J.voff = c.n_vox;
+        J.nvox = 0;
+      }
+    return hipSuccess;
+  }
+  // unit table: VoxParams[nu] | ubase[nu+1] | tile_off[nu+1] | uvoff[nu+1]
+  std::vector<VoxParams> P(nu);
+  std::vector<uint64_t> ubase(nu + 1);
+  std::vector<uint32_t> toff(nu + 1);
+  uint64_t ntab = 0, ntiles = 0, ntri = 0;
+  for (uint32_t u = 0; u < nu; u++) {
+    if (units[u].n > 0xFFFFFFF0ull) {
+      set_error("partition holds more than 2^32 triangles");
+      return hipErrorInvalidValue;
+    }
+    fill_params(units[u], c, P[u]);
+    P[u].table_base = ntab;
+    ubase[u] = ntab;
+    toff[u] = (uint32_t)ntiles;
+    ntab += unit_bricks(units[u]);
+    ntiles += (units[u].n + kTile - 1) / kTile;
+    ntri += units[u].n;
+  }
+  ubase[nu] = ntab;
+  toff[nu] = (uint32_t)ntiles;
+  if (ntab > 0xFFFFFFF0ull || ntiles > 0xFFFFFFF0ull || ntri > 0xFFFFFFF0ull) {
+    set_error("voxelize pass too large for 32-bit brick, tile or triangle indices");
     return hipErrorInvalidValue;
   }
-  // the table and the pool are all-zero whenever no pass is running
-  if (c.table_entries < nbricks) {
-    SVO_TRY(ensure(c.table, nbricks * 4));
-    SVO_TRY(fill_u32(c.table.as<uint32_t>(), 0, nbricks, c.stream));
-    c.table_entries = nbricks;
+  // dense path: a byte per voxel of every unit when that fits the HBM budget (no
+  // allocation pass needed); otherwise bricks get pool slots from a touch pass
+  const uint64_t dense_bytes = ntab << kBrickBits;
+  const bool dense = dense_bytes <= c.dense_limit;
+  // the table, the dense grid and the byte pool are all-zero whenever no pass is running
+  if (c.table_entries < ntab) {
+    SVO_TRY(ensure(c.table, ntab * 4));
+    SVO_TRY(fill_u32(c.table.as<uint32_t>(), 0, ntab, c.stream));
+    c.table_entries = ntab;
   }
-  if (c.pool_cap == 0) {
-    // lost install races leave unused slots behind, so the pool is not capped at nbricks
-    const uint64_t cap = std::max<uint64_t>(1024, n / 64);
+  if (dense) {
+    if (c.grid_zeroed < dense_bytes) {
+      SVO_TRY(ensure(c.grid8, dense_bytes));
+      SVO_TRY(fill_u64(c.grid8.as<uint64_t>(), 0, dense_bytes / 8, c.stream));
+      c.grid_zeroed = dense_bytes;
+    }
+    SVO_TRY(ensure(c.pool, ntab * kBrickWords * 8));
+    for (uint32_t u = 0; u < nu; u++) {
+      P[u].dense = 1;
+      P[u].grid8 = c.grid8.as<uint8_t>() + (ubase[u] << kBrickBits);
+    }
+  } else if (c.pool_cap == 0) {
+    const uint64_t cap = std::max<uint64_t>(1024, ntri / 64);
     SVO_TRY(ensure(c.pool, cap * kBrickWords * 8));
     SVO_TRY(ensure(c.pool8, cap * kBrickVox));
     SVO_TRY(fill_u64(c.pool8.as<uint64_t>(), 0, cap * kBrickVox / 8, c.stream));
     c.pool_cap = cap;
   }
-  SVO_TRY(ensure(c.large_tri, n * 4));
-  SVO_TRY(ensure(c.large_cols, n * 4));
-  SVO_TRY(ensure(c.large_off, n * 8 + 8));
-  SVO_TRY(ensure(c.occ_list, nbricks * 4));
-  SVO_TRY(ensure_scan_scratch(c, std::max<uint64_t>(n, nbricks), 8));
+  const size_t szP = ((nu * sizeof(VoxParams)) + 255) & ~(size_t)255;
+  const size_t szB = (((nu + 1) * 8) + 255) & ~(size_t)255;
+  const size_t szT = (((nu + 1) * 4) + 255) & ~(size_t)255;
+  SVO_TRY(ensure(c.unitdev, szP + 2 * szB + szT));
+  VoxParams* d_units = (VoxParams*)c.unitdev.p;
+  uint64_t* d_ubase = (uint64_t*)((char*)c.unitdev.p + szP);
+  uint32_t* d_toff = (uint32_t*)((char*)c.unitdev.p + szP + szB);
+  uint64_t* d_uvoff = (uint64_t*)((char*)c.unitdev.p + szP + szB + szT);
+  SVO_TRY(upload_async(c, d_units, P.data(), nu * sizeof(VoxParams)));
+  SVO_TRY(upload_async(c, d_ubase, ubase.data(), (nu + 1) * 8));
+  SVO_TRY(upload_async(c, d_toff, toff.data(), (nu + 1) * 4));
+
+  SVO_TRY(ensure(c.large_unit, ntri * 4));
+  SVO_TRY(ensure(c.large_tri, ntri * 4));
+  SVO_TRY(ensure(c.large_cols, ntri * 4));
+  SVO_TRY(ensure(c.large_off, ntri * 8 + 8));
+  SVO_TRY(ensure(c.occ_list, ntab * 4));
+  SVO_TRY(ensure_scan_scratch(c, std::max<uint64_t>(ntri, ntab), 8));
   ull* ctr = c.counters.as<ull>();
   ull hc[kNumCounters];
-  const unsigned nblk = (unsigned)((n + 255) / 256);
+  const unsigned tile_grid = (unsigned)std::min<uint64_t>(ntiles, 2048);
   const unsigned large_grid = 2048;
   auto brick_grid = [&](uint64_t bricks) {
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((bricks * kWave + 255) / 256, 8192));
   };
+  auto args = [&]() {
+    ActArgs A;
+    A.table = c.table.as<uint32_t>();
+    A.pool8 = c.pool8.as<uint8_t>();
+    A.pool = c.pool.as<ull>();
+    A.wprefix = c.wprefix.as<uint32_t>();
+    A.winner = c.winner.as<uint32_t>();
+    A.ctr = ctr;
+    A.pool_cap = c.pool_cap;
+    return A;
+  };
+  uint32_t* l_unit = c.large_unit.as<uint32_t>();
+  uint32_t* l_tri = c.large_tri.as<uint32_t>();
+  uint32_t* l_cols = c.large_cols.as<uint32_t>();
+  uint64_t* l_off = c.large_off.as<uint64_t>();
+  auto tile_pass = [&](auto tag, const char* name, ull* queue, int collect, int skip) {
+    using Act = decltype(tag);
+    ktimer_begin(c, name);
+    hipLaunchKernelGGL((k_vox_tile<Act>), dim3(tile_grid), dim3(kTile), 0, c.stream, d_units, nu,
+                       d_toff, args(), queue, l_unit, l_tri, l_cols, collect, skip);
+    ktimer_end(c);
+    return hipGetLastError();
+  };
+  auto large_pass = [&](auto tag, const char* name, int skip) {
+    using Act = decltype(tag);
+    ktimer_begin(c, name);
+    hipLaunchKernelGGL((k_vox_large<Act>), dim3(large_grid), dim3(256), 0, c.stream, d_units,
+                       args(), l_unit, l_tri, l_off, skip);
+    ktimer_end(c);
+    return hipGetLastError();
+  };
+  auto occ_scan = [&]() {
+    return scan_exclusive<uint32_t>(
+        OccLoad{c.table.as<uint32_t>()},
+        OccStore{c.occ_list.as<uint32_t>(), c.table.as<uint32_t>(), ctr, ntab}, ntab,
+        c.scan_scratch.as<uint32_t>(), c.stream);
+  };
 
   // ---- phase A: mark, compact counts; one synchronization at its end
+  std::vector<uint64_t> uvoff(nu + 1);
   for (int attempt = 0;; attempt++) {
-    const uint64_t bc_bound = std::min<uint64_t>(nbricks, c.pool_cap);
+    const uint64_t bc_bound = dense ? ntab : std::min<uint64_t>(ntab, c.pool_cap);
     SVO_TRY(ensure(c.bcount, bc_bound * 8 + 8));
-    SVO_TRY(fill_u64((uint64_t*)ctr, 0, 7, c.stream));  // counters 0..6
-    // touch pass: flag the bricks of every candidate voxel, collect the large triangles
-    TouchAction touch{c.table.as<uint32_t>(), morton_start >> kBrickBits};
-    ktimer_begin(c, "vox_small_touch");
-    hipLaunchKernelGGL((k_vox_tile<TouchAction>), dim3(nblk), dim3(kTile), 0, c.stream, P, touch,
-                       c.large_tri.as<uint32_t>(), c.large_cols.as<uint32_t>(), ctr, 1);
-    ktimer_end(c);
-    SVO_TRY(hipGetLastError());
+    SVO_TRY(fill_u64((uint64_t*)ctr, 0, kCtrPassClear, c.stream));
+    // first pass: dense units mark directly, touch units flag their bricks; both collect
+    // the large triangles
+    if (dense)
+      SVO_TRY(tile_pass(MarkAction{}, "vox_small_mark", ctr + kCtrQueueA, 1, 0));
+    else
+      SVO_TRY(tile_pass(TouchAction{}, "vox_small_touch", ctr + kCtrQueueA, 1, 0));
     SVO_TRY((scan_exclusive<uint64_t>(
-        ColLoad{c.large_cols.as<uint32_t>()},
-        CountStore{c.large_off.as<uint64_t>(), ctr + kCtrCols, ctr + kCtrLargeCount}, n,
+        ColLoad{l_cols}, CountStore{l_off, ctr + kCtrCols, ctr + kCtrLargeCount}, ntri,
         c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrLargeCount)));
-    ktimer_begin(c, "vox_large_touch");
-    hipLaunchKernelGGL((k_vox_large<TouchAction>), dim3(large_grid), dim3(256), 0, c.stream, P,
-                       touch, c.large_tri.as<uint32_t>(), c.large_off.as<uint64_t>(), ctr);
-    ktimer_end(c);
-    SVO_TRY(hipGetLastError());
-    // number the flagged bricks in Morton order: table entry = slot + 1, occ = brick list
-    SVO_TRY((scan_exclusive<uint32_t>(
-        OccLoad{c.table.as<uint32_t>()},
-        OccStore{c.occ_list.as<uint32_t>(), c.table.as<uint32_t>(), ctr, nbricks}, nbricks,
-        c.scan_scratch.as<uint32_t>(), c.stream)));
-    // mark pass: byte stores into the slots
-    MarkAction mark{c.table.as<uint32_t>(), c.pool8.as<uint8_t>(), ctr, c.pool_cap, morton_start};
-    ktimer_begin(c, "vox_small_mark");
-    hipLaunchKernelGGL((k_vox_tile<MarkAction>), dim3(nblk), dim3(kTile), 0, c.stream, P, mark,
-                       c.large_tri.as<uint32_t>(), c.large_cols.as<uint32_t>(), ctr, 0);
-    ktimer_end(c);
-    SVO_TRY(hipGetLastError());
-    ktimer_begin(c, "vox_large_mark");
-    hipLaunchKernelGGL((k_vox_large<MarkAction>), dim3(large_grid), dim3(256), 0, c.stream, P,
-                       mark, c.large_tri.as<uint32_t>(), c.large_off.as<uint64_t>(), ctr);
-    ktimer_end(c);
-    SVO_TRY(hipGetLastError());
-    // per-brick voxel counts (bytes packed to bits) and their offsets
+    if (dense) {
+      SVO_TRY(large_pass(MarkAction{}, "vox_large_mark", 0));
+      // number the flagged bricks in Morton order: table entry = rank + 1, occ = list
+      SVO_TRY(occ_scan());
+    } else {
+      SVO_TRY(large_pass(TouchAction{}, "vox_large_touch", 0));
+      SVO_TRY(occ_scan());
+      // mark pass: byte stores into the slots
+      SVO_TRY(tile_pass(MarkAction{}, "vox_small_mark", ctr + kCtrQueueB, 0, 0));
+      SVO_TRY(large_pass(MarkAction{}, "vox_large_mark", 0));
+    }
+    // per-brick voxel counts (bytes packed to bits, bytes cleared) and their offsets
     ktimer_begin(c, "brick_count");
     hipLaunchKernelGGL(k_brick_count, dim3(brick_grid(bc_bound)), dim3(256), 0, c.stream,
-                       c.occ_list.as<uint32_t>(), ctr, c.pool_cap, c.table.as<uint32_t>(),
+                       c.occ_list.as<uint32_t>(), ctr, bc_bound, d_units, d_ubase, nu,
                        c.pool8.as<uint8_t>(), c.pool.as<ull>(), c.bcount.as<uint64_t>());
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
@@ -998,79 +1213,129 @@ hipError_t voxelize_partition(Context& c, uint64_t part, uint64_t morton_start,
         ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
         CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox, ctr + kCtrNumOcc}, bc_bound,
         c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrNumOcc)));
+    hipLaunchKernelGGL(k_unit_voxels, dim3((nu + 256) / 256), dim3(256), 0, c.stream,
+                       c.occ_list.as<uint32_t>(), ctr, d_ubase, nu, c.bcount.as<uint64_t>(),
+                       d_uvoff);
+    SVO_TRY(hipGetLastError());
+    SVO_TRY(hipMemcpyAsync(uvoff.data(), d_uvoff, (nu + 1) * 8, hipMemcpyDeviceToHost,
+                           c.stream));
     SVO_TRY(read_counters(c, hc));
     ktimer_flush(c);
-    if (!hc[kCtrOverflow] && hc[kCtrNumOcc] <= c.pool_cap) break;
+    if (dense || (!hc[kCtrOverflow] && hc[kCtrNumOcc] <= c.pool_cap)) break;
     // pool too small for the flagged bricks: clear everything, grow, redo
     const uint64_t want = std::max<uint64_t>(c.pool_cap * 2, hc[kCtrNumOcc] + 1024);
-    SVO_TRY(fill_u32(c.table.as<uint32_t>(), 0, nbricks, c.stream));
+    if (attempt > 8 || want > (1ull << 26)) {
+      set_error("brick pool could not be sized");
+      return hipErrorOutOfMemory;
+    }
+    SVO_TRY(fill_u32(c.table.as<uint32_t>(), 0, ntab, c.stream));
     release(c.pool);
     release(c.pool8);
     SVO_TRY(ensure(c.pool, want * kBrickWords * 8));
     SVO_TRY(ensure(c.pool8, want * kBrickVox));
     SVO_TRY(fill_u64(c.pool8.as<uint64_t>(), 0, want * kBrickVox / 8, c.stream));
     c.pool_cap = want;
-    if (attempt > 8 || want > (1ull << 26)) {
-      set_error("brick pool could not be sized");
-      return hipErrorOutOfMemory;
-    }
   }
   const uint64_t n_occ = hc[kCtrNumOcc];
   const uint64_t nvox = hc[kCtrNumVox];
   const uint64_t n_large = hc[kCtrLargeCount];
+  if (c.n_vox + nvox >= (1ull << 40)) {
+    set_error("voxel stream too long");
+    return hipErrorInvalidValue;
+  }
+
+  // ---- job layout: voxel and data-record ranges
+  const uint64_t s0 = c.n_vox;
+  bool any_payload = false;
+  if (batch) {
+    // the units hold every job's partitions, in job order: ranges follow from the unit
+    // offsets; a job without voxels gets an empty range where it would start
+    for (Job& J : c.jobs) {
+      J.nvox = 0;
+      J.voff = ~0ull;
+    }
+    for (uint32_t u = 0; u < nu; u++) {
+      Job& J = c.jobs[units[u].job];
+      if (J.voff == ~0ull) J.voff = s0 + uvoff[u];
+      J.nvox += uvoff[u + 1] - uvoff[u];
+    }
+    uint64_t prev_end = s0;
+    for (Job& J : c.jobs) {
+      if (J.voff == ~0ull) J.voff = prev_end;
+      prev_end = J.voff + J.nvox;
+    }
+  } else {
+    Job& J = c.jobs[0];
+    J.voff = 0;
+    J.nvox = s0 + nvox;
+  }
+  uint64_t n_records = 0;
+  for (size_t j = 0; j < c.jobs.size(); j++) {
+    Job& J = c.jobs[j];
+    J.dbase = n_records;
+    J.n_data = J.payload ? 1 + J.nvox : 2;
+    n_records += J.n_data;
+    any_payload |= J.payload != 0;
+  }
+  std::vector<JobDev> jd(c.jobs.size());
+  for (size_t j = 0; j < c.jobs.size(); j++) {
+    const Job& J = c.jobs[j];
+    JobDev& d = jd[j];
+    memset(&d, 0, sizeof(d));
+    d.tris = J.tris;
+    d.unit_div = 1.0f / J.unit_vox;
+    d.color_mode = J.color_mode;
+    d.payload = J.payload;
+    d.gridsize = J.gridsize;
+    int D = 0;
+    while ((1ull << D) < J.gridsize) D++;
+    d.depth = D;
+    d.voff = J.voff;
+    d.nvox = J.nvox;
+    d.dbase = J.dbase;
+  }
+  SVO_TRY(ensure(c.jobdev, jd.size() * sizeof(JobDev)));
+  SVO_TRY(upload_async(c, c.jobdev.p, jd.data(), jd.size() * sizeof(JobDev)));
 
   // ---- phase B: emit sorted codes, payload, clear; no synchronization
   if (n_occ) {
-    if (c.payload) SVO_TRY(ensure(c.wprefix, c.pool_cap * kBrickWords * 4));
-    SVO_TRY(ensure_keep(c.codes, (c.n_vox + nvox) * 8, c.n_vox * 8, c.stream));
+    if (any_payload) SVO_TRY(ensure(c.wprefix, n_occ * kBrickWords * 4));
+    SVO_TRY(ensure_keep(c.codes, (s0 + nvox) * 8, s0 * 8, c.stream));
     ktimer_begin(c, "brick_emit");
     hipLaunchKernelGGL(k_brick_emit, dim3(brick_grid(n_occ)), dim3(256), 0, c.stream,
-                       c.occ_list.as<uint32_t>(), ctr, c.table.as<uint32_t>(), c.pool.as<ull>(),
-                       c.bcount.as<uint64_t>(), morton_start, c.codes.as<uint64_t>() + c.n_vox,
-                       c.payload ? c.wprefix.as<uint32_t>() : (uint32_t*)nullptr);
+                       c.occ_list.as<uint32_t>(), ctr, d_units, d_ubase, nu, c.pool.as<ull>(),
+                       c.bcount.as<uint64_t>(), c.codes.as<uint64_t>() + s0,
+                       any_payload ? c.wprefix.as<uint32_t>() : (uint32_t*)nullptr);
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
 
-    if (c.payload) {
+    if (any_payload) {
       if (nvox > 0xFFFFFFF0ull) {
-        set_error("partition holds more than 2^32 voxels");
+        set_error("voxelize pass holds more than 2^32 voxels");
         return hipErrorInvalidValue;
       }
       SVO_TRY(ensure(c.winner, nvox * 4));
       SVO_TRY(fill_u32(c.winner.as<uint32_t>(), 0xFFFFFFFFu, nvox, c.stream));
-      WinnerAction win{c.table.as<uint32_t>(), c.pool.as<ull>(), c.wprefix.as<uint32_t>(),
-                       c.winner.as<uint32_t>(), ctr, morton_start};
-      ktimer_begin(c, "vox_small_winner");
-      hipLaunchKernelGGL((k_vox_tile<WinnerAction>), dim3(nblk), dim3(kTile), 0, c.stream, P,
-                         win, c.large_tri.as<uint32_t>(), c.large_cols.as<uint32_t>(), ctr, 0);
-      ktimer_end(c);
-      SVO_TRY(hipGetLastError());
-      if (n_large) {
-        ktimer_begin(c, "vox_large_winner");
-        hipLaunchKernelGGL((k_vox_large<WinnerAction>), dim3(large_grid), dim3(256), 0,
-                           c.stream, P, win, c.large_tri.as<uint32_t>(),
-                           c.large_off.as<uint64_t>(), ctr);
-        ktimer_end(c);
-        SVO_TRY(hipGetLastError());
-      }
-      // records: slot 0 of c.records is the null record
-      SVO_TRY(ensure_keep(c.records, (1 + c.n_vox + nvox) * 32, (1 + c.n_vox) * 32, c.stream));
+      SVO_TRY(tile_pass(WinnerAction{}, "vox_small_winner", ctr + kCtrQueueC, 0, kSkipBinary));
+      if (n_large) SVO_TRY(large_pass(WinnerAction{}, "vox_large_winner", kSkipBinary));
+      // records: each job's first record is its null record
+      const uint64_t keep = batch ? 0 : (1 + s0) * 32;
+      SVO_TRY(ensure_keep(c.records, n_records * 32, keep, c.stream));
       ktimer_begin(c, "payload_records");
       hipLaunchKernelGGL(k_payload_records, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
-                         c.stream, c.tris.as<float>(), c.codes.as<uint64_t>() + c.n_vox,
-                         c.winner.as<uint32_t>(), nvox, P.unit_div, color_mode, gridsize,
-                         (float*)((char*)c.records.p + (1 + c.n_vox) * 32), ctr);
+                         c.stream, c.jobdev.as<JobDev>(), c.codes.as<uint64_t>(), s0,
+                         c.winner.as<uint32_t>(), nvox, c.records.as<float>(), ctr);
       ktimer_end(c);
       SVO_TRY(hipGetLastError());
     }
     ktimer_begin(c, "brick_clear");
-    hipLaunchKernelGGL(k_brick_clear, dim3(brick_grid(n_occ)), dim3(256), 0, c.stream,
-                       c.occ_list.as<uint32_t>(), ctr, c.table.as<uint32_t>(),
-                       c.pool8.as<uint8_t>());
+    hipLaunchKernelGGL(k_brick_clear, dim3((unsigned)std::min<uint64_t>((n_occ + 255) / 256, 4096)),
+                       dim3(256), 0, c.stream, c.occ_list.as<uint32_t>(), ctr,
+                       c.table.as<uint32_t>());
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
   }
-  c.n_vox += nvox;
+  c.n_vox = s0 + nvox;
   *nfilled = nvox;
   return hipSuccess;
 }

@@ -52,16 +52,27 @@ def _subdivide(v, f):
 
 
 def icosphere(subdivisions: int = 5, radius: float = 0.4, center=(0.5, 0.5, 0.5),
-              noise: float = 0.02, seed: int = 1) -> np.ndarray:
+              noise: float = 0.02, seed: int = 1, payload: bool = False) -> np.ndarray:
     """Watertight icosphere; each shared vertex is displaced radially once by
-    U(-noise, noise). Returns (N, 9) float32 binary records."""
+    U(-noise, noise). Returns (N, 9) float32 binary records, or (N, 21) payload records
+    (vertex normal = radial direction, per-vertex RGB U[0,1])."""
     v, f = _icosahedron()
     for _ in range(subdivisions):
         v, f = _subdivide(v, f)
     rng = np.random.default_rng(seed)
     r = radius + rng.uniform(-noise, noise, size=len(v))
     p = (v * r[:, None] + np.asarray(center, dtype=np.float64)).astype(np.float32)
-    return p[f].reshape(len(f), 9)
+    geo = p[f].reshape(len(f), 9)
+    if not payload:
+        return geo
+    rec = np.empty((len(f), 21), dtype=np.float32)
+    rec[:, 0:9] = geo
+    m = v[f].sum(axis=1) / 3.0
+    m /= np.linalg.norm(m, axis=1, keepdims=True)
+    rec[:, 9:12] = m.astype(np.float32)
+    C = rng.uniform(0.0, 1.0, size=(len(v), 3)).astype(np.float32)
+    rec[:, 12:21] = C[f].reshape(len(f), 9)
+    return rec
 
 
 # ----------------------------------------------------------------------------
@@ -71,7 +82,7 @@ def icosphere(subdivisions: int = 5, radius: float = 0.4, center=(0.5, 0.5, 0.5)
 
 def _hash_u32(ix, iy, seed):
     h = (ix.astype(np.uint64) * np.uint64(0x8DA6B343) ^ iy.astype(np.uint64) * np.uint64(0xD8163841)
-         ^ np.uint64(seed) * np.uint64(0xCB1AB31F)) & np.uint64(0xFFFFFFFF)
+         ^ np.uint64(seed & 0xFFFFFFFF) * np.uint64(0xCB1AB31F)) & np.uint64(0xFFFFFFFF)
     h ^= h >> np.uint64(15)
     h = (h * np.uint64(0x2C1B3C6D)) & np.uint64(0xFFFFFFFF)
     h ^= h >> np.uint64(12)
@@ -194,3 +205,63 @@ def sphere_soup(n_tris: int, n_spheres: int = 200, seed: int = 3,
             v = ctr + rad[:, None] * (np.cos(ang)[:, None] * t1 + np.sin(ang)[:, None] * t2)
             out[s:s + m, 3 * k:3 * k + 3] = v.astype(np.float32)
     return out
+
+
+# ----------------------------------------------------------------------------
+# job mixes for the batched and served forms
+# ----------------------------------------------------------------------------
+
+MIX_GRIDS = (64, 128, 256, 512, 1024)
+
+
+def splitmix64(seed: int, n: int) -> list:
+    """n successive SplitMix64 outputs from 'seed' (the per-mesh seeds of a mix)."""
+    out = []
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    for _ in range(n):
+        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        out.append(z ^ (z >> 31))
+    return out
+
+
+def mix_spec(n_jobs: int = 512, seed: int = 4, n_min: int = 1000, n_max: int = 2_000_000):
+    """Job shapes of a mix: per mesh (kind, size parameter, payload, gridsize, seed).
+    50/50 icosphere/terrain, triangle count log-uniform in [n_min, n_max] (the icosphere
+    subdivision or terrain resolution nearest to it), gridsize uniform over MIX_GRIDS,
+    30% payload."""
+    spec = []
+    for s in splitmix64(seed, n_jobs):
+        rng = np.random.default_rng(s)
+        kind = "icosphere" if rng.uniform() < 0.5 else "terrain"
+        n_tri = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        if kind == "icosphere":  # 20 * 4**k triangles
+            size = int(np.clip(np.round(np.log(n_tri / 20.0) / np.log(4.0)), 0, 8))
+        else:  # 2 * (n - 1)**2 triangles
+            size = int(max(2, np.round(np.sqrt(n_tri / 2.0)) + 1))
+        payload = bool(rng.uniform() < 0.3)
+        G = int(MIX_GRIDS[rng.integers(0, len(MIX_GRIDS))])
+        spec.append((kind, size, payload, G, int(s & 0x7FFFFFFF)))
+    return spec
+
+
+def make_job_mesh(spec_item) -> np.ndarray:
+    kind, size, payload, G, s = spec_item
+    if kind == "icosphere":
+        return icosphere(subdivisions=size, seed=s, payload=payload)
+    return terrain(n=size, seed=s, payload=payload)
+
+
+def job_mix(n_jobs: int = 512, seed: int = 4, workers: int = 1, **kw) -> list:
+    """The mix as job dicts (records, payload, gridsize), meshes built with 'workers'
+    processes."""
+    spec = mix_spec(n_jobs, seed, **kw)
+    if workers > 1:
+        from concurrent.futures import ProcessPoolExecutor
+        with ProcessPoolExecutor(workers) as ex:
+            meshes = list(ex.map(make_job_mesh, spec, chunksize=4))
+    else:
+        meshes = [make_job_mesh(x) for x in spec]
+    return [dict(records=m, payload=x[2], gridsize=x[3]) for m, x in zip(meshes, spec)]

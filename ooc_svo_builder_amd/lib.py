@@ -37,6 +37,27 @@ class RunInfo(C.Structure):
                 ("ms_build", C.c_float)]
 
 
+class Job(C.Structure):
+    """svo_job: one mesh of a batch."""
+    _fields_ = [("records", C.c_void_p), ("n_triangles", C.c_uint64), ("payload", C.c_int),
+                ("src_on_device", C.c_int), ("bbox", C.c_float * 6), ("gridsize", C.c_uint64),
+                ("memory_limit_mb", C.c_uint64), ("color_mode", C.c_int),
+                ("generate_levels", C.c_int)]
+
+
+class JobResult(C.Structure):
+    """svo_job_result: a batch job's counts and its ranges in the batch's arrays."""
+    _fields_ = [("n_partitions", C.c_uint64), ("nfilled", C.c_uint64),
+                ("n_nodes", C.c_uint64), ("n_data", C.c_uint64),
+                ("node_offset", C.c_uint64), ("data_offset", C.c_uint64), ("hazard", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("batches", C.c_uint64), ("max_batch_jobs_seen", C.c_uint64),
+                ("peak_admitted_bytes", C.c_uint64)]
+
+
 SIGNATURES = {
     "svo_last_error": (C.c_char_p, []),
     "svo_version": (C.c_char_p, []),
@@ -44,6 +65,7 @@ SIGNATURES = {
     "svo_ctx_destroy": (None, [_vp]),
     "svo_ctx_set_stream": (_i32, [_vp, _vp]),
     "svo_ctx_stream": (_vp, [_vp]),
+    "svo_ctx_set_tuning": (_i32, [_vp, _i32, _u64]),
     "svo_estimate_partitions": (_u64, [_u64, _u64]),
     "svo_mesh_load": (_i32, [_vp, _vp, _u64, _i32, _i32]),
     "svo_partition": (_i32, [_vp, _f32, _u64, _u64, _vp]),
@@ -62,6 +84,19 @@ SIGNATURES = {
     "svo_kernel_timing_reset": (_i32, [_vp]),
     "svo_run": (_i32, [_vp, _vp, _u64, _i32, _i32, _vp, _u64, _u64, _i32, _i32,
                        C.POINTER(RunInfo)]),
+    "svo_batch_run": (_i32, [_vp, C.POINTER(Job), C.c_uint32, C.POINTER(JobResult),
+                             C.POINTER(RunInfo)]),
+    "svo_server_create": (_i32, [_i32, _u64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
+    "svo_server_destroy": (None, [_vp]),
+    "svo_server_submit": (_i32, [_vp, C.POINTER(Job), C.POINTER(_u64)]),
+    "svo_server_poll": (_i32, [_vp, _u64, C.POINTER(_i32)]),
+    "svo_server_wait": (_i32, [_vp, _u64, C.POINTER(JobResult), C.POINTER(C.c_double)]),
+    "svo_server_copy_nodes": (_i32, [_vp, _u64, _vp, _u64, _u64]),
+    "svo_server_copy_data": (_i32, [_vp, _u64, _vp, _u64, _u64]),
+    "svo_server_nodes_device": (_vp, [_vp, _u64]),
+    "svo_server_data_device": (_vp, [_vp, _u64]),
+    "svo_server_release": (_i32, [_vp, _u64]),
+    "svo_server_stats_get": (_i32, [_vp, C.POINTER(ServerStats)]),
 }
 
 _lib = None
@@ -102,6 +137,101 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def fill_job(q: Job, j: dict):
+    """Fill an svo_job from a dict with keys records ((N, 9|21) float32 numpy array, or a
+    device tensor), payload, gridsize and optionally memory_limit_mb (2048), bbox
+    ((0,0,0,1,1,1)), color_mode (0) and levels (False). Returns the object that must stay
+    alive while the job's records are read."""
+    r = j["records"]
+    if hasattr(r, "data_ptr"):
+        q.records = r.data_ptr()
+        q.n_triangles = int(r.shape[0])
+        q.src_on_device = 1 if r.is_cuda else 0
+        keep = r
+    else:
+        keep = np.ascontiguousarray(r, np.float32)
+        q.records = keep.ctypes.data
+        q.n_triangles = keep.shape[0]
+        q.src_on_device = 0
+    q.payload = int(bool(j["payload"]))
+    for k, v in enumerate(j.get("bbox", (0, 0, 0, 1, 1, 1))):
+        q.bbox[k] = float(v)
+    q.gridsize = int(j["gridsize"])
+    q.memory_limit_mb = int(j.get("memory_limit_mb", 2048))
+    q.color_mode = int(j.get("color_mode", 0))
+    q.generate_levels = int(bool(j.get("levels", False)))
+    return keep
+
+
+class SvoServer:
+    """Served form: submit jobs as they arrive; results stay in HBM per ticket."""
+
+    def __init__(self, device: int = 0, budget_mb: int = 0, max_batch: int = 0, slots: int = 0):
+        L = load_library()
+        h = C.c_void_p()
+        _check(L.svo_server_create(device, budget_mb, max_batch, slots, C.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def close(self):
+        if self._h:
+            load_library().svo_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, job: dict) -> int:
+        q = Job()
+        keep = fill_job(q, job)
+        t = C.c_uint64(0)
+        _check(load_library().svo_server_submit(self._h, C.byref(q), C.byref(t)))
+        self._keep[t.value] = keep
+        return int(t.value)
+
+    def poll(self, ticket: int) -> int:
+        st = C.c_int(0)
+        _check(load_library().svo_server_poll(self._h, ticket, C.byref(st)))
+        return int(st.value)
+
+    def wait(self, ticket: int):
+        """(JobResult, latency in ms) once the job is done; raises if it failed."""
+        r = JobResult()
+        lat = C.c_double(0)
+        rc = load_library().svo_server_wait(self._h, ticket, C.byref(r), C.byref(lat))
+        self._keep.pop(ticket, None)
+        _check(rc)
+        return r, float(lat.value)
+
+    def nodes(self, ticket: int, n: int) -> np.ndarray:
+        out = np.empty(n, NODE_DTYPE)
+        _check(load_library().svo_server_copy_nodes(self._h, ticket, _ptr(out), 0, n))
+        return out
+
+    def data(self, ticket: int, n: int) -> np.ndarray:
+        out = np.empty(n, DATA_DTYPE)
+        _check(load_library().svo_server_copy_data(self._h, ticket, _ptr(out), 0, n))
+        return out
+
+    def copy_nodes_to(self, ticket: int, dst_ptr: int, n: int):
+        _check(load_library().svo_server_copy_nodes(self._h, ticket, C.c_void_p(dst_ptr), 0, n))
+
+    def copy_data_to(self, ticket: int, dst_ptr: int, n: int):
+        _check(load_library().svo_server_copy_data(self._h, ticket, C.c_void_p(dst_ptr), 0, n))
+
+    def release(self, ticket: int):
+        _check(load_library().svo_server_release(self._h, ticket))
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(load_library().svo_server_stats_get(self._h, C.byref(st)))
+        return st
+
+
 class SvoContext:
     """One device context: mesh, partitions, voxel stream and built octree in HBM."""
 
@@ -135,6 +265,11 @@ class SvoContext:
 
     def set_stream(self, stream_ptr: int | None):
         _check(load_library().svo_ctx_set_stream(self._h, C.c_void_p(stream_ptr or 0)))
+
+    def set_tuning(self, small_cols: int = 0, dense_limit_mb: int | None = None):
+        """Voxelizer split threshold and dense-grid limit (output is unchanged)."""
+        lim = 0xFFFFFFFFFFFFFFFF if dense_limit_mb is None else int(dense_limit_mb)
+        _check(load_library().svo_ctx_set_tuning(self._h, int(small_cols), lim))
 
     # -- stages -----------------------------------------------------------------
     def load_mesh(self, records, payload: bool):
@@ -208,6 +343,35 @@ class SvoContext:
                                       memory_limit_mb, color_mode, int(levels), C.byref(info)))
         self.last_info = info
         return info
+
+    def batch_run(self, jobs) -> tuple[list, RunInfo]:
+        """Batched form: jobs is a list of dicts (see fill_job). Returns (per-job JobResult
+        list, totals)."""
+        n = len(jobs)
+        arr = (Job * n)()
+        keep = [fill_job(arr[i], j) for i, j in enumerate(jobs)]
+        res = (JobResult * n)()
+        info = RunInfo()
+        _check(load_library().svo_batch_run(self._h, arr, n, res, C.byref(info)))
+        self.last_info = info
+        return list(res), info
+
+    def nodes_range(self, first: int, count: int) -> np.ndarray:
+        out = np.empty(count, NODE_DTYPE)
+        _check(load_library().svo_copy_nodes(self._h, _ptr(out), first, count))
+        return out
+
+    def copy_nodes_to(self, dst_ptr: int, first: int, count: int):
+        """Node records [first, first+count) into device (or host) memory at dst_ptr."""
+        _check(load_library().svo_copy_nodes(self._h, C.c_void_p(dst_ptr), first, count))
+
+    def copy_data_to(self, dst_ptr: int, first: int, count: int):
+        _check(load_library().svo_copy_data(self._h, C.c_void_p(dst_ptr), first, count))
+
+    def data_range(self, first: int, count: int) -> np.ndarray:
+        out = np.empty(count, DATA_DTYPE)
+        _check(load_library().svo_copy_data(self._h, _ptr(out), first, count))
+        return out
 
     def set_kernel_timing(self, on: bool):
         _check(load_library().svo_ctx_set_kernel_timing(self._h, int(on)))

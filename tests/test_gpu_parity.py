@@ -176,9 +176,10 @@ def test_builder_payload_and_hazard(gpu_ctx):
         assert gpu_ctx.data(nd).tobytes() == data_w.tobytes()
 
 
-def _run_both(ctx, tris, payload, G, L, color_mode=0):
-    info = ctx.run(tris, payload, G, L, color_mode=color_mode)
-    nodes_w, data_w, hz_w, np_w, nf_w = oracle.pipeline(tris, payload, G, L, color_mode=color_mode)
+def _run_both(ctx, tris, payload, G, L, color_mode=0, levels=False):
+    info = ctx.run(tris, payload, G, L, color_mode=color_mode, levels=levels)
+    nodes_w, data_w, hz_w, np_w, nf_w = oracle.pipeline(tris, payload, G, L, color_mode=color_mode,
+                                                        levels=levels)
     assert info.n_partitions == np_w
     assert info.nfilled == nf_w
     assert info.hazard == hz_w
@@ -226,13 +227,39 @@ def test_empty_and_outside(gpu_ctx):
     _run_both(gpu_ctx, far, False, 64, 2048)
 
 
+@pytest.mark.parametrize("payload,G,L,mode", [(False, 128, 2048, 0), (True, 128, 2048, 0),
+                                               (True, 64, 2048, 3), (True, 256, 2, 2),
+                                               (False, 256, 2, 0)])
+def test_pipeline_levels(gpu_ctx, payload, G, L, mode):
+    """-levels (OctreeBuilder.cpp:82-99): averaged records of internal nodes, every data
+    record in post-order; geometry-only builds write NaN normals (zero leaf caches)."""
+    _say("pipeline levels payload=%d G=%d L=%d mode=%d" % (payload, G, L, mode))
+    if payload:
+        tris = generators.terrain(n=57, seed=8, payload=True)
+    else:
+        tris = generators.icosphere(subdivisions=3, radius=0.3)
+    _run_both(gpu_ctx, tris, payload, G, L, color_mode=mode, levels=True)
+
+
+def test_levels_small_grids(gpu_ctx):
+    """-levels on the degenerate grids: empty, 1^3 and 2^3."""
+    _say("levels small grids")
+    t = generators.terrain(n=9, seed=9, payload=True)
+    for G in (1, 2, 4):
+        _run_both(gpu_ctx, t, True, G, 2048, levels=True)
+        _run_both(gpu_ctx, t[:, :9].copy(), False, G, 2048, levels=True)
+    _run_both(gpu_ctx, np.zeros((0, 21), np.float32), True, 64, 2048, levels=True)
+
+
 def test_cli_files_identical(tmp_path):
     """Drop-in CLI end to end: GPU svo_builder(_binary) and the oracle CLI write
     byte-identical .octree/.octreenodes/.octreedata."""
     _say("cli files identical")
     cases = [("ico", generators.icosphere(subdivisions=4), False, ["-s", "128", "-l", "2048"]),
              ("ter", generators.terrain(n=49, seed=3, payload=True), True, ["-s", "64", "-c", "linear"]),
-             ("part", generators.icosphere(subdivisions=3), False, ["-s", "256", "-l", "2"])]
+             ("part", generators.icosphere(subdivisions=3), False, ["-s", "256", "-l", "2"]),
+             ("lvl", generators.terrain(n=33, seed=4, payload=True), True,
+              ["-s", "64", "-l", "2048", "-levels"])]
     for name, tris, payload, args in cases:
         for who in ("gpu", "ref"):
             d = tmp_path / who
@@ -248,3 +275,18 @@ def test_cli_files_identical(tmp_path):
         P = pkg.estimate_partitions(int(size), int(args[3]) if args[2] == "-l" else 2048)
         base = "%s%s_%d" % (name, size, P)
         assert files_identical(str(tmp_path / "gpu" / base), str(tmp_path / "ref" / base)), name
+
+
+@pytest.mark.parametrize("small_cols,dense_mb", [(1024, None), (4, None), (1024, 0), (16, 0)])
+def test_voxelizer_paths_identical(small_cols, dense_mb):
+    """Dense byte grid and touched-brick pool, small and large tier thresholds: the same
+    bytes as the oracle (these knobs change the work split, never the output)."""
+    _say("voxelizer paths small_cols=%d dense_mb=%s" % (small_cols, dense_mb))
+    rng = np.random.default_rng(123)
+    tris = np.concatenate([generators.icosphere(subdivisions=4),
+                           adversarial_soup(rng, 500, 128)])
+    with SvoContext(0) as ctx:
+        ctx.set_tuning(small_cols=small_cols, dense_limit_mb=dense_mb)
+        for G, L in ((128, 2048), (256, 2)):
+            _run_both(ctx, tris, False, G, L)
+        _run_both(ctx, generators.terrain(n=49, seed=8, payload=True), True, 128, 2048)

@@ -146,3 +146,98 @@ def test_single_voxel_grid():
     assert len(nodes) == 1 and nodes[0]["data"] == 1 and hz == 0
     nodes, data, hz = oracle.build_octree(np.zeros(0, np.uint64), 1, False)
     assert len(nodes) == 1 and hz == 1  # the reference reads an empty buffer here
+
+
+def _post_order(nodes, D):
+    """(record index, level) of every node below the root, children in child order, each
+    node after its subtree (the order in which the streaming builder completes nodes)."""
+    out = []
+
+    def visit(idx, level):
+        n = nodes[idx]
+        if level < D:
+            for k in range(8):
+                if n["offsets"][k] >= 0:
+                    visit(int(n["children_base"]) + int(n["offsets"][k]), level + 1)
+        out.append((idx, level))
+    visit(len(nodes) - 1, 0)
+    return out
+
+
+def _f32_average(kids):
+    """OctreeBuilder.cpp:82-99 in float32: the 8 child caches summed in child order (None =
+    null child, adds zeros), divided by the non-null count, the normal normalized."""
+    f = np.float32
+    c = [f(0), f(0), f(0)]
+    nrm = [f(0), f(0), f(0)]
+    notnull = f(0)
+    for kid in kids:
+        col, nor = (np.zeros(3, np.float32), np.zeros(3, np.float32)) if kid is None else kid
+        if kid is not None:
+            notnull = f(notnull + f(1))
+        for a in range(3):
+            c[a] = f(c[a] + col[a])
+            nrm[a] = f(nrm[a] + nor[a])
+    col = np.array([f(x / notnull) for x in c], np.float32)
+    t = [f(x / notnull) for x in nrm]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = f(f(1) / np.sqrt(f(f(t[0] * t[0] + t[1] * t[1]) + f(t[2] * t[2]))))
+        nor = np.array([f(x * inv) for x in t], np.float32)
+    return col, nor
+
+
+@pytest.mark.parametrize("payload", [True, False])
+def test_levels_post_order(payload):
+    """-levels: data indices are post-order ranks (every node for payload, internal nodes
+    only for binary after the white record), leaves keep their records, internal records
+    are the child averages with morton 0; binary leaves have zero caches, so their
+    ancestors' normals are NaN (written as the x86 default NaN)."""
+    G, D = 16, 4
+    rng = np.random.default_rng(31 + payload)
+    codes = np.unique(rng.choice(G ** 3, 300, replace=False)).astype(np.uint64)
+    if payload:
+        recs = np.zeros(len(codes), DATA_DTYPE)
+        recs["morton"] = codes
+        recs["color"] = rng.uniform(0, 1, (len(codes), 3))
+        recs["normal"] = rng.normal(0, 1, (len(codes), 3))
+        nodes, data, hz = oracle.build_octree(recs, G, True, levels=True)
+    else:
+        nodes, data, hz = oracle.build_octree(codes, G, False, levels=True)
+    # the node structure is the one without -levels; only internal data fields change
+    plain = octree_by_definition(codes, G, payload)
+    assert np.array_equal(nodes["children_base"], plain["children_base"])
+    assert np.array_equal(nodes["offsets"], plain["offsets"])
+    order = _post_order(nodes, D)
+    internal = [(i, l) for i, l in order if l < D]
+    assert len(data) == (1 + len(codes) if payload else 2) + len(internal)
+    cache = {}
+    rank = 0
+    leaf_rank = 0
+    for idx, level in order:
+        n = nodes[idx]
+        if level == D:
+            if payload:
+                assert int(n["data"]) == 1 + rank
+                assert data[int(n["data"])].tobytes() == recs[leaf_rank].tobytes()
+                cache[idx] = (recs[leaf_rank]["color"], recs[leaf_rank]["normal"])
+                rank += 1
+            else:
+                assert int(n["data"]) == 1
+                cache[idx] = (np.zeros(3, np.float32), np.zeros(3, np.float32))
+            leaf_rank += 1
+            continue
+        want_idx = (1 + rank) if payload else (2 + rank)
+        assert int(n["data"]) == want_idx
+        rank += 1
+        kids = [cache[int(n["children_base"]) + int(n["offsets"][k])] if n["offsets"][k] >= 0
+                else None for k in range(8)]
+        col, nor = _f32_average(kids)
+        cache[idx] = (col, nor)
+        rec = data[want_idx]
+        assert int(rec["morton"]) == 0
+        assert rec["color"].tobytes() == col.tobytes()
+        if payload:
+            assert rec["normal"].tobytes() == nor.tobytes()
+        else:
+            assert np.isnan(nor).all()
+            assert rec["normal"].view(np.uint32).tolist() == [0xFFC00000] * 3
