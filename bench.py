@@ -263,6 +263,10 @@ class MixBench:
         from ooc_svo_builder_amd import SvoContext
         b = SvoContext(0)
         res, info = b.batch_run(self.jobs)  # warm-up (allocations)
+        t = time.perf_counter()
+        knobs = b.autotune(self.jobs)
+        _log("autotuned small_cols/tile_blocks/large_blocks = %s in %.1f s" % (
+            knobs, time.perf_counter() - t))
         reps = 3
         times = []
         for _ in range(reps):
@@ -292,6 +296,8 @@ class MixBench:
                 "plain_loop_seconds": self.plain_s, "plain_jobs_per_s": n / self.plain_s,
                 "speedup_vs_plain": self.plain_s / best,
                 "identical_jobs": same, "identical": same == n,
+                "autotuned": {"small_cols": knobs[0], "tile_blocks": knobs[1],
+                              "large_blocks": knobs[2]},
                 "voxels": int(info.nfilled), "nodes": int(info.n_nodes)}
 
     def served(self, seconds):

@@ -738,4 +738,70 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
   return SVO_OK;
 }
 
+int svo_ctx_autotune(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, int best_out[3]) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  std::vector<svo_job_result> res(n_jobs);
+  // device time of the voxelize stage (the only one the knobs touch), best of two runs
+  auto measure = [&](float* ms) -> int {
+    float best = 1e30f;
+    for (int rep = 0; rep < 2; rep++) {
+      svo_run_info info;
+      const int r = svo_batch_run(ctx, jobs, n_jobs, res.data(), &info);
+      if (r) return r;
+      best = std::min(best, info.ms_voxelize);
+    }
+    *ms = best;
+    return SVO_OK;
+  };
+  float cur = 0;
+  int r = measure(&cur);  // also the warm-up (allocations)
+  if (r) return r;
+  r = measure(&cur);
+  if (r) return r;
+  // coordinate descent over the three knobs; every setting gives the same output
+  static const int cand_cols[] = {64, 256, 1024, 4096};
+  static const int cand_tile[] = {768, 1536, 2048, 4096};
+  static const int cand_large[] = {1024, 2048, 4096};
+  int* knob[3] = {&c.tune.small_cols, &c.tune.tile_blocks, &c.tune.large_blocks};
+  const int* cands[3] = {cand_cols, cand_tile, cand_large};
+  const int ncand[3] = {4, 4, 3};
+  for (int k = 0; k < 3; k++) {
+    const int keep = *knob[k];
+    int best = keep;
+    float best_ms = cur;
+    for (int q = 0; q < ncand[k]; q++) {
+      if (cands[k][q] == keep) continue;
+      *knob[k] = cands[k][q];
+      float ms = 0;
+      r = measure(&ms);
+      if (r) {
+        *knob[k] = keep;
+        return r;
+      }
+      if (ms < best_ms * 0.98f) {
+        best_ms = ms;
+        best = cands[k][q];
+      }
+    }
+    *knob[k] = best;
+    cur = best_ms;
+  }
+  if (best_out) {
+    best_out[0] = c.tune.small_cols;
+    best_out[1] = c.tune.tile_blocks;
+    best_out[2] = c.tune.large_blocks;
+  }
+  return SVO_OK;
+}
+
+int svo_ctx_get_tuning(svo_ctx* ctx, int out[3]) {
+  CHECK_CTX();
+  if (!out) return SVO_ERR_INVALID;
+  out[0] = ctx->c.tune.small_cols;
+  out[1] = ctx->c.tune.tile_blocks;
+  out[2] = ctx->c.tune.large_blocks;
+  return SVO_OK;
+}
+
 }  // extern "C"

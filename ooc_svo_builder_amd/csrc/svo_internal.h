@@ -55,8 +55,9 @@ struct DevBuf {
 
 // Voxelizer tuning: small/large split threshold (columns per triangle) and block size.
 struct VoxTuning {
-  int small_cols = 1024;
-  int block = 256;
+  int small_cols = 1024;    // columns above which a triangle goes to the large tier
+  int tile_blocks = 2048;   // persistent grid of the small tier
+  int large_blocks = 2048;  // grid of the large tier (grid-stride)
 };
 
 struct StageTimes {

@@ -1133,8 +1133,8 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   SVO_TRY(ensure_scan_scratch(c, std::max<uint64_t>(ntri, ntab), 8));
   ull* ctr = c.counters.as<ull>();
   ull hc[kNumCounters];
-  const unsigned tile_grid = (unsigned)std::min<uint64_t>(ntiles, 2048);
-  const unsigned large_grid = 2048;
+  const unsigned tile_grid = (unsigned)std::min<uint64_t>(ntiles, c.tune.tile_blocks);
+  const unsigned large_grid = (unsigned)c.tune.large_blocks;
   auto brick_grid = [&](uint64_t bricks) {
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((bricks * kWave + 255) / 256, 8192));
   };

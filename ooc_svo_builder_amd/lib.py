@@ -86,6 +86,8 @@ SIGNATURES = {
                        C.POINTER(RunInfo)]),
     "svo_batch_run": (_i32, [_vp, C.POINTER(Job), C.c_uint32, C.POINTER(JobResult),
                              C.POINTER(RunInfo)]),
+    "svo_ctx_autotune": (_i32, [_vp, C.POINTER(Job), C.c_uint32, C.POINTER(C.c_int * 3)]),
+    "svo_ctx_get_tuning": (_i32, [_vp, C.POINTER(C.c_int * 3)]),
     "svo_server_create": (_i32, [_i32, _u64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
     "svo_server_destroy": (None, [_vp]),
     "svo_server_submit": (_i32, [_vp, C.POINTER(Job), C.POINTER(_u64)]),
@@ -355,6 +357,20 @@ class SvoContext:
         _check(load_library().svo_batch_run(self._h, arr, n, res, C.byref(info)))
         self.last_info = info
         return list(res), info
+
+    def autotune(self, jobs) -> tuple:
+        """Pick the voxelizer knobs (split threshold, grid sizes) on this batch."""
+        n = len(jobs)
+        arr = (Job * n)()
+        keep = [fill_job(arr[i], j) for i, j in enumerate(jobs)]  # noqa: F841
+        out = (C.c_int * 3)()
+        _check(load_library().svo_ctx_autotune(self._h, arr, n, C.byref(out)))
+        return tuple(out)
+
+    def get_tuning(self) -> tuple:
+        out = (C.c_int * 3)()
+        _check(load_library().svo_ctx_get_tuning(self._h, C.byref(out)))
+        return tuple(out)
 
     def nodes_range(self, first: int, count: int) -> np.ndarray:
         out = np.empty(count, NODE_DTYPE)

@@ -12,6 +12,18 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) to run")
 
 
+@pytest.fixture(scope="session", autouse=True)
+def _torch_runtime_first(request):
+    """PyTorch bundles its own HIP and HSA runtimes; in a process that also loads
+    libsvo_amd.so (linked to the system ROCm), PyTorch finds the GPU only if it
+    initializes first. GPU sessions therefore start PyTorch's device before any test."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    yield
+
+
 @pytest.fixture(scope="session")
 def oracle_lib():
     import oracle

@@ -73,6 +73,13 @@ def test_batch_matches_plain():
         sub = [jobs[i] for i in (9, 3, 0, 12, 7)]
         _say("second batch of %d jobs" % len(sub))
         _check_batch(b, p, sub)
+        # knobs picked by the autotuner, and extreme ones: the same bytes
+        _say("autotune")
+        knobs = b.autotune(sub)
+        assert b.get_tuning() == knobs
+        _check_batch(b, p, jobs)
+        b.set_tuning(small_cols=4)
+        _check_batch(b, p, sub)
 
 
 def test_batch_matches_oracle():
