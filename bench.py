@@ -63,7 +63,12 @@ def main():
     ap.add_argument("--mix-jobs", type=int, default=512, help="meshes in the job mix")
     ap.add_argument("--served-seconds", type=float, default=2.0,
                     help="arrival window per rate of the served sweep")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (the reference kernel "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(the multi-GPU sharding is a design, not a measured path)" % args.gpus,
@@ -203,6 +208,20 @@ def main():
         except Exception as e:
             out["served"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def calibrate_copy():
+    """A device-to-device copy of exactly 1 GiB, 5 times: its counters read 1 GiB fetched
+    and 1 GiB written per launch on an ideal memory system."""
+    import torch
+    n = 1 << 30
+    a = torch.ones(n // 4, dtype=torch.int32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_per_launch": n, "launches": 5}), flush=True)
     return 0
 
 

@@ -17,6 +17,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libsvo_amd.so")
 CLI_PAYLOAD = os.path.join(PKG, "bin", "svo_builder")
 CLI_BINARY = os.path.join(PKG, "bin", "svo_builder_binary")
+OCTREE_TOOL = os.path.join(PKG, "bin", "svo_octree_tool")
 
 SVO_OK = 0
 SVO_ERR_INVALID = 1
