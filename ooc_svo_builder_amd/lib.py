@@ -18,6 +18,8 @@ LIB_PATH = os.path.join(PKG, "lib", "libsvo_amd.so")
 CLI_PAYLOAD = os.path.join(PKG, "bin", "svo_builder")
 CLI_BINARY = os.path.join(PKG, "bin", "svo_builder_binary")
 OCTREE_TOOL = os.path.join(PKG, "bin", "svo_octree_tool")
+TRI_CONVERT = os.path.join(PKG, "bin", "tri_convert")
+TRI_CONVERT_BINARY = os.path.join(PKG, "bin", "tri_convert_binary")
 
 SVO_OK = 0
 SVO_ERR_INVALID = 1
