@@ -72,6 +72,13 @@ uint64_t svo_estimate_partitions(uint64_t gridsize, uint64_t memory_limit_mb);
 int svo_mesh_load(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payload,
                   int src_on_device);
 
+/* TriReader (TriReader.h:9-83) for a whole .tridata file: streams it into HBM through two
+ * pinned 32 MiB chunks, the read of one overlapping the copy of the other. A file with
+ * fewer than n_triangles records fails with SVO_ERR_INVALID; got_out (optional) receives
+ * the complete records found. Resets the voxel stream like svo_mesh_load. */
+int svo_mesh_load_file(svo_ctx* ctx, const char* tridata_path, uint64_t n_triangles,
+                       int payload, uint64_t* got_out);
+
 /* partitioner.cpp:101-149 on the device: for each of n_partitions Morton-range
  * partitions, the in-order list of triangles whose bbox touches the partition's world
  * box (unitlength from the .tri bbox, partitioner.cpp:45). counts_out[n_partitions]

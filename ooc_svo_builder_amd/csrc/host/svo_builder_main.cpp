@@ -274,24 +274,22 @@ int main(int argc, char* argv[]) {
     return 0;
   }
 
-  // read the whole .tridata once (the reference streams it through TriReader)
-  const size_t stride = kPayload ? 21 : 9;
-  std::vector<float> tris((size_t)ti.n_triangles * stride);
-  {
-    FILE* f = fopen((ti.base + ".tridata").c_str(), "rb");
-    size_t got = f ? fread(tris.data(), stride * sizeof(float), ti.n_triangles, f) : 0;
-    if (f) fclose(f);
-    if (got != ti.n_triangles) {
-      std::cerr << "svo_builder: " << ti.base << ".tridata holds " << got
-                << " triangles, the header says " << ti.n_triangles << endl;
-      return 1;
-    }
-  }
-
+  // stream the .tridata into HBM (the reference streams it through TriReader, 8192
+  // triangles at a time)
   svo_ctx* ctx = nullptr;
   if (svo_ctx_create(device, &ctx) != SVO_OK) return die(nullptr, "svo_ctx_create");
-  if (svo_mesh_load(ctx, tris.data(), ti.n_triangles, kPayload ? 1 : 0, 0) != SVO_OK)
-    return die(ctx, "svo_mesh_load");
+  {
+    uint64_t got = 0;
+    const int rc = svo_mesh_load_file(ctx, (ti.base + ".tridata").c_str(), ti.n_triangles,
+                                      kPayload ? 1 : 0, &got);
+    if (rc == SVO_ERR_INVALID && got < ti.n_triangles) {
+      std::cerr << "svo_builder: " << ti.base << ".tridata holds " << got
+                << " triangles, the header says " << ti.n_triangles << endl;
+      svo_ctx_destroy(ctx);
+      return 1;
+    }
+    if (rc != SVO_OK) return die(ctx, "svo_mesh_load_file");
+  }
 
   // PARTITIONING (main.cpp:293-302)
   cout << "Estimating best partition count ..." << endl;

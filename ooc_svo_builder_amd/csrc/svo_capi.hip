@@ -260,6 +260,10 @@ void svo_ctx_destroy(svo_ctx* ctx) {
   for (DevBuf* b : bufs) release(*b);
   if (c.stage) (void)hipHostFree(c.stage);
   c.stage = nullptr;
+  for (int k = 0; k < 2; k++) {
+    if (c.ingest[k]) (void)hipHostFree(c.ingest[k]);
+    if (c.ingest_ev[k]) (void)hipEventDestroy(c.ingest_ev[k]);
+  }
   ktimer_flush(c);
   for (hipEvent_t e : c.evpool) (void)hipEventDestroy(e);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
@@ -319,6 +323,72 @@ int svo_mesh_load(svo_ctx* ctx, const float* records, uint64_t n_triangles, int 
          "svo_mesh_load");
   TRYC(fill_u64(c.counters.as<uint64_t>(), 0, kNumCounters, c.stream), "svo_mesh_load");
   TRYC(hipStreamSynchronize(c.stream), "svo_mesh_load");
+  c.n_tris = n_triangles;
+  c.payload = payload ? 1 : 0;
+  reset_plain(c);
+  return SVO_OK;
+}
+
+int svo_mesh_load_file(svo_ctx* ctx, const char* tridata_path, uint64_t n_triangles,
+                       int payload, uint64_t* got_out) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (got_out) *got_out = 0;
+  if (!tridata_path) {
+    g_err = "null path";
+    return SVO_ERR_INVALID;
+  }
+  FILE* f = fopen(tridata_path, "rb");
+  if (!f) {
+    g_err = std::string("cannot open ") + tridata_path;
+    return SVO_ERR_INVALID;
+  }
+  const uint64_t R = payload ? 84 : 36;
+  const uint64_t bytes = n_triangles * R;
+  hipError_t e = ensure(c.tris, bytes);
+  if (e != hipSuccess) {
+    fclose(f);
+    return fail(e, "svo_mesh_load_file");
+  }
+  // two pinned chunks: the file read of one overlaps the copy of the other
+  const size_t chunk = 32u << 20;
+  if (c.ingest_cap < chunk) {
+    for (int k = 0; k < 2; k++) {
+      if (c.ingest[k]) (void)hipHostFree(c.ingest[k]);
+      c.ingest[k] = nullptr;
+      if ((e = hipHostMalloc(&c.ingest[k], chunk, hipHostMallocDefault)) != hipSuccess ||
+          (!c.ingest_ev[k] && (e = hipEventCreate(&c.ingest_ev[k])) != hipSuccess)) {
+        fclose(f);
+        c.ingest_cap = 0;
+        return fail(e, "svo_mesh_load_file");
+      }
+    }
+    c.ingest_cap = chunk;
+  }
+  uint64_t off = 0;
+  for (uint64_t k = 0; off < bytes; k++) {
+    const int b = (int)(k & 1);
+    if (k >= 2 && (e = hipEventSynchronize(c.ingest_ev[b])) != hipSuccess) break;
+    const size_t want = (size_t)std::min<uint64_t>(chunk, bytes - off);
+    const size_t got = fread(c.ingest[b], 1, want, f);
+    if (got &&
+        ((e = hipMemcpyAsync((char*)c.tris.p + off, c.ingest[b], got, hipMemcpyHostToDevice,
+                             c.stream)) != hipSuccess ||
+         (e = hipEventRecord(c.ingest_ev[b], c.stream)) != hipSuccess))
+      break;
+    off += got;
+    if (got < want) break;
+  }
+  fclose(f);
+  if (e != hipSuccess) return fail(e, "svo_mesh_load_file");
+  TRYC(fill_u64(c.counters.as<uint64_t>(), 0, kNumCounters, c.stream), "svo_mesh_load_file");
+  TRYC(hipStreamSynchronize(c.stream), "svo_mesh_load_file");
+  if (got_out) *got_out = off / R;
+  if (off < bytes) {
+    g_err = std::string(tridata_path) + " holds " + std::to_string(off / R) +
+            " triangles, the header says " + std::to_string(n_triangles);
+    return SVO_ERR_INVALID;
+  }
   c.n_tris = n_triangles;
   c.payload = payload ? 1 : 0;
   reset_plain(c);

@@ -169,6 +169,11 @@ struct Context {
   bool data_in_records2 = false;            // the built data records are in records2
   const DevBuf& data_out() const { return data_in_records2 ? records2 : records; }
 
+  // streaming mesh ingest: two pinned chunk buffers and their copy events
+  void* ingest[2] = {nullptr, nullptr};
+  size_t ingest_cap = 0;
+  hipEvent_t ingest_ev[2] = {nullptr, nullptr};
+
   // pinned host staging for small uploads (bump allocated; wraps after a stream sync)
   void* stage = nullptr;
   size_t stage_cap = 0, stage_used = 0;

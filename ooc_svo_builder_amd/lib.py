@@ -71,6 +71,7 @@ SIGNATURES = {
     "svo_ctx_set_tuning": (_i32, [_vp, _i32, _u64]),
     "svo_estimate_partitions": (_u64, [_u64, _u64]),
     "svo_mesh_load": (_i32, [_vp, _vp, _u64, _i32, _i32]),
+    "svo_mesh_load_file": (_i32, [_vp, C.c_char_p, _u64, _i32, C.POINTER(_u64)]),
     "svo_partition": (_i32, [_vp, _f32, _u64, _u64, _vp]),
     "svo_partition_indices": (_i32, [_vp, _u64, _vp]),
     "svo_voxelize": (_i32, [_vp, _u64, _u64, _u64, _f32, _u64, _i32, C.POINTER(_u64)]),
@@ -288,6 +289,14 @@ class SvoContext:
             r = np.ascontiguousarray(records, np.float32)
             _check(L.svo_mesh_load(self._h, _ptr(r), r.shape[0], int(payload), 0))
         self.payload = bool(payload)
+
+    def load_mesh_file(self, tridata_path: str, n_triangles: int, payload: bool) -> int:
+        """Stream a .tridata file into HBM; returns the triangle count loaded."""
+        got = C.c_uint64(0)
+        _check(load_library().svo_mesh_load_file(self._h, tridata_path.encode(), n_triangles,
+                                                 int(payload), C.byref(got)))
+        self.payload = bool(payload)
+        return int(got.value)
 
     def partition(self, unitlength: float, gridsize: int, n_partitions: int) -> np.ndarray:
         counts = np.zeros(n_partitions, np.uint64)

@@ -290,3 +290,22 @@ def test_voxelizer_paths_identical(small_cols, dense_mb):
         for G, L in ((128, 2048), (256, 2)):
             _run_both(ctx, tris, False, G, L)
         _run_both(ctx, generators.terrain(n=49, seed=8, payload=True), True, 128, 2048)
+
+
+def test_mesh_load_file_stream(gpu_ctx, tmp_path):
+    """TriReader replacement: a .tridata file streamed through the pinned chunks gives the
+    same voxels as the in-memory load; a short file is an error naming what it holds."""
+    _say("mesh load file")
+    tris = generators.terrain(n=257, seed=12, payload=True)  # 131,072 triangles, 11 MB
+    tri = write_tri(str(tmp_path / "t"), tris, True)
+    data = tri[:-4] + ".tridata"
+    assert gpu_ctx.load_mesh_file(data, tris.shape[0], True) == tris.shape[0]
+    gpu_ctx.voxelize(0, 0, 128 ** 3, 1.0 / 128, 128)
+    got = gpu_ctx.codes()
+    want = gpu_codes(gpu_ctx, tris, True, 128, 1.0 / 128)
+    assert np.array_equal(got, want)
+    with open(data, "r+b") as f:
+        f.truncate(84 * 1000 + 40)
+    with pytest.raises(Exception) as e:
+        gpu_ctx.load_mesh_file(data, tris.shape[0], True)
+    assert "holds 1000 triangles" in str(e.value)
