@@ -415,6 +415,175 @@ __global__ __launch_bounds__(256) void k_emit_leaves(Arena A, LevelMeta M, Tags 
   put_node(nodes, gpos, data, 0ull, ~0ull);
 }
 
+// ---------------------------------------------------------------- brick fast path
+// Every occupied 16^3 brick of the voxel stream is an aligned subtree of height 4 whose
+// occupancy is its 64 words (word w = voxels 64w..64w+63 in Morton order). Within it:
+//   height 1: a byte of a word (8 voxels), child mask = the byte;
+//   height 2: a word, child mask = its non-zero bytes;
+//   height 3: 8 consecutive words, child mask = its non-zero words;
+//   height 4: the brick, child mask = its non-zero word groups.
+// One wave per brick (lane = word) computes the subtree sizes, and later writes every
+// record below the brick node at its post-order position. Heights >= 4 use the level
+// scans. Sizes: s1 = 1 + popc(byte), s2 = 1 + nnz bytes + popc(word), s3 = 1 + sum s2,
+// s4 = 1 + sum s3 (non-null children only).
+
+// sum over the 8 lanes of a lane's group (all lanes active)
+__device__ __forceinline__ uint32_t group8_sum(uint32_t v) {
+  v += (uint32_t)__shfl_xor((int)v, 1, kWave);
+  v += (uint32_t)__shfl_xor((int)v, 2, kWave);
+  v += (uint32_t)__shfl_xor((int)v, 4, kWave);
+  return v;
+}
+
+struct BrickShape {
+  uint64_t word;
+  uint32_t m2;      // non-zero bytes of this word
+  uint32_t s2;      // subtree size of this word's node (0 if empty)
+  uint32_t mask3;   // non-zero words of this lane's group
+  uint32_t s3;      // subtree size of the group's node (0 if empty)
+  uint32_t mask4;   // non-zero groups
+  uint32_t s4;      // subtree size of the brick node
+};
+
+__device__ __forceinline__ BrickShape brick_shape(const uint64_t* __restrict__ bwords,
+                                                  uint64_t b) {
+  BrickShape B;
+  const int lane = lane_id();
+  B.word = bwords[b * kBrickWords + lane];
+  B.m2 = 0;
+#pragma unroll
+  for (int j = 0; j < 8; j++)
+    if ((B.word >> (8 * j)) & 0xFFull) B.m2 |= 1u << j;
+  B.s2 = B.word ? 1u + (uint32_t)__popc(B.m2) + (uint32_t)__popcll(B.word) : 0u;
+  const uint64_t nz = __ballot(B.word != 0ull);
+  const int g = lane >> 3;
+  B.mask3 = (uint32_t)(nz >> (8 * g)) & 0xFFu;
+  const uint32_t sum2 = group8_sum(B.s2);
+  B.s3 = B.mask3 ? 1u + sum2 : 0u;
+  B.mask4 = 0;
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+    if ((nz >> (8 * q)) & 0xFFull) B.mask4 |= 1u << q;
+  // s4 = 1 + sum of the groups' s3: each group counted once (its first lane)
+  uint32_t v = ((lane & 7) == 0) ? B.s3 : 0u;
+  for (int d = 1; d < kWave; d <<= 1) v += (uint32_t)__shfl_xor((int)v, d, kWave);
+  B.s4 = 1u + v;
+  return B;
+}
+
+// Height-4 level of the arena from the brick stream: code, subtree size, child mask.
+__global__ __launch_bounds__(256) void k_brick_level(Arena A, LevelMeta M, Tags T,
+                                                     const JobDev* __restrict__ jobs,
+                                                     const uint64_t* __restrict__ bwords,
+                                                     const uint64_t* __restrict__ bcode,
+                                                     uint64_t nb,
+                                                     unsigned long long* root_size) {
+  const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (w >= nb) return;  // uniform per wave
+  const BrickShape B = brick_shape(bwords, w);
+  if (lane_id() == 0) {
+    const uint64_t c0 = bcode[w];
+    const uint64_t code = (c0 & T.tag_mask) | ((c0 & ~T.tag_mask) >> 12);
+    const uint64_t k = M.off[4] + w;
+    if (k >= A.cap) {
+      M.flag[0] = 1;
+      return;
+    }
+    A.code[k] = code;
+    A.size[k] = B.s4;
+    A.mask[k] = (uint8_t)B.mask4;
+    A.first[k] = 0;
+    const uint32_t j = T.job(code);
+    if (jobs[j].depth == 4) root_size[j] = B.s4;
+  }
+}
+
+// Every record below each brick node: the word groups (height 3), words (height 2),
+// bytes (height 1) and voxels (leaves), at the positions the post-order formula gives
+// from the brick node's start.
+__global__ __launch_bounds__(256) void k_emit_bricks(Arena A, LevelMeta M, Tags T,
+                                                     const JobDev* __restrict__ jobs,
+                                                     const uint64_t* __restrict__ bwords,
+                                                     const uint64_t* __restrict__ bcode,
+                                                     const uint64_t* __restrict__ bvoff,
+                                                     uint64_t nb, uint64_t node_cap,
+                                                     uint64_t* __restrict__ nodes) {
+  if (M.flag[0]) return;
+  const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (w >= nb) return;  // uniform per wave
+  const int lane = lane_id();
+  const int g = lane >> 3, q = lane & 7;
+  const BrickShape B = brick_shape(bwords, w);
+  const JobDev& J = jobs[T.job(bcode[w])];
+  const uint64_t S = A.start[M.off[4] + w];
+  const uint64_t lim = J.n_nodes;
+  bool bad = false;
+  // the brick's own children (groups): start = S + sum of (s3 - 1) of the groups before
+  const uint32_t cb4 = B.s4 - 1u - (uint32_t)__popc(B.mask4);
+  const uint32_t v3 = (q == 0 && B.s3) ? B.s3 - 1u : 0u;
+  const uint32_t inc3 = wave_incl_scan<uint32_t>(v3);
+  const uint32_t pre3 = (uint32_t)__shfl((int)(inc3 - v3), g * 8, kWave);
+  const uint64_t start3 = S + pre3;
+  const uint64_t cb3 = start3 + B.s3 - 1u - (uint32_t)__popc(B.mask3);
+  if (q == 0 && B.s3) {
+    const uint64_t pos = S + cb4 + (uint32_t)__popc(B.mask4 & ((1u << g) - 1u));
+    if (pos < lim && J.nbase + pos < node_cap)
+      put_node(nodes, J.nbase + pos, 0ull, cb3, offsets_of(B.mask3));
+    else
+      bad = true;
+  }
+  // this lane's word node: start = start3 + sum of (s2 - 1) of the words before it in
+  // the group
+  const uint32_t v2 = B.word ? B.s2 - 1u : 0u;
+  uint32_t inc2 = v2;
+#pragma unroll
+  for (int d = 1; d < 8; d <<= 1) {
+    const uint32_t o = (uint32_t)__shfl_up((int)inc2, d, kWave);
+    if (q >= d) inc2 += o;
+  }
+  const uint64_t start2 = start3 + (inc2 - v2);
+  const uint64_t cb2 = start2 + B.s2 - 1u - (uint32_t)__popc(B.m2);
+  // voxels before this word in the brick (for payload data indices)
+  const uint32_t pw = (uint32_t)__popcll(B.word);
+  const uint32_t incw = wave_incl_scan<uint32_t>(pw);
+  const uint64_t vox0 = bvoff[w] + (incw - pw) - J.voff;
+  if (B.word) {
+    const uint64_t pos2 = cb3 + (uint32_t)__popc(B.mask3 & ((1u << q) - 1u));
+    if (pos2 < lim && J.nbase + pos2 < node_cap)
+      put_node(nodes, J.nbase + pos2, 0ull, cb2, offsets_of(B.m2));
+    else
+      bad = true;
+    // byte nodes and their voxels
+    uint64_t start1 = start2;
+    uint32_t rank1 = 0;
+    uint32_t vr = 0;
+    for (int j = 0; j < 8; j++) {
+      const uint32_t byte = (uint32_t)(B.word >> (8 * j)) & 0xFFu;
+      if (!byte) continue;
+      const uint64_t pos1 = cb2 + rank1;
+      // s1 - 1 - popc(byte) = 0: the byte's leaves start at its own start
+      const uint64_t cb1 = start1;
+      if (pos1 < lim && J.nbase + pos1 < node_cap)
+        put_node(nodes, J.nbase + pos1, 0ull, cb1, offsets_of(byte));
+      else
+        bad = true;
+      const uint32_t nl = (uint32_t)__popc(byte);
+      for (uint32_t r = 0; r < nl; r++) {
+        const uint64_t posl = cb1 + r;
+        const uint64_t data = J.payload ? 1ull + vox0 + vr : 1ull;
+        if (posl < lim && J.nbase + posl < node_cap)
+          put_node(nodes, J.nbase + posl, data, 0ull, ~0ull);
+        else
+          bad = true;
+        vr++;
+      }
+      start1 += nl;  // s1 - 1 of this byte
+      rank1++;
+    }
+  }
+  if (bad) M.flag[1] = 1;
+}
+
 #define SVO_TRY(x)                   \
   do {                               \
     hipError_t _e = (x);             \
@@ -465,6 +634,13 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     n_records += J.n_data;
     if (J.nvox) H = std::max(H, D);
   }
+  // brick fast path: the four lowest levels come from the voxelizer's brick stream (not
+  // for -levels, for voxels added from outside, or for meshes below 16^3)
+  bool fast = c.bricks_ok && !levels && c.n_bricks > 0;
+  for (uint32_t j = 0; j < n_jobs; j++)
+    if (jd[j].nvox && jd[j].depth < 4) fast = false;
+  const uint64_t NB = fast ? c.n_bricks : 0;
+  const int h0 = fast ? 4 : 0;  // lowest height the level scans start from
   // per-height upper bounds (host) size the launch grids
   bound.assign(H + 2, 0);
   bound[0] = V;
@@ -476,6 +652,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
       b += (3 * e >= 63) ? jd[j].nvox : std::min<uint64_t>(jd[j].nvox, 1ull << (3 * e));
     }
     bound[h] = std::min(bound[h - 1], b);
+    if (fast && h == 4) bound[h] = NB;
   }
   SVO_TRY(ensure(c.jobdev, n_jobs * sizeof(JobDev)));
   SVO_TRY(upload_async(c, c.jobdev.p, jd.data(), n_jobs * sizeof(JobDev)));
@@ -484,7 +661,11 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
   JobDev* d_jobs = c.jobdev.as<JobDev>();
   const Tags T{tagged ? ~kCodeMask : 0ull};
 
-  uint64_t cap = std::max<uint64_t>(V / 2 + 1024, 4096);
+  // arena: the internal nodes the level scans produce (heights >= h0)
+  uint64_t cap = fast ? NB + NB / 2 + 4096 : std::max<uint64_t>(V / 2 + 1024, 4096);
+  // internal nodes below the brick nodes (fast path): bytes, words, word groups
+  const uint64_t in_brick =
+      fast ? std::min<uint64_t>(V, 512 * NB) + std::min<uint64_t>(V, 64 * NB) + 8 * NB : 0;
   for (int attempt = 0;; attempt++) {
     SVO_TRY(ensure(c.lvl_code, cap * 8));
     SVO_TRY(ensure(c.lvl_size, cap * 8));
@@ -501,8 +682,8 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
       SVO_TRY(ensure(c.records2, (n_records + cap) * 32));
     }
     float* out_rec = levels ? c.records2.as<float>() : c.records.as<float>();
-    SVO_TRY(ensure(c.lvl_leafp, std::max<uint64_t>(V, 1) * 4));
-    const uint64_t node_cap = cap + V + n_jobs;
+    if (!fast) SVO_TRY(ensure(c.lvl_leafp, std::max<uint64_t>(V, 1) * 4));
+    const uint64_t node_cap = cap + V + in_brick + n_jobs;
     SVO_TRY(ensure(c.nodes, node_cap * 24));
     SVO_TRY(ensure(c.scan_scratch, scan_scratch_elems(std::max<uint64_t>(V, 1)) * 4 + 256));
     Arena A{c.lvl_code.as<uint64_t>(), c.lvl_size.as<uint64_t>(), c.lvl_mask.as<uint8_t>(),
@@ -517,9 +698,19 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     // n[0] = V, off[1] = 0, flags = 0, root sizes = 0
     SVO_TRY(fill_u64((uint64_t*)meta, 0, 2 * 64 + 8 + 2 * (uint64_t)n_jobs, c.stream));
     SVO_TRY(fill_u64((uint64_t*)meta, V, 1, c.stream));
+    if (fast) {  // n[4] = NB, off[4] = 0, off[5] = NB
+      SVO_TRY(fill_u64((uint64_t*)meta + 4, NB, 1, c.stream));
+      SVO_TRY(fill_u64((uint64_t*)meta + 64 + 5, NB, 1, c.stream));
+      ktimer_begin(c, "brick_level");
+      hipLaunchKernelGGL(k_brick_level, dim3((unsigned)((NB * kWave + 255) / 256)), dim3(256), 0,
+                         c.stream, A, M, T, (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),
+                         c.bcode.as<uint64_t>(), NB, root_size);
+      ktimer_end(c);
+      SVO_TRY(hipGetLastError());
+    }
 
     // ---- bottom-up: heights 1 .. H
-    for (int h = 0; h < H; h++) {
+    for (int h = h0; h < H; h++) {
       HeadLoad ld{A, M, T, d_jobs, c.codes.as<uint64_t>(), h};
       ParentStore st{A, M, T, d_jobs, c.codes.as<uint64_t>(), c.lvl_leafp.as<uint32_t>(),
                      root_size, root_isize, c.records.as<float>() + 8, h};
@@ -531,7 +722,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
                        c.nodes.as<uint64_t>(), c.records.as<float>(), out_rec);
     SVO_TRY(hipGetLastError());
     // ---- top-down: heights H .. 1, then the leaves
-    for (int h = H; h >= 1; h--) {
+    for (int h = H; h >= std::max(h0, 1); h--) {
       ktimer_begin(c, "emit_children");
       hipLaunchKernelGGL(k_emit_level, dim3((unsigned)std::max<uint64_t>((bound[h] + 255) / 256, 1)),
                          dim3(256), 0, c.stream, A, M, T, (const JobDev*)d_jobs, h, node_cap,
@@ -539,7 +730,15 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
       ktimer_end(c);
       SVO_TRY(hipGetLastError());
     }
-    if (V) {
+    if (fast) {
+      ktimer_begin(c, "emit_bricks");
+      hipLaunchKernelGGL(k_emit_bricks, dim3((unsigned)((NB * kWave + 255) / 256)), dim3(256), 0,
+                         c.stream, A, M, T, (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),
+                         c.bcode.as<uint64_t>(), c.bvoff.as<uint64_t>(), NB, node_cap,
+                         c.nodes.as<uint64_t>());
+      ktimer_end(c);
+      SVO_TRY(hipGetLastError());
+    } else if (V) {
       ktimer_begin(c, "emit_leaves");
       hipLaunchKernelGGL(k_emit_leaves, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, c.stream,
                          A, M, T, (const JobDev*)d_jobs, node_cap, c.codes.as<uint64_t>(),

@@ -179,6 +179,8 @@ static void reset_plain(Context& c) {
   c.hazard = 0;
   c.batch_state = false;
   c.data_in_records2 = false;
+  c.n_bricks = 0;
+  c.bricks_ok = true;
 }
 
 // The 6-significant-digit text round trip a bbox value takes through the .trip header
@@ -256,7 +258,7 @@ void svo_ctx_destroy(svo_ctx* ctx) {
                     &c.wprefix, &c.winner, &c.scan_scratch, &c.codes, &c.records, &c.nodes,
                     &c.lvl_code, &c.lvl_size, &c.lvl_mask, &c.lvl_first, &c.lvl_start,
                     &c.lvl_aux, &c.lvl_leafp, &c.lvl_pidx, &c.lvl_isize, &c.lvl_cache,
-                    &c.lvl_pstart, &c.records2};
+                    &c.lvl_pstart, &c.records2, &c.bwords, &c.bcode, &c.bvoff};
   for (DevBuf* b : bufs) release(*b);
   if (c.stage) (void)hipHostFree(c.stage);
   c.stage = nullptr;
@@ -498,6 +500,7 @@ int svo_add_voxels(svo_ctx* ctx, const uint64_t* codes, const void* records32, u
     }
     last = codes[i];
   }
+  c.bricks_ok = false;  // these voxels come without their bricks
   TRYC(ensure_keep(c.codes, (c.n_vox + n) * 8, c.n_vox * 8, c.stream), "svo_add_voxels");
   TRYC(hipMemcpyAsync(c.codes.as<uint64_t>() + c.n_vox, codes, n * 8, hipMemcpyHostToDevice,
                       c.stream),

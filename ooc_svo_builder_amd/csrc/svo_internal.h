@@ -153,6 +153,13 @@ struct Context {
   DevBuf occ_list, bcount, wprefix, winner;
   DevBuf scan_scratch;
 
+  // brick stream: every occupied 16^3 brick of the voxel stream, in stream order: its 64
+  // occupancy words, its first voxel's code (tagged) and its first voxel's stream index.
+  // The build derives the four lowest tree levels from it.
+  DevBuf bwords, bcode, bvoff;
+  uint64_t n_bricks = 0;
+  bool bricks_ok = true;  // every voxel of the stream came with its brick, bricks unique
+
   // voxel stream across partitions and jobs (sorted by construction)
   DevBuf codes;    // u64 Morton codes (job tag in the high bits for batches)
   DevBuf records;  // 32 B data records of every job (each job: null record first)

@@ -679,7 +679,10 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
                                                     const ull* __restrict__ pool,
                                                     const uint64_t* __restrict__ boff,
                                                     uint64_t* __restrict__ codes,
-                                                    uint32_t* __restrict__ wprefix) {
+                                                    uint32_t* __restrict__ wprefix,
+                                                    uint64_t s0, uint64_t* __restrict__ bwords,
+                                                    uint64_t* __restrict__ bcode,
+                                                    uint64_t* __restrict__ bvoff) {
   const uint64_t n_occ = ctr[kCtrNumOcc];
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
     const uint64_t ti = occ[wv];
@@ -690,9 +693,15 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
     const uint32_t inc = wave_incl_scan<uint32_t>(c);
     uint64_t pos = boff[wv] + (inc - c);
     if (wprefix) wprefix[slot] = (uint32_t)pos;
-    const uint64_t cb = (units[u].bstart + ((ti - ubase[u]) << kBrickBits) +
-                         ((uint64_t)lane_id() << 6)) |
-                        ((uint64_t)units[u].job << kJobShift);
+    const uint64_t tag = (uint64_t)units[u].job << kJobShift;
+    const uint64_t b0 = units[u].bstart + ((ti - ubase[u]) << kBrickBits);
+    // the brick stream (appended in stream order)
+    bwords[slot] = word;
+    if (lane_id() == 0) {
+      bcode[wv] = b0 | tag;
+      bvoff[wv] = s0 + boff[wv];
+    }
+    const uint64_t cb = (b0 + ((uint64_t)lane_id() << 6)) | tag;
     while (word) {
       const int bit = __ffsll(word) - 1;
       codes[pos++] = cb + (uint64_t)bit;
@@ -1067,6 +1076,11 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   std::vector<uint32_t> toff(nu + 1);
   uint64_t ntab = 0, ntiles = 0, ntri = 0;
   for (uint32_t u = 0; u < nu; u++) {
+    // a unit smaller than a brick shares its brick with its neighbours: the brick stream
+    // would hold that brick more than once
+    const uint64_t G = units[u].gridsize;
+    if (units[u].mend - units[u].mstart < std::min<uint64_t>(kBrickVox, G * G * G))
+      c.bricks_ok = false;
     if (units[u].n > 0xFFFFFFF0ull) {
       set_error("partition holds more than 2^32 triangles");
       return hipErrorInvalidValue;
@@ -1301,11 +1315,19 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   if (n_occ) {
     if (any_payload) SVO_TRY(ensure(c.wprefix, n_occ * kBrickWords * 4));
     SVO_TRY(ensure_keep(c.codes, (s0 + nvox) * 8, s0 * 8, c.stream));
+    const uint64_t nb0 = c.n_bricks;
+    SVO_TRY(ensure_keep(c.bwords, (nb0 + n_occ) * kBrickWords * 8, nb0 * kBrickWords * 8,
+                        c.stream));
+    SVO_TRY(ensure_keep(c.bcode, (nb0 + n_occ) * 8, nb0 * 8, c.stream));
+    SVO_TRY(ensure_keep(c.bvoff, (nb0 + n_occ) * 8, nb0 * 8, c.stream));
     ktimer_begin(c, "brick_emit");
     hipLaunchKernelGGL(k_brick_emit, dim3(brick_grid(n_occ)), dim3(256), 0, c.stream,
                        c.occ_list.as<uint32_t>(), ctr, d_units, d_ubase, nu, c.pool.as<ull>(),
                        c.bcount.as<uint64_t>(), c.codes.as<uint64_t>() + s0,
-                       any_payload ? c.wprefix.as<uint32_t>() : (uint32_t*)nullptr);
+                       any_payload ? c.wprefix.as<uint32_t>() : (uint32_t*)nullptr, s0,
+                       c.bwords.as<uint64_t>() + nb0 * kBrickWords, c.bcode.as<uint64_t>() + nb0,
+                       c.bvoff.as<uint64_t>() + nb0);
+    c.n_bricks = nb0 + n_occ;
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
 
