@@ -60,6 +60,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched section")
     ap.add_argument("--no-served", action="store_true", help="skip the served section")
+    ap.add_argument("--only", choices=["headline", "batched", "served"],
+                    help="run one section only (profiling)")
     ap.add_argument("--mix-jobs", type=int, default=512, help="meshes in the job mix")
     ap.add_argument("--served-seconds", type=float, default=2.0,
                     help="arrival window per rate of the served sweep")
@@ -69,6 +71,13 @@ def main():
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy()
+    if args.only in ("batched", "served"):
+        args.no_cpu = True
+        args.no_batched = args.only != "batched"
+        args.no_served = args.only != "served"
+        args.steps, args.warmup = 1, 0
+    elif args.only == "headline":
+        args.no_batched = args.no_served = True
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(the multi-GPU sharding is a design, not a measured path)" % args.gpus,
@@ -203,6 +212,7 @@ def main():
         except Exception as e:
             out["batched"] = {"error": repr(e)}
     if mix is not None and not args.no_served:
+        mix.drop_outputs()  # the served sweep needs the HBM
         try:
             out["served"] = mix.served(args.served_seconds)
         except Exception as e:
@@ -276,6 +286,10 @@ class MixBench:
 
     def _plain(self, j):
         return self.ctx.run(j["records"], j["payload"], j["gridsize"])
+
+    def drop_outputs(self):
+        self.want = []
+        self.torch.cuda.empty_cache()
 
     def batched(self):
         torch = self.torch
@@ -351,6 +365,8 @@ class MixBench:
             tickets = []
             cv = threading.Condition()
 
+            failed = []
+
             def collector():
                 k = 0
                 while k < n:
@@ -358,8 +374,11 @@ class MixBench:
                         while len(tickets) <= k:
                             cv.wait()
                         tk = tickets[k]
-                    r, ms = srv.wait(tk)
-                    lat.append(ms)
+                    try:
+                        r, ms = srv.wait(tk)
+                        lat.append(ms)
+                    except Exception as e:  # counted and reported, the sweep goes on
+                        failed.append(repr(e))
                     done_t.append(time.perf_counter())
                     srv.release(tk)
                     k += 1
@@ -386,6 +405,7 @@ class MixBench:
                 fin[i] = tcur
             plat = (fin - arrive) * 1e3
             row = {"rate_jobs_per_s": lam, "jobs": n, "large_jobs": int(sum(p < 0 for p in pick)),
+                   "failed_jobs": len(failed), "first_failure": failed[0] if failed else None,
                    "jobs_per_s": n / span, "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
                    "p999_ms": _pct(lat, 99.9),
                    "mean_batch_jobs": (st1.completed - st0.completed) /

@@ -206,8 +206,8 @@ int svo_ctx_get_tuning(svo_ctx* ctx, int out[3]);
  * All svo_server_* calls are thread-safe. A host 'records' pointer must stay valid until
  * the job is done (the records are read when its batch starts). */
 typedef struct svo_server svo_server;
-/* admission_budget_mb = 0 selects 200 GiB; max_batch_jobs = 0 selects 512; slots = 0
- * selects 2. */
+/* admission_budget_mb = 0 selects 200 GiB (at most three quarters of the device memory
+ * free at creation); max_batch_jobs = 0 selects 512; slots = 0 selects 2. */
 int svo_server_create(int device, uint64_t admission_budget_mb, uint32_t max_batch_jobs,
                       uint32_t slots, svo_server** out);
 /* Waits for running batches, then frees every ticket and context. */

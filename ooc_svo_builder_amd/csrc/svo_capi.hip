@@ -770,8 +770,37 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
     }
   }
   TRYC(hipEventRecord(c.ev0, c.stream), "svo_batch_run");
+  // passes of consecutive units whose Morton spans fit the dense byte grid together (a
+  // larger single unit is a pass of its own, on the touched-brick path)
+  for (Job& J : c.jobs) {
+    J.voff = ~0ull;
+    J.nvox = 0;
+  }
   uint64_t nf = 0;
-  TRYC(voxelize_units(c, units, true, &nf), "svo_batch_run");
+  for (size_t u0 = 0; u0 < units.size();) {
+    size_t u1 = u0;
+    uint64_t span = 0;
+    while (u1 < units.size()) {
+      const uint64_t bstart = units[u1].mstart & ~(uint64_t)(kBrickVox - 1);
+      const uint64_t ub = ((units[u1].mend - bstart + kBrickVox - 1) >> kBrickBits) << kBrickBits;
+      if (u1 > u0 && span + ub > c.dense_limit) break;
+      span += ub;
+      u1++;
+    }
+    uint64_t nf1 = 0;
+    TRYC(voxelize_units(c, std::vector<VoxUnit>(units.begin() + u0, units.begin() + u1), true,
+                        &nf1),
+         "svo_batch_run");
+    nf += nf1;
+    u0 = u1;
+  }
+  {  // jobs without voxels: an empty range where they would start
+    uint64_t prev_end = 0;
+    for (Job& J : c.jobs) {
+      if (J.voff == ~0ull) J.voff = prev_end;
+      prev_end = J.voff + J.nvox;
+    }
+  }
   c.nfilled = nf;
   ms_v = elapsed(c);
   TRYC(hipEventRecord(c.ev0, c.stream), "svo_batch_run");

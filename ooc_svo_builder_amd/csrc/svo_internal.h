@@ -17,6 +17,9 @@ namespace svo {
 constexpr int kBrickBits = 12;
 constexpr int kBrickVox = 1 << kBrickBits;
 constexpr int kBrickWords = 1 << (kBrickBits - 6);
+// Per-brick counts carry a non-empty flag at bit kNonEmptyShift above the voxel count.
+constexpr int kNonEmptyShift = 40;
+constexpr uint64_t kVoxMask = (1ull << kNonEmptyShift) - 1;
 
 // Device counters block (one 256-byte line of u64s).
 enum Counter : int {

@@ -52,7 +52,8 @@ uint64_t estimate_job_bytes(const svo_job& q) {
   const double N = (double)q.n_triangles;
   const double R = q.payload ? 84.0 : 36.0;
   const double v = std::min(G * G * G, 8.0 * G * G + 16.0 * N);
-  const double per_vox = 8 + 28 + 24 + 8 + (q.payload ? 32 : 0);
+  // code 8, node records ~1.15 x 24, level arena ~24, brick stream ~18, winner/prefix 8
+  const double per_vox = 8 + 28 + 24 + 18 + 8 + (q.payload ? 32 : 0);
   const double b = (q.src_on_device ? 0.0 : N * R) + 24.0 * N + v * per_vox;
   return (uint64_t)b + (1ull << 20);
 }
@@ -95,11 +96,21 @@ struct svo_server {
       }
     }
     void* p = nullptr;
-    if (hipMalloc(&p, cls) != hipSuccess) {
-      (void)hipGetLastError();
-      return nullptr;
+    if (hipMalloc(&p, cls) == hipSuccess) return p;
+    (void)hipGetLastError();
+    // out of device memory: return the cached buffers and try once more
+    std::vector<void*> drop;
+    {
+      std::lock_guard<std::mutex> lk(m);
+      for (auto& kv : free_bufs) {
+        drop.insert(drop.end(), kv.second.begin(), kv.second.end());
+        kv.second.clear();
+      }
     }
-    return p;
+    for (void* q : drop) (void)hipFree(q);
+    if (hipMalloc(&p, cls) == hipSuccess) return p;
+    (void)hipGetLastError();
+    return nullptr;
   }
   void give_buf(void* p, size_t cls) {  // caller holds m
     if (p) free_bufs[cls].push_back(p);
@@ -250,6 +261,13 @@ int svo_server_create(int device, uint64_t admission_budget_mb, uint32_t max_bat
   }
   s->device = device;
   s->budget = (admission_budget_mb ? admission_budget_mb : 200ull * 1024) << 20;
+  {
+    // never admit more than three quarters of what the device has free now (the slots'
+    // own scratch and the caller's allocations live beside the admitted jobs)
+    size_t free_b = 0, total_b = 0;
+    if (hipSetDevice(device) == hipSuccess && hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+      s->budget = std::min<uint64_t>(s->budget, (uint64_t)free_b / 4 * 3);
+  }
   s->max_batch = max_batch_jobs ? std::min<uint32_t>(max_batch_jobs, svo::kMaxJobs) : 512;
   const uint32_t n = slots ? std::min<uint32_t>(slots, 8) : 2;
   for (uint32_t i = 0; i < n; i++) {

@@ -652,7 +652,10 @@ __global__ __launch_bounds__(256) void k_brick_count(const uint32_t* __restrict_
     pool[wv * kBrickWords + lane_id()] = m;
     const uint32_t c = (uint32_t)__popcll(m);
     const uint32_t tot = wave_incl_scan<uint32_t>(c);
-    if (lane_id() == kWave - 1) bcount[wv] = tot;
+    // voxel count, plus a flag in bit kNonEmptyShift: one scan gives every brick its
+    // voxel offset and its rank among the non-empty bricks (the touch pass flags a
+    // superset of the bricks that end up holding voxels)
+    if (lane_id() == kWave - 1) bcount[wv] = tot + (tot ? (1ull << kNonEmptyShift) : 0ull);
   }
 }
 
@@ -691,15 +694,19 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
     ull word = pool[slot];
     const uint32_t c = (uint32_t)__popcll(word);
     const uint32_t inc = wave_incl_scan<uint32_t>(c);
-    uint64_t pos = boff[wv] + (inc - c);
+    const uint64_t bo = boff[wv];
+    uint64_t pos = (bo & kVoxMask) + (inc - c);
     if (wprefix) wprefix[slot] = (uint32_t)pos;
     const uint64_t tag = (uint64_t)units[u].job << kJobShift;
     const uint64_t b0 = units[u].bstart + ((ti - ubase[u]) << kBrickBits);
-    // the brick stream (appended in stream order)
-    bwords[slot] = word;
-    if (lane_id() == 0) {
-      bcode[wv] = b0 | tag;
-      bvoff[wv] = s0 + boff[wv];
+    // the brick stream (non-empty bricks, appended in stream order)
+    if (__ballot(word != 0ull)) {
+      const uint64_t sb = bo >> kNonEmptyShift;
+      bwords[sb * kBrickWords + lane_id()] = word;
+      if (lane_id() == 0) {
+        bcode[sb] = b0 | tag;
+        bvoff[sb] = s0 + (bo & kVoxMask);
+      }
     }
     const uint64_t cb = (b0 + ((uint64_t)lane_id() << 6)) | tag;
     while (word) {
@@ -728,7 +735,7 @@ __global__ void k_unit_voxels(const uint32_t* __restrict__ occ, const ull* __res
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u > n_units) return;
   const uint64_t n_occ = ctr[kCtrNumOcc];
-  const uint64_t V = ctr[kCtrNumVox];
+  const uint64_t V = ctr[kCtrNumVox] & kVoxMask;
   if (u == n_units) {
     uvoff[u] = V;
     return;
@@ -742,7 +749,7 @@ __global__ void k_unit_voxels(const uint32_t* __restrict__ occ, const ull* __res
     else
       hi = mid;
   }
-  uvoff[u] = (lo < n_occ) ? boff[lo] : V;
+  uvoff[u] = (lo < n_occ) ? (boff[lo] & kVoxMask) : V;
 }
 
 // ---------------------------------------------------------------- payload records
@@ -1062,14 +1069,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   for (const VoxUnit& u : hu)
     if (u.n) units.push_back(u);  // main.cpp:330 skips empty partitions
   const uint32_t nu = (uint32_t)units.size();
-  if (nu == 0) {
-    if (batch)
-      for (Job& J : c.jobs) {
-        J.voff = c.n_vox;
-        J.nvox = 0;
-      }
-    return hipSuccess;
-  }
+  if (nu == 0) return hipSuccess;
   // unit table: VoxParams[nu] | ubase[nu+1] | tile_off[nu+1] | uvoff[nu+1]
   std::vector<VoxParams> P(nu);
   std::vector<uint64_t> ubase(nu + 1);
@@ -1251,7 +1251,8 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     c.pool_cap = want;
   }
   const uint64_t n_occ = hc[kCtrNumOcc];
-  const uint64_t nvox = hc[kCtrNumVox];
+  const uint64_t nvox = hc[kCtrNumVox] & kVoxMask;
+  const uint64_t n_nonempty = hc[kCtrNumVox] >> kNonEmptyShift;
   const uint64_t n_large = hc[kCtrLargeCount];
   if (c.n_vox + nvox >= (1ull << 40)) {
     set_error("voxel stream too long");
@@ -1262,35 +1263,30 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   const uint64_t s0 = c.n_vox;
   bool any_payload = false;
   if (batch) {
-    // the units hold every job's partitions, in job order: ranges follow from the unit
-    // offsets; a job without voxels gets an empty range where it would start
-    for (Job& J : c.jobs) {
-      J.nvox = 0;
-      J.voff = ~0ull;
-    }
+    // the passes of a batch take its jobs' partitions in job order: a job's range starts
+    // at its first voxel and grows with every pass that holds its partitions (jobs start
+    // with voff = ~0, nvox = 0; the caller places jobs without voxels afterwards)
     for (uint32_t u = 0; u < nu; u++) {
       Job& J = c.jobs[units[u].job];
-      if (J.voff == ~0ull) J.voff = s0 + uvoff[u];
-      J.nvox += uvoff[u + 1] - uvoff[u];
-    }
-    uint64_t prev_end = s0;
-    for (Job& J : c.jobs) {
-      if (J.voff == ~0ull) J.voff = prev_end;
-      prev_end = J.voff + J.nvox;
+      const uint64_t k = uvoff[u + 1] - uvoff[u];
+      if (J.voff == ~0ull && k) J.voff = s0 + uvoff[u];
+      J.nvox += k;
     }
   } else {
     Job& J = c.jobs[0];
     J.voff = 0;
     J.nvox = s0 + nvox;
   }
+  // data ranges: a job's records start after the complete ranges of the jobs before it
+  // (jobs after the last one of this pass have no records yet, their bases move later)
   uint64_t n_records = 0;
   for (size_t j = 0; j < c.jobs.size(); j++) {
     Job& J = c.jobs[j];
     J.dbase = n_records;
     J.n_data = J.payload ? 1 + J.nvox : 2;
     n_records += J.n_data;
-    any_payload |= J.payload != 0;
   }
+  for (uint32_t u = 0; u < nu; u++) any_payload |= units[u].stride == 21;
   std::vector<JobDev> jd(c.jobs.size());
   for (size_t j = 0; j < c.jobs.size(); j++) {
     const Job& J = c.jobs[j];
@@ -1304,7 +1300,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     int D = 0;
     while ((1ull << D) < J.gridsize) D++;
     d.depth = D;
-    d.voff = J.voff;
+    d.voff = J.voff == ~0ull ? 0 : J.voff;
     d.nvox = J.nvox;
     d.dbase = J.dbase;
   }
@@ -1327,7 +1323,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
                        any_payload ? c.wprefix.as<uint32_t>() : (uint32_t*)nullptr, s0,
                        c.bwords.as<uint64_t>() + nb0 * kBrickWords, c.bcode.as<uint64_t>() + nb0,
                        c.bvoff.as<uint64_t>() + nb0);
-    c.n_bricks = nb0 + n_occ;
+    c.n_bricks = nb0 + n_nonempty;
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
 
@@ -1341,7 +1337,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
       SVO_TRY(tile_pass(WinnerAction{}, "vox_small_winner", ctr + kCtrQueueC, 0, kSkipBinary));
       if (n_large) SVO_TRY(large_pass(WinnerAction{}, "vox_large_winner", kSkipBinary));
       // records: each job's first record is its null record
-      const uint64_t keep = batch ? 0 : (1 + s0) * 32;
+      const uint64_t keep = batch ? c.records.cap : (1 + s0) * 32;
       SVO_TRY(ensure_keep(c.records, n_records * 32, keep, c.stream));
       ktimer_begin(c, "payload_records");
       hipLaunchKernelGGL(k_payload_records, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
