@@ -240,6 +240,8 @@ typedef struct svo_server_stats {
   uint64_t submitted, completed, failed, batches;
   uint64_t max_batch_jobs_seen;
   uint64_t peak_admitted_bytes;
+  uint64_t batched_jobs; /* jobs over all batches */
+  double batch_ms;       /* wall time the slots spent running batches (summed) */
 } svo_server_stats;
 int svo_server_stats_get(svo_server* srv, svo_server_stats* out);
 

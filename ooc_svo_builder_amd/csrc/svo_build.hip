@@ -498,42 +498,60 @@ __global__ __launch_bounds__(256) void k_brick_level(Arena A, LevelMeta M, Tags 
   }
 }
 
-// Every record below each brick node: the word groups (height 3), words (height 2),
-// bytes (height 1) and voxels (leaves), at the positions the post-order formula gives
-// from the brick node's start.
-__global__ __launch_bounds__(256) void k_emit_bricks(Arena A, LevelMeta M, Tags T,
-                                                     const JobDev* __restrict__ jobs,
-                                                     const uint64_t* __restrict__ bwords,
-                                                     const uint64_t* __restrict__ bcode,
-                                                     const uint64_t* __restrict__ bvoff,
-                                                     uint64_t nb, uint64_t node_cap,
-                                                     uint64_t* __restrict__ nodes) {
-  if (M.flag[0]) return;
-  const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  if (w >= nb) return;  // uniform per wave
+// Every record below each brick node, written record by record: the brick's records form
+// the contiguous range [S, S + s4 - 1) laid out as the word groups' subtree ranges, then
+// the group nodes; a group's range is its words' ranges, then its word nodes; a word's
+// range is its bytes' voxels, then its byte nodes. Per brick a wave builds the offsets of
+// these ranges in LDS, then lane r writes records r, r + 64, ... of the range (looking
+// its node up in the offsets), so consecutive lanes store consecutive records.
+constexpr int kEmitWaves = 4;  // waves per block of k_emit_bricks
+
+// last k in [0, 8) with a[k] <= v (a non-decreasing, a[0] <= v)
+__device__ __forceinline__ int last_le8(const uint32_t* a, uint32_t v) {
+  int k = 0;
+  if (a[k + 4] <= v) k += 4;
+  if (a[k + 2] <= v) k += 2;
+  if (a[k + 1] <= v) k += 1;
+  return k;
+}
+// position of the n-th set bit of m (n < popcount(m))
+__device__ __forceinline__ int nth_bit(uint32_t m, uint32_t n) {
+  for (uint32_t q = 0; q < n; q++) m &= m - 1;
+  return __ffs(m) - 1;
+}
+
+__global__ __launch_bounds__(kWave * kEmitWaves) void k_emit_bricks(
+    Arena A, LevelMeta M, Tags T, const JobDev* __restrict__ jobs,
+    const uint64_t* __restrict__ bwords, const uint64_t* __restrict__ bcode,
+    const uint64_t* __restrict__ bvoff, uint64_t nb, uint64_t node_cap,
+    uint64_t* __restrict__ nodes) {
+  __shared__ uint32_t s_go[kEmitWaves][9];    // group ranges (relative to S)
+  __shared__ uint32_t s_gs3[kEmitWaves][8];   // group subtree sizes
+  __shared__ uint32_t s_gm3[kEmitWaves][8];   // group child masks (non-zero words)
+  __shared__ uint32_t s_wo[kEmitWaves][64];   // word ranges (relative to S)
+  __shared__ uint32_t s_s2[kEmitWaves][64];   // word subtree sizes
+  __shared__ uint32_t s_m2[kEmitWaves][64];   // word child masks (non-zero bytes)
+  __shared__ uint32_t s_vw[kEmitWaves][64];   // voxels before the word in the brick
+  __shared__ uint64_t s_wd[kEmitWaves][64];   // the words
+  __shared__ uint32_t s_bo[kEmitWaves][512];  // byte voxel ranges (relative to S)
+  const int ws = threadIdx.x / kWave;
   const int lane = lane_id();
   const int g = lane >> 3, q = lane & 7;
-  const BrickShape B = brick_shape(bwords, w);
-  const JobDev& J = jobs[T.job(bcode[w])];
-  const uint64_t S = A.start[M.off[4] + w];
-  const uint64_t lim = J.n_nodes;
-  bool bad = false;
-  // the brick's own children (groups): start = S + sum of (s3 - 1) of the groups before
-  const uint32_t cb4 = B.s4 - 1u - (uint32_t)__popc(B.mask4);
+  const uint64_t w = (uint64_t)blockIdx.x * kEmitWaves + ws;
+  const bool live = w < nb && !M.flag[0];
+  // every wave of the block reaches the barrier below; a wave without a brick works on
+  // zeros and writes nothing
+  BrickShape B;
+  if (live) {
+    B = brick_shape(bwords, w);
+  } else {
+    B.word = 0;
+    B.m2 = B.s2 = B.mask3 = B.s3 = B.mask4 = 0;
+    B.s4 = 1;
+  }
   const uint32_t v3 = (q == 0 && B.s3) ? B.s3 - 1u : 0u;
   const uint32_t inc3 = wave_incl_scan<uint32_t>(v3);
   const uint32_t pre3 = (uint32_t)__shfl((int)(inc3 - v3), g * 8, kWave);
-  const uint64_t start3 = S + pre3;
-  const uint64_t cb3 = start3 + B.s3 - 1u - (uint32_t)__popc(B.mask3);
-  if (q == 0 && B.s3) {
-    const uint64_t pos = S + cb4 + (uint32_t)__popc(B.mask4 & ((1u << g) - 1u));
-    if (pos < lim && J.nbase + pos < node_cap)
-      put_node(nodes, J.nbase + pos, 0ull, cb3, offsets_of(B.mask3));
-    else
-      bad = true;
-  }
-  // this lane's word node: start = start3 + sum of (s2 - 1) of the words before it in
-  // the group
   const uint32_t v2 = B.word ? B.s2 - 1u : 0u;
   uint32_t inc2 = v2;
 #pragma unroll
@@ -541,47 +559,82 @@ __global__ __launch_bounds__(256) void k_emit_bricks(Arena A, LevelMeta M, Tags 
     const uint32_t o = (uint32_t)__shfl_up((int)inc2, d, kWave);
     if (q >= d) inc2 += o;
   }
-  const uint64_t start2 = start3 + (inc2 - v2);
-  const uint64_t cb2 = start2 + B.s2 - 1u - (uint32_t)__popc(B.m2);
-  // voxels before this word in the brick (for payload data indices)
+  const uint32_t wo = pre3 + (inc2 - v2);
   const uint32_t pw = (uint32_t)__popcll(B.word);
   const uint32_t incw = wave_incl_scan<uint32_t>(pw);
-  const uint64_t vox0 = bvoff[w] + (incw - pw) - J.voff;
-  if (B.word) {
-    const uint64_t pos2 = cb3 + (uint32_t)__popc(B.mask3 & ((1u << q) - 1u));
-    if (pos2 < lim && J.nbase + pos2 < node_cap)
-      put_node(nodes, J.nbase + pos2, 0ull, cb2, offsets_of(B.m2));
-    else
-      bad = true;
-    // byte nodes and their voxels
-    uint64_t start1 = start2;
-    uint32_t rank1 = 0;
-    uint32_t vr = 0;
-    for (int j = 0; j < 8; j++) {
-      const uint32_t byte = (uint32_t)(B.word >> (8 * j)) & 0xFFu;
-      if (!byte) continue;
-      const uint64_t pos1 = cb2 + rank1;
-      // s1 - 1 - popc(byte) = 0: the byte's leaves start at its own start
-      const uint64_t cb1 = start1;
-      if (pos1 < lim && J.nbase + pos1 < node_cap)
-        put_node(nodes, J.nbase + pos1, 0ull, cb1, offsets_of(byte));
-      else
-        bad = true;
-      const uint32_t nl = (uint32_t)__popc(byte);
-      for (uint32_t r = 0; r < nl; r++) {
-        const uint64_t posl = cb1 + r;
-        const uint64_t data = J.payload ? 1ull + vox0 + vr : 1ull;
-        if (posl < lim && J.nbase + posl < node_cap)
-          put_node(nodes, J.nbase + posl, data, 0ull, ~0ull);
-        else
-          bad = true;
-        vr++;
-      }
-      start1 += nl;  // s1 - 1 of this byte
-      rank1++;
-    }
+  if (q == 0) {
+    s_go[ws][g] = pre3;
+    s_gs3[ws][g] = B.s3;
+    s_gm3[ws][g] = B.mask3;
   }
-  if (bad) M.flag[1] = 1;
+  if (lane == 0) s_go[ws][8] = B.s4 - 1u - (uint32_t)__popc(B.mask4);
+  s_wo[ws][lane] = wo;
+  s_s2[ws][lane] = B.s2;
+  s_m2[ws][lane] = B.m2;
+  s_vw[ws][lane] = incw - pw;
+  s_wd[ws][lane] = B.word;
+  uint32_t bo = wo;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    s_bo[ws][lane * 8 + j] = bo;
+    bo += (uint32_t)__popc((uint32_t)(B.word >> (8 * j)) & 0xFFu);
+  }
+  __syncthreads();
+  if (!live) return;
+  const JobDev& J = jobs[T.job(bcode[w])];
+  const uint64_t S = A.start[M.off[4] + w];
+  const uint64_t vox0 = bvoff[w] - J.voff;
+  const uint32_t R = B.s4 - 1u;        // records below the brick node
+  const uint32_t RG = s_go[ws][8];     // records inside the groups' ranges
+  if (S + R > J.n_nodes || J.nbase + S + R > node_cap) {
+    if (lane == 0) M.flag[1] = 1;
+    return;
+  }
+  uint64_t* out = nodes + 3 * (J.nbase + S);
+  for (uint32_t r = lane; r < R; r += kWave) {
+    uint64_t d0, d1, d2;
+    if (r >= RG) {  // a group node
+      const int gg = nth_bit(B.mask4, r - RG);
+      const uint32_t m3 = s_gm3[ws][gg];
+      d0 = 0;
+      d1 = S + s_go[ws][gg] + s_gs3[ws][gg] - 1u - (uint32_t)__popc(m3);
+      d2 = offsets_of(m3);
+    } else {
+      const int gg = last_le8(s_go[ws], r);
+      const uint32_t m3 = s_gm3[ws][gg];
+      const uint32_t r1 = r - s_go[ws][gg];
+      const uint32_t l3 = s_gs3[ws][gg] - 1u, nw = (uint32_t)__popc(m3);
+      if (r1 >= l3 - nw) {  // a word node
+        const int ww = gg * 8 + nth_bit(m3, r1 - (l3 - nw));
+        const uint32_t m2 = s_m2[ws][ww];
+        d0 = 0;
+        d1 = S + s_wo[ws][ww] + s_s2[ws][ww] - 1u - (uint32_t)__popc(m2);
+        d2 = offsets_of(m2);
+      } else {
+        const int ww = gg * 8 + last_le8(&s_wo[ws][gg * 8], r);
+        const uint32_t m2 = s_m2[ws][ww];
+        const uint32_t r2 = r - s_wo[ws][ww];
+        const uint32_t l2 = s_s2[ws][ww] - 1u, nbz = (uint32_t)__popc(m2);
+        const uint64_t word = s_wd[ws][ww];
+        if (r2 >= l2 - nbz) {  // a byte node: its voxels start at its own range start
+          const int j = nth_bit(m2, r2 - (l2 - nbz));
+          d0 = 0;
+          d1 = S + s_bo[ws][ww * 8 + j];
+          d2 = offsets_of((uint32_t)(word >> (8 * j)) & 0xFFu);
+        } else {  // a voxel
+          const int j = last_le8(&s_bo[ws][ww * 8], r);
+          const uint32_t k = r - s_bo[ws][ww * 8 + j];
+          const uint32_t before = (uint32_t)__popcll(word & ((1ull << (8 * j)) - 1ull));
+          d0 = J.payload ? 1ull + vox0 + s_vw[ws][ww] + before + k : 1ull;
+          d1 = 0;
+          d2 = ~0ull;
+        }
+      }
+    }
+    out[3 * (uint64_t)r] = d0;
+    out[3 * (uint64_t)r + 1] = d1;
+    out[3 * (uint64_t)r + 2] = d2;
+  }
 }
 
 #define SVO_TRY(x)                   \
@@ -732,7 +785,8 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     }
     if (fast) {
       ktimer_begin(c, "emit_bricks");
-      hipLaunchKernelGGL(k_emit_bricks, dim3((unsigned)((NB * kWave + 255) / 256)), dim3(256), 0,
+      hipLaunchKernelGGL(k_emit_bricks, dim3((unsigned)((NB + kEmitWaves - 1) / kEmitWaves)),
+                         dim3(kWave * kEmitWaves), 0,
                          c.stream, A, M, T, (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),
                          c.bcode.as<uint64_t>(), c.bvoff.as<uint64_t>(), NB, node_cap,
                          c.nodes.as<uint64_t>());

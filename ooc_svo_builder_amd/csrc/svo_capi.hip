@@ -21,8 +21,18 @@ static thread_local std::string g_err;
 
 void set_error(const std::string& msg) { g_err = msg; }
 
+// Growth policy: a buffer that has to grow takes 1.5x its old size when that is more
+// (bounded by twice the request), so that a sequence of slightly larger requests (the
+// batches of a server) settles after a few reallocations; every hipFree waits for the
+// whole device.
+static size_t grown(const DevBuf& b, size_t bytes) {
+  if (!b.p) return std::max<size_t>(bytes, 256);
+  return std::max<size_t>(bytes, std::min<size_t>(b.cap + b.cap / 2, 2 * bytes));
+}
+
 hipError_t ensure(DevBuf& b, size_t bytes) {
   if (bytes <= b.cap && b.p) return hipSuccess;
+  const size_t want = grown(b, bytes);
   if (b.p) {
     hipError_t e = hipDeviceSynchronize();
     if (e != hipSuccess) return e;
@@ -30,7 +40,6 @@ hipError_t ensure(DevBuf& b, size_t bytes) {
     b.p = nullptr;
     b.cap = 0;
   }
-  size_t want = std::max<size_t>(bytes, 256);
   hipError_t e = hipMalloc(&b.p, want);
   if (e != hipSuccess) {
     b.p = nullptr;
@@ -44,7 +53,7 @@ hipError_t ensure(DevBuf& b, size_t bytes) {
 
 hipError_t ensure_keep(DevBuf& b, size_t bytes, size_t used, hipStream_t s) {
   if (bytes <= b.cap && b.p) return hipSuccess;
-  size_t want = std::max<size_t>(bytes, 256);
+  const size_t want = grown(b, bytes);
   void* np = nullptr;
   hipError_t e = hipMalloc(&np, want);
   if (e != hipSuccess) {
@@ -229,7 +238,9 @@ int svo_ctx_create(int device, svo_ctx** out) {
   ctx->c.own_stream = true;
   if ((e = ensure(ctx->c.counters, kNumCounters * 8)) != hipSuccess ||
       (e = hipEventCreate(&ctx->c.ev0)) != hipSuccess ||
-      (e = hipEventCreate(&ctx->c.ev1)) != hipSuccess) {
+      (e = hipEventCreate(&ctx->c.ev1)) != hipSuccess ||
+      (e = hipEventCreate(&ctx->c.ev2)) != hipSuccess ||
+      (e = hipEventCreate(&ctx->c.ev3)) != hipSuccess) {
     svo_ctx_destroy(ctx);
     return fail(e, "svo_ctx_create");
   }
@@ -268,8 +279,8 @@ void svo_ctx_destroy(svo_ctx* ctx) {
   }
   ktimer_flush(c);
   for (hipEvent_t e : c.evpool) (void)hipEventDestroy(e);
-  if (c.ev0) (void)hipEventDestroy(c.ev0);
-  if (c.ev1) (void)hipEventDestroy(c.ev1);
+  for (hipEvent_t ev : {c.ev0, c.ev1, c.ev2, c.ev3})
+    if (ev) (void)hipEventDestroy(ev);
   if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
   delete ctx;
 }
@@ -642,33 +653,30 @@ int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payloa
   const float u_part = (bbox[3] - bbox[0]) / (float)gridsize;  // partitioner.cpp:45
   const float u_vox =
       (text_roundtrip(bbox[3]) - text_roundtrip(bbox[0])) / (float)gridsize;  // main.cpp:311
+  // stage boundaries as events, read once the build has synchronized
   float ms_p = 0, ms_v = 0, ms_b = 0;
   TRYC(hipEventRecord(c.ev0, c.stream), "svo_run");
   std::vector<uint64_t> counts(np, 0);
   r = svo_partition(ctx, u_part, gridsize, np, counts.data());
   if (r) return r;
   TRYC(hipEventRecord(c.ev1, c.stream), "svo_run");
-  TRYC(hipEventSynchronize(c.ev1), "svo_run");
-  TRYC(hipEventElapsedTime(&ms_p, c.ev0, c.ev1), "svo_run");
   const uint64_t morton_part = (gridsize * gridsize * gridsize) / np;
-  TRYC(hipEventRecord(c.ev0, c.stream), "svo_run");
   for (uint64_t i = 0; i < np; i++) {
     if (counts[i] == 0) continue;  // main.cpp:330
     r = svo_voxelize(ctx, i, i * morton_part, (i + 1) * morton_part, u_vox, gridsize, color_mode,
                      nullptr);
     if (r) return r;
   }
-  TRYC(hipEventRecord(c.ev1, c.stream), "svo_run");
-  TRYC(hipEventSynchronize(c.ev1), "svo_run");
-  TRYC(hipEventElapsedTime(&ms_v, c.ev0, c.ev1), "svo_run");
-  TRYC(hipEventRecord(c.ev0, c.stream), "svo_run");
+  TRYC(hipEventRecord(c.ev2, c.stream), "svo_run");
   uint64_t nn = 0, nd = 0;
   int hz = 0;
   r = svo_build(ctx, gridsize, generate_levels, &nn, &nd, &hz);
   if (r) return r;
-  TRYC(hipEventRecord(c.ev1, c.stream), "svo_run");
-  TRYC(hipEventSynchronize(c.ev1), "svo_run");
-  TRYC(hipEventElapsedTime(&ms_b, c.ev0, c.ev1), "svo_run");
+  TRYC(hipEventRecord(c.ev3, c.stream), "svo_run");
+  TRYC(hipEventSynchronize(c.ev3), "svo_run");
+  TRYC(hipEventElapsedTime(&ms_p, c.ev0, c.ev1), "svo_run");
+  TRYC(hipEventElapsedTime(&ms_v, c.ev1, c.ev2), "svo_run");
+  TRYC(hipEventElapsedTime(&ms_b, c.ev2, c.ev3), "svo_run");
   if (info) {
     info->n_partitions = np;
     info->nfilled = c.nfilled;
@@ -680,14 +688,6 @@ int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payloa
     info->ms_build = ms_b;
   }
   return SVO_OK;
-}
-
-static float elapsed(Context& c) {
-  float ms = 0;
-  if (hipEventRecord(c.ev1, c.stream) != hipSuccess || hipEventSynchronize(c.ev1) != hipSuccess ||
-      hipEventElapsedTime(&ms, c.ev0, c.ev1) != hipSuccess)
-    return 0.0f;
-  return ms;
 }
 
 int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_result* results,
@@ -748,7 +748,7 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
     TRYC(partition_job(c, j, u_part, q.gridsize, np), "svo_batch_run");
     total_parts += np;
   }
-  ms_p = elapsed(c);
+  TRYC(hipEventRecord(c.ev1, c.stream), "svo_batch_run");
   std::vector<VoxUnit> units;
   for (uint32_t j = 0; j < n_jobs; j++) {
     const Job& J = c.jobs[j];
@@ -769,7 +769,6 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
       units.push_back(u);
     }
   }
-  TRYC(hipEventRecord(c.ev0, c.stream), "svo_batch_run");
   // passes of consecutive units whose Morton spans fit the dense byte grid together (a
   // larger single unit is a pass of its own, on the touched-brick path)
   for (Job& J : c.jobs) {
@@ -802,11 +801,14 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
     }
   }
   c.nfilled = nf;
-  ms_v = elapsed(c);
-  TRYC(hipEventRecord(c.ev0, c.stream), "svo_batch_run");
+  TRYC(hipEventRecord(c.ev2, c.stream), "svo_batch_run");
   hipError_t e = build_jobs(c, n_jobs, true, 0);
   if (e != hipSuccess) return fail(e, "svo_batch_run");
-  ms_b = elapsed(c);
+  TRYC(hipEventRecord(c.ev3, c.stream), "svo_batch_run");
+  TRYC(hipEventSynchronize(c.ev3), "svo_batch_run");
+  TRYC(hipEventElapsedTime(&ms_p, c.ev0, c.ev1), "svo_batch_run");
+  TRYC(hipEventElapsedTime(&ms_v, c.ev1, c.ev2), "svo_batch_run");
+  TRYC(hipEventElapsedTime(&ms_b, c.ev2, c.ev3), "svo_batch_run");
   unsigned long long err = 0;
   TRYC(copy_sync(c, &err, c.counters.as<unsigned long long>() + kCtrError, 8),
        "svo_batch_run");

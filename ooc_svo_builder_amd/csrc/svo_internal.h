@@ -189,7 +189,7 @@ struct Context {
   size_t stage_cap = 0, stage_used = 0;
 
   StageTimes times;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
 
   bool ktiming = false;
   std::vector<PendingTimer> kpending;

@@ -214,10 +214,14 @@ void svo_server::worker(uint32_t slot) {
       stats.max_batch_jobs_seen = std::max<uint64_t>(stats.max_batch_jobs_seen, batch.size());
       stats.peak_admitted_bytes = std::max<uint64_t>(stats.peak_admitted_bytes, admitted);
     }
+    const Clock::time_point tb = Clock::now();
     const int rc = run_jobs(slot, batch);
+    const double ms = std::chrono::duration<double, std::milli>(Clock::now() - tb).count();
     {
       std::lock_guard<std::mutex> lk(m);
       const Clock::time_point now = Clock::now();
+      stats.batch_ms += ms;
+      stats.batched_jobs += batch.size();
       for (Ticket* t : batch) {
         admitted -= t->work_bytes;
         t->work_bytes = 0;

@@ -1111,6 +1111,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   }
   if (dense) {
     if (c.grid_zeroed < dense_bytes) {
+      if (c.grid8.cap < dense_bytes) release(c.grid8);  // exact size, no growth slack
       SVO_TRY(ensure(c.grid8, dense_bytes));
       SVO_TRY(fill_u64(c.grid8.as<uint64_t>(), 0, dense_bytes / 8, c.stream));
       c.grid_zeroed = dense_bytes;

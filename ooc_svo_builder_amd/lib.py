@@ -58,7 +58,8 @@ class JobResult(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
                 ("batches", C.c_uint64), ("max_batch_jobs_seen", C.c_uint64),
-                ("peak_admitted_bytes", C.c_uint64)]
+                ("peak_admitted_bytes", C.c_uint64), ("batched_jobs", C.c_uint64),
+                ("batch_ms", C.c_double)]
 
 
 SIGNATURES = {
