@@ -781,7 +781,9 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
     uint64_t span = 0;
     while (u1 < units.size()) {
       const uint64_t bstart = units[u1].mstart & ~(uint64_t)(kBrickVox - 1);
-      const uint64_t ub = ((units[u1].mend - bstart + kBrickVox - 1) >> kBrickBits) << kBrickBits;
+      // dense cells: a byte per voxel, 4 bytes for payload meshes
+      const uint64_t ub = (((units[u1].mend - bstart + kBrickVox - 1) >> kBrickBits)
+                           << kBrickBits) * (units[u1].stride == 21 ? 4 : 1);
       if (u1 > u0 && span + ub > c.dense_limit) break;
       span += ub;
       u1++;

@@ -51,6 +51,7 @@ struct VoxParams {
   int small_cols;
   uint32_t job;        // tag of the unit's codes (bits kJobShift and up)
   int dense;           // 1: voxels in grid8; 0: in pool slots flagged by the touch pass
+  int wide;            // dense payload unit: 4-byte cells holding ~(first triangle)
   uint64_t table_base; // first entry of the unit's segment of the brick table
   uint8_t* grid8;      // dense byte grid of the unit (nullptr on the touch path)
 };
@@ -366,7 +367,8 @@ struct TouchAction {
 //  * dense units: a byte per voxel of the unit's Morton range (the reference's own layout,
 //    voxelizer.cpp:144, kept in HBM); the brick's table entry is flagged with a plain
 //    store when the thread enters a brick. Only flagged bricks are compacted and cleared
-//    afterwards, so the grid is never swept.
+//    afterwards, so the grid is never swept. Dense payload units keep 4 bytes per voxel
+//    and resolve the winning triangle in the same walk (atomicMax of ~triangle).
 //  * touch units: 4096 bytes per pool slot; the slot comes from the table (touch pass +
 //    table scan), cached per thread.
 struct MarkAction {
@@ -376,7 +378,7 @@ struct MarkAction {
   ull* ctr;
   uint64_t pool_cap;
   uint64_t base;  // code of the first voxel of the unit's first brick
-  int dense;
+  int dense, wide;
   uint32_t cbrick = 0xFFFFFFFFu, cslot = 0;
   __device__ static MarkAction make(const ActArgs& A, const VoxParams& P) {
     MarkAction m;
@@ -386,14 +388,20 @@ struct MarkAction {
     m.pool_cap = A.pool_cap;
     m.base = P.bstart;
     m.dense = P.dense;
+    m.wide = P.wide;
     return m;
   }
   __device__ __forceinline__ void touch(uint64_t) {}
-  __device__ __forceinline__ void add(uint64_t code, uint32_t) {
+  __device__ __forceinline__ void add(uint64_t code, uint32_t tri) {
     const uint64_t rel = code - base;
     const uint32_t b = (uint32_t)(rel >> kBrickBits);
     if (dense) {
-      bytes[rel] = 1;
+      // payload: the lowest covering triangle (voxelizer.cpp:263, first in partition
+      // order wins) as the largest ~tri; 0 = unmarked
+      if (wide)
+        atomicMax((uint32_t*)bytes + rel, ~tri);
+      else
+        bytes[rel] = 1;
       if (b != cbrick) {
         cbrick = b;
         table[b] = 1u;
@@ -631,23 +639,36 @@ __global__ __launch_bounds__(256) void k_brick_count(const uint32_t* __restrict_
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
     const uint64_t ti = occ[wv];
     const uint32_t u = upper_index<uint64_t>(ubase, n_units + 1, ti);
-    uint8_t* base = units[u].dense ? units[u].grid8 + (ti - ubase[u]) * kBrickVox
-                                   : pool8 + wv * kBrickVox;
-    uint4* src = (uint4*)(base + lane_id() * 64);
-    uint4 v[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) v[q] = src[q];
     ull m = 0;
+    if (units[u].wide) {
+      // 4-byte cells (~winner, 0 = unmarked): kept for k_brick_emit, which clears them
+      const uint4* src = (const uint4*)((const uint32_t*)units[u].grid8 +
+                                        (ti - ubase[u]) * kBrickVox + lane_id() * 64);
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const uint32_t w4[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
-#pragma unroll
-      for (int d = 0; d < 4; d++) {
-        const uint32_t x = w4[d];  // four bytes, each 0 or 1
-        const uint32_t nib = (x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u);
-        m |= (ull)nib << (16 * q + 4 * d);
+      for (int q = 0; q < 16; q++) {
+        const uint4 v = src[q];
+        const uint32_t nib = (v.x != 0u ? 1u : 0u) | (v.y != 0u ? 2u : 0u) |
+                             (v.z != 0u ? 4u : 0u) | (v.w != 0u ? 8u : 0u);
+        m |= (ull)nib << (4 * q);
       }
-      if (v[q].x | v[q].y | v[q].z | v[q].w) src[q] = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+      uint8_t* base = units[u].dense ? units[u].grid8 + (ti - ubase[u]) * kBrickVox
+                                     : pool8 + wv * kBrickVox;
+      uint4* src = (uint4*)(base + lane_id() * 64);
+      uint4 v[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) v[q] = src[q];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t w4[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+          const uint32_t x = w4[d];  // four bytes, each 0 or 1
+          const uint32_t nib = (x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u);
+          m |= (ull)nib << (16 * q + 4 * d);
+        }
+        if (v[q].x | v[q].y | v[q].z | v[q].w) src[q] = make_uint4(0u, 0u, 0u, 0u);
+      }
     }
     pool[wv * kBrickWords + lane_id()] = m;
     const uint32_t c = (uint32_t)__popcll(m);
@@ -685,7 +706,8 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ wprefix,
                                                     uint64_t s0, uint64_t* __restrict__ bwords,
                                                     uint64_t* __restrict__ bcode,
-                                                    uint64_t* __restrict__ bvoff) {
+                                                    uint64_t* __restrict__ bvoff,
+                                                    uint32_t* __restrict__ winner) {
   const uint64_t n_occ = ctr[kCtrNumOcc];
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
     const uint64_t ti = occ[wv];
@@ -709,8 +731,16 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
       }
     }
     const uint64_t cb = (b0 + ((uint64_t)lane_id() << 6)) | tag;
+    // dense payload unit: each voxel's winner from its cell, which is cleared
+    uint32_t* cells = units[u].wide ? (uint32_t*)units[u].grid8 + (ti - ubase[u]) * kBrickVox +
+                                          lane_id() * 64
+                                    : nullptr;
     while (word) {
       const int bit = __ffsll(word) - 1;
+      if (cells) {
+        winner[pos] = ~cells[bit];
+        cells[bit] = 0u;
+      }
       codes[pos++] = cb + (uint64_t)bit;
       word &= word - 1;
     }
@@ -1058,6 +1088,7 @@ static void fill_params(const VoxUnit& h, const Context& c, VoxParams& P) {
   P.small_cols = c.tune.small_cols;
   P.job = h.job;
   P.dense = 0;
+  P.wide = 0;
   P.table_base = 0;
   P.grid8 = nullptr;
 }
@@ -1099,9 +1130,14 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     set_error("voxelize pass too large for 32-bit brick, tile or triangle indices");
     return hipErrorInvalidValue;
   }
-  // dense path: a byte per voxel of every unit when that fits the HBM budget (no
-  // allocation pass needed); otherwise bricks get pool slots from a touch pass
-  const uint64_t dense_bytes = ntab << kBrickBits;
+  // dense path: a cell per voxel of every unit (a byte, 4 bytes for payload units) when
+  // that fits the HBM budget (no allocation pass needed); otherwise bricks get pool slots
+  // from a touch pass
+  std::vector<uint64_t> goff(nu + 1, 0);
+  for (uint32_t u = 0; u < nu; u++)
+    goff[u + 1] = goff[u] + ((ubase[u + 1] - ubase[u]) << kBrickBits) *
+                                (units[u].stride == 21 ? 4 : 1);
+  const uint64_t dense_bytes = goff[nu];
   const bool dense = dense_bytes <= c.dense_limit;
   // the table, the dense grid and the byte pool are all-zero whenever no pass is running
   if (c.table_entries < ntab) {
@@ -1119,7 +1155,8 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     SVO_TRY(ensure(c.pool, ntab * kBrickWords * 8));
     for (uint32_t u = 0; u < nu; u++) {
       P[u].dense = 1;
-      P[u].grid8 = c.grid8.as<uint8_t>() + (ubase[u] << kBrickBits);
+      P[u].wide = units[u].stride == 21 ? 1 : 0;
+      P[u].grid8 = c.grid8.as<uint8_t>() + goff[u];
     }
   } else if (c.pool_cap == 0) {
     const uint64_t cap = std::max<uint64_t>(1024, ntri / 64);
@@ -1310,7 +1347,15 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
 
   // ---- phase B: emit sorted codes, payload, clear; no synchronization
   if (n_occ) {
-    if (any_payload) SVO_TRY(ensure(c.wprefix, n_occ * kBrickWords * 4));
+    // dense passes resolve payload winners while marking (emitted with the codes); the
+    // touched-brick path finds them with a second walk
+    const bool winner_walk = any_payload && !dense;
+    if (any_payload && nvox > 0xFFFFFFF0ull) {
+      set_error("voxelize pass holds more than 2^32 voxels");
+      return hipErrorInvalidValue;
+    }
+    if (winner_walk) SVO_TRY(ensure(c.wprefix, n_occ * kBrickWords * 4));
+    if (any_payload) SVO_TRY(ensure(c.winner, nvox * 4));
     SVO_TRY(ensure_keep(c.codes, (s0 + nvox) * 8, s0 * 8, c.stream));
     const uint64_t nb0 = c.n_bricks;
     SVO_TRY(ensure_keep(c.bwords, (nb0 + n_occ) * kBrickWords * 8, nb0 * kBrickWords * 8,
@@ -1321,22 +1366,20 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     hipLaunchKernelGGL(k_brick_emit, dim3(brick_grid(n_occ)), dim3(256), 0, c.stream,
                        c.occ_list.as<uint32_t>(), ctr, d_units, d_ubase, nu, c.pool.as<ull>(),
                        c.bcount.as<uint64_t>(), c.codes.as<uint64_t>() + s0,
-                       any_payload ? c.wprefix.as<uint32_t>() : (uint32_t*)nullptr, s0,
+                       winner_walk ? c.wprefix.as<uint32_t>() : (uint32_t*)nullptr, s0,
                        c.bwords.as<uint64_t>() + nb0 * kBrickWords, c.bcode.as<uint64_t>() + nb0,
-                       c.bvoff.as<uint64_t>() + nb0);
+                       c.bvoff.as<uint64_t>() + nb0,
+                       any_payload ? c.winner.as<uint32_t>() : (uint32_t*)nullptr);
     c.n_bricks = nb0 + n_nonempty;
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
 
     if (any_payload) {
-      if (nvox > 0xFFFFFFF0ull) {
-        set_error("voxelize pass holds more than 2^32 voxels");
-        return hipErrorInvalidValue;
+      if (winner_walk) {
+        SVO_TRY(fill_u32(c.winner.as<uint32_t>(), 0xFFFFFFFFu, nvox, c.stream));
+        SVO_TRY(tile_pass(WinnerAction{}, "vox_small_winner", ctr + kCtrQueueC, 0, kSkipBinary));
+        if (n_large) SVO_TRY(large_pass(WinnerAction{}, "vox_large_winner", kSkipBinary));
       }
-      SVO_TRY(ensure(c.winner, nvox * 4));
-      SVO_TRY(fill_u32(c.winner.as<uint32_t>(), 0xFFFFFFFFu, nvox, c.stream));
-      SVO_TRY(tile_pass(WinnerAction{}, "vox_small_winner", ctr + kCtrQueueC, 0, kSkipBinary));
-      if (n_large) SVO_TRY(large_pass(WinnerAction{}, "vox_large_winner", kSkipBinary));
       // records: each job's first record is its null record
       const uint64_t keep = batch ? c.records.cap : (1 + s0) * 32;
       SVO_TRY(ensure_keep(c.records, n_records * 32, keep, c.stream));
