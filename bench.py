@@ -62,6 +62,9 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the served section")
     ap.add_argument("--only", choices=["headline", "batched", "served"],
                     help="run one section only (profiling)")
+    ap.add_argument("--knobs", default=None,
+                    help="voxelizer knobs small_cols,direct_cols,tile_blocks,large_blocks "
+                         "for the headline (default: the library's)")
     ap.add_argument("--mix-jobs", type=int, default=512, help="meshes in the job mix")
     ap.add_argument("--served-seconds", type=float, default=2.0,
                     help="arrival window per rate of the served sweep")
@@ -98,6 +101,8 @@ def main():
     torch.cuda.synchronize()
 
     ctx = SvoContext(0)
+    if args.knobs:
+        ctx.set_knobs(*[int(x) for x in args.knobs.split(",")])
     ctx.load_mesh(d_tris, payload)
     stream = torch.cuda.ExternalStream(ctx.stream)
     for _ in range(args.warmup):
@@ -185,6 +190,8 @@ def main():
             "build": build_bytes / (stage_ms["build"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
         "kernels_ms_per_step": {k: v[1] / kt_steps for k, v in ktimes.items()},
+        "knobs": dict(zip(("small_cols", "direct_cols", "tile_blocks", "large_blocks"),
+                          ctx.get_tuning())),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_ms": avg_launch_ms,
@@ -336,8 +343,8 @@ class MixBench:
                 "plain_loop_seconds": self.plain_s, "plain_jobs_per_s": n / self.plain_s,
                 "speedup_vs_plain": self.plain_s / best,
                 "identical_jobs": same, "identical": same == n,
-                "autotuned": {"small_cols": knobs[0], "tile_blocks": knobs[1],
-                              "large_blocks": knobs[2]},
+                "autotuned": {"small_cols": knobs[0], "direct_cols": knobs[1],
+                              "tile_blocks": knobs[2], "large_blocks": knobs[3]},
                 "voxels": int(info.nfilled), "nodes": int(info.n_nodes)}
 
     def served(self, seconds):

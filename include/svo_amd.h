@@ -186,14 +186,17 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
                   svo_run_info* info);
 
 /* Autotuner (no reference counterpart): runs the batch once per candidate value of the
- * voxelizer's knobs — the small/large-triangle split threshold and the grid sizes of the
- * persistent small-tier launch and of the large tier — by coordinate descent, keeping the
- * setting with the lowest voxelize device time on the context. No knob changes any
- * output. best_out (optional) receives {small_cols, tile_blocks, large_blocks}; the
- * context's batch outputs afterwards are those of the last run. */
-int svo_ctx_autotune(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, int best_out[3]);
-/* The context's current {small_cols, tile_blocks, large_blocks}. */
-int svo_ctx_get_tuning(svo_ctx* ctx, int out[3]);
+ * voxelizer's knobs — the small/large-triangle split threshold, the column count up to
+ * which a triangle's own thread walks it, and the grid sizes of the persistent
+ * small-tier launch and of the large tier — by coordinate descent, keeping the setting
+ * with the lowest voxelize device time on the context. No knob changes any output.
+ * best_out (optional) receives {small_cols, direct_cols, tile_blocks, large_blocks};
+ * the context's batch outputs afterwards are those of the last run. */
+int svo_ctx_autotune(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, int best_out[4]);
+/* The context's current knobs {small_cols, direct_cols, tile_blocks, large_blocks}, and
+ * setting them explicitly. */
+int svo_ctx_get_tuning(svo_ctx* ctx, int out[4]);
+int svo_ctx_set_knobs(svo_ctx* ctx, const int knobs[4]);
 
 /* Served form (no reference counterpart). A server owns 'slots' contexts, each with its
  * own stream and a worker thread. Submitted jobs queue in arrival order; a free slot

@@ -802,10 +802,12 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
       SVO_TRY(hipGetLastError());
     }
     unsigned long long flags[3] = {0, 0, 0};
-    SVO_TRY(hipMemcpyAsync(flags, M.flag, 24, hipMemcpyDeviceToHost, c.stream));
-    SVO_TRY(hipMemcpyAsync(jd.data(), d_jobs, n_jobs * sizeof(JobDev), hipMemcpyDeviceToHost,
-                           c.stream));
-    SVO_TRY(hipStreamSynchronize(c.stream));
+    {
+      const ReadReq rq[3] = {{flags, M.flag, 24},
+                             {jd.data(), d_jobs, n_jobs * sizeof(JobDev)},
+                             {&c.err_count, c.counters.as<unsigned long long>() + kCtrError, 8}};
+      SVO_TRY(read_back(c, rq, 3));
+    }
     ktimer_flush(c);
     if (flags[1] && !flags[0]) {
       set_error("internal: node record out of range");

@@ -164,6 +164,12 @@ static int fail(hipError_t e, const char* where) {
 
 static int is_pow2_u64(uint64_t x) { return x != 0 && !(x & (x - 1)); }
 
+// the sticky consistency-failure counter
+static hipError_t read_err(Context& c, unsigned long long* err) {
+  *err = c.err_count;  // read with the build's results
+  return hipSuccess;
+}
+
 // Copies on the context's own stream (never the null stream, which would wait for every
 // other context's work), direction from the pointers.
 static hipError_t copy_sync(Context& c, void* dst, const void* src, size_t bytes) {
@@ -273,6 +279,8 @@ void svo_ctx_destroy(svo_ctx* ctx) {
   for (DevBuf* b : bufs) release(*b);
   if (c.stage) (void)hipHostFree(c.stage);
   c.stage = nullptr;
+  if (c.hread) (void)hipHostFree(c.hread);
+  c.hread = nullptr;
   for (int k = 0; k < 2; k++) {
     if (c.ingest[k]) (void)hipHostFree(c.ingest[k]);
     if (c.ingest_ev[k]) (void)hipEventDestroy(c.ingest_ev[k]);
@@ -545,7 +553,7 @@ int svo_build(svo_ctx* ctx, uint64_t gridsize, int generate_levels, uint64_t* n_
   if (e != hipSuccess) return fail(e, "svo_build");
   c.hazard = J.hazard;
   unsigned long long err = 0;
-  TRYC(copy_sync(c, &err, c.counters.as<unsigned long long>() + kCtrError, 8),
+  TRYC(read_err(c, &err),
        "svo_build");
   if (err) {
     g_err = "internal: " + std::to_string(err) +
@@ -812,7 +820,7 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
   TRYC(hipEventElapsedTime(&ms_v, c.ev1, c.ev2), "svo_batch_run");
   TRYC(hipEventElapsedTime(&ms_b, c.ev2, c.ev3), "svo_batch_run");
   unsigned long long err = 0;
-  TRYC(copy_sync(c, &err, c.counters.as<unsigned long long>() + kCtrError, 8),
+  TRYC(read_err(c, &err),
        "svo_batch_run");
   if (err) {
     g_err = "internal: " + std::to_string(err) + " voxelizer consistency failures";
@@ -844,7 +852,7 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
   return SVO_OK;
 }
 
-int svo_ctx_autotune(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, int best_out[3]) {
+int svo_ctx_autotune(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, int best_out[4]) {
   CHECK_CTX();
   Context& c = ctx->c;
   std::vector<svo_job_result> res(n_jobs);
@@ -867,12 +875,14 @@ int svo_ctx_autotune(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, int bes
   if (r) return r;
   // coordinate descent over the three knobs; every setting gives the same output
   static const int cand_cols[] = {64, 256, 1024, 4096};
+  static const int cand_direct[] = {0, 8, 16, 32};
   static const int cand_tile[] = {768, 1536, 2048, 4096};
   static const int cand_large[] = {1024, 2048, 4096};
-  int* knob[3] = {&c.tune.small_cols, &c.tune.tile_blocks, &c.tune.large_blocks};
-  const int* cands[3] = {cand_cols, cand_tile, cand_large};
-  const int ncand[3] = {4, 4, 3};
-  for (int k = 0; k < 3; k++) {
+  int* knob[4] = {&c.tune.small_cols, &c.tune.direct_cols, &c.tune.tile_blocks,
+                  &c.tune.large_blocks};
+  const int* cands[4] = {cand_cols, cand_direct, cand_tile, cand_large};
+  const int ncand[4] = {4, 4, 4, 3};
+  for (int k = 0; k < 4; k++) {
     const int keep = *knob[k];
     int best = keep;
     float best_ms = cur;
@@ -895,18 +905,33 @@ int svo_ctx_autotune(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, int bes
   }
   if (best_out) {
     best_out[0] = c.tune.small_cols;
-    best_out[1] = c.tune.tile_blocks;
-    best_out[2] = c.tune.large_blocks;
+    best_out[1] = c.tune.direct_cols;
+    best_out[2] = c.tune.tile_blocks;
+    best_out[3] = c.tune.large_blocks;
   }
   return SVO_OK;
 }
 
-int svo_ctx_get_tuning(svo_ctx* ctx, int out[3]) {
+int svo_ctx_get_tuning(svo_ctx* ctx, int out[4]) {
   CHECK_CTX();
   if (!out) return SVO_ERR_INVALID;
   out[0] = ctx->c.tune.small_cols;
-  out[1] = ctx->c.tune.tile_blocks;
-  out[2] = ctx->c.tune.large_blocks;
+  out[1] = ctx->c.tune.direct_cols;
+  out[2] = ctx->c.tune.tile_blocks;
+  out[3] = ctx->c.tune.large_blocks;
+  return SVO_OK;
+}
+
+int svo_ctx_set_knobs(svo_ctx* ctx, const int knobs[4]) {
+  CHECK_CTX();
+  if (!knobs || knobs[0] <= 0 || knobs[1] < 0 || knobs[2] <= 0 || knobs[3] <= 0) {
+    g_err = "invalid voxelizer knobs";
+    return SVO_ERR_INVALID;
+  }
+  ctx->c.tune.small_cols = knobs[0];
+  ctx->c.tune.direct_cols = knobs[1];
+  ctx->c.tune.tile_blocks = knobs[2];
+  ctx->c.tune.large_blocks = knobs[3];
   return SVO_OK;
 }
 

@@ -59,7 +59,8 @@ struct DevBuf {
 // Voxelizer tuning: small/large split threshold (columns per triangle) and block size.
 struct VoxTuning {
   int small_cols = 1024;    // columns above which a triangle goes to the large tier
-  int tile_blocks = 2048;   // persistent grid of the small tier
+  int direct_cols = 16;     // columns up to which a triangle's own thread walks them
+  int tile_blocks = 1024;   // persistent grid of the small tier
   int large_blocks = 2048;  // grid of the large tier (grid-stride)
 };
 
@@ -173,6 +174,7 @@ struct Context {
   DevBuf nodes;
   uint64_t n_nodes = 0, n_data = 0;
   int hazard = 0;
+  unsigned long long err_count = 0;  // the sticky failure counter, read by the build
   DevBuf lvl_code, lvl_size, lvl_mask, lvl_first, lvl_start, lvl_aux, lvl_leafp, lvl_pidx;
   DevBuf lvl_isize, lvl_cache, lvl_pstart;  // -levels: internal counts, averages, data starts
   DevBuf records2;                          // -levels: data records in post-order
@@ -187,6 +189,9 @@ struct Context {
   // pinned host staging for small uploads (bump allocated; wraps after a stream sync)
   void* stage = nullptr;
   size_t stage_cap = 0, stage_used = 0;
+  // pinned host landing area for device-to-host reads (counters, layouts)
+  void* hread = nullptr;
+  size_t hread_cap = 0;
 
   StageTimes times;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
@@ -212,6 +217,15 @@ void release(DevBuf& b);
 // Copy 'bytes' from host memory to device memory 'dst' asynchronously on c.stream
 // through the pinned staging buffer (the source may be reused at once).
 hipError_t upload_async(Context& c, void* dst, const void* src, size_t bytes);
+
+// Read device regions into host memory with one stream synchronization, through the
+// pinned landing area (a direct copy, not the staged path of pageable memory).
+struct ReadReq {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+hipError_t read_back(Context& c, const ReadReq* reqs, int n);
 
 // Stage drivers (return hipSuccess or an error; message in last error).
 // Partition job j's mesh into n_parts Morton-range partitions; lists appended to

@@ -49,6 +49,7 @@ struct VoxParams {
   float coord_limit;  // |vertex| beyond this walks full columns
   double slab_margin; // voxels added on each side of the exact plane slab
   int small_cols;
+  int direct_cols;     // triangles with at most this many columns: walked by their thread
   uint32_t job;        // tag of the unit's codes (bits kJobShift and up)
   int dense;           // 1: voxels in grid8; 0: in pool slots flagged by the touch pass
   int wide;            // dense payload unit: 4-byte cells holding ~(first triangle)
@@ -476,7 +477,10 @@ __device__ __forceinline__ uint32_t upper_index(const T* __restrict__ a, uint32_
 // items; then the threads share the items, one column each, so the work per lane is a few
 // voxel tests regardless of triangle size. Triangles with more than small_cols columns
 // (or non-slab triangles with more than small_cols * 8 tests) go to the column-parallel
-// tier. Units the pass does not apply to (skip mask) contribute no items.
+// tier. Triangles with at most direct_cols columns skip the item layout: their own
+// thread walks their columns from registers (the per-item lookup and constant loads
+// would cost more than the walk). Units the pass does not apply to (skip mask)
+// contribute no items.
 constexpr int kTile = 256;
 constexpr int kSkipDense = 1;   // units voxelized in a dense grid (touch pass)
 constexpr int kSkipBinary = 2;  // geometry-only units (payload winner pass)
@@ -510,6 +514,8 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
     const bool active = !skipped(P, skip);
     const uint32_t i = (tile - tile_off[u]) * kTile + tid;
     uint32_t items = 0;
+    bool direct = false;
+    Walk D;  // the walk of a triangle this thread handles alone
     if (active && i < P.n) {
       const uint32_t t = P.list ? P.list[i] : i;
       Setup S;
@@ -525,6 +531,9 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
           large_tri[idx] = t;
           large_cols[idx] = (uint32_t)ncols;
         }
+      } else if (ncols <= (uint64_t)P.direct_cols) {
+        direct = true;
+        D = K;
       } else {
         items = (uint32_t)ncols;
 #pragma unroll
@@ -539,6 +548,18 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
     if (tid == kTile - 1) soff[kTile] = total;
     __syncthreads();
     Act act = Act::make(A, P);
+    if (direct) {  // few columns: this thread walks them from registers
+      const uint32_t eu = (uint32_t)(D.i[I_G1U] - D.i[I_G0U] + 1);
+      const uint32_t nc = (uint32_t)D.columns();
+      for (uint32_t local = 0; local < nc; local++) {
+        const int a = D.i[I_G0U] + (int)(local % eu);
+        const int b = D.i[I_G0V] + (int)(local / eu);
+        if constexpr (Act::kTouch)
+          touch_column(D, P, a, b, act);
+        else
+          walk_column(D, P, a, b, act);
+      }
+    }
     for (uint32_t it = tid; it < total; it += kTile) {
       // owner: last j with soff[j] <= it (triangles without items have empty ranges)
       const uint32_t lo = upper_index<uint32_t>(soff, kTile, it);
@@ -983,10 +1004,32 @@ hipError_t upload_async(Context& c, void* dst, const void* src, size_t bytes) {
   return hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, c.stream);
 }
 
-static hipError_t read_counters(Context& c, ull* h) {
-  SVO_TRY(hipMemcpyAsync(h, c.counters.p, kNumCounters * sizeof(ull), hipMemcpyDeviceToHost,
-                         c.stream));
-  return hipStreamSynchronize(c.stream);
+hipError_t read_back(Context& c, const ReadReq* reqs, int n) {
+  size_t total = 0;
+  for (int k = 0; k < n; k++) total += (reqs[k].bytes + 255) & ~(size_t)255;
+  if (total > c.hread_cap) {
+    SVO_TRY(hipStreamSynchronize(c.stream));
+    if (c.hread) (void)hipHostFree(c.hread);
+    c.hread = nullptr;
+    c.hread_cap = 0;
+    const size_t cap = std::max<size_t>(total, 1u << 20);
+    SVO_TRY(hipHostMalloc(&c.hread, cap, hipHostMallocDefault));
+    c.hread_cap = cap;
+  }
+  size_t off = 0;
+  for (int k = 0; k < n; k++) {
+    if (reqs[k].bytes)
+      SVO_TRY(hipMemcpyAsync((char*)c.hread + off, reqs[k].src, reqs[k].bytes,
+                             hipMemcpyDeviceToHost, c.stream));
+    off += (reqs[k].bytes + 255) & ~(size_t)255;
+  }
+  SVO_TRY(hipStreamSynchronize(c.stream));
+  off = 0;
+  for (int k = 0; k < n; k++) {
+    memcpy(reqs[k].dst, (char*)c.hread + off, reqs[k].bytes);
+    off += (reqs[k].bytes + 255) & ~(size_t)255;
+  }
+  return hipSuccess;
 }
 
 static hipError_t ensure_scan_scratch(Context& c, uint64_t n, size_t elem) {
@@ -1048,8 +1091,8 @@ hipError_t partition_job(Context& c, uint32_t jid, float unitlength, uint64_t gr
       SVO_TRY((scan_exclusive<uint32_t>(ld, st, n, c.scan_scratch.as<uint32_t>(), c.stream)));
     }
     ull tot = 0;
-    SVO_TRY(hipMemcpyAsync(&tot, d_tot, 8, hipMemcpyDeviceToHost, c.stream));
-    SVO_TRY(hipStreamSynchronize(c.stream));
+    const ReadReq rq{&tot, d_tot, 8};
+    SVO_TRY(read_back(c, &rq, 1));
     J.part_off[j] = c.part_used;
     J.part_cnt[j] = tot;
     c.part_used += tot;
@@ -1086,6 +1129,7 @@ static void fill_params(const VoxUnit& h, const Context& c, VoxParams& P) {
   // below 2.1e-5 * G voxels. The margin is 0.5 + 4e-5 * G voxels, at least twice that.
   P.slab_margin = 0.5 + 4e-5 * (double)h.gridsize;
   P.small_cols = c.tune.small_cols;
+  P.direct_cols = c.tune.direct_cols;
   P.job = h.job;
   P.dense = 0;
   P.wide = 0;
@@ -1173,9 +1217,13 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   uint64_t* d_ubase = (uint64_t*)((char*)c.unitdev.p + szP);
   uint32_t* d_toff = (uint32_t*)((char*)c.unitdev.p + szP + szB);
   uint64_t* d_uvoff = (uint64_t*)((char*)c.unitdev.p + szP + szB + szT);
-  SVO_TRY(upload_async(c, d_units, P.data(), nu * sizeof(VoxParams)));
-  SVO_TRY(upload_async(c, d_ubase, ubase.data(), (nu + 1) * 8));
-  SVO_TRY(upload_async(c, d_toff, toff.data(), (nu + 1) * 4));
+  {  // one upload for the unit table (same layout as on the device)
+    std::vector<uint8_t> img(szP + szB + szT, 0);
+    memcpy(img.data(), P.data(), nu * sizeof(VoxParams));
+    memcpy(img.data() + szP, ubase.data(), (nu + 1) * 8);
+    memcpy(img.data() + szP + szB, toff.data(), (nu + 1) * 4);
+    SVO_TRY(upload_async(c, d_units, img.data(), img.size()));
+  }
 
   SVO_TRY(ensure(c.large_unit, ntri * 4));
   SVO_TRY(ensure(c.large_tri, ntri * 4));
@@ -1269,9 +1317,11 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
                        c.occ_list.as<uint32_t>(), ctr, d_ubase, nu, c.bcount.as<uint64_t>(),
                        d_uvoff);
     SVO_TRY(hipGetLastError());
-    SVO_TRY(hipMemcpyAsync(uvoff.data(), d_uvoff, (nu + 1) * 8, hipMemcpyDeviceToHost,
-                           c.stream));
-    SVO_TRY(read_counters(c, hc));
+    {
+      const ReadReq rq[2] = {{uvoff.data(), d_uvoff, (nu + 1) * 8},
+                             {hc, c.counters.p, kNumCounters * sizeof(ull)}};
+      SVO_TRY(read_back(c, rq, 2));
+    }
     ktimer_flush(c);
     if (dense || (!hc[kCtrOverflow] && hc[kCtrNumOcc] <= c.pool_cap)) break;
     // pool too small for the flagged bricks: clear everything, grow, redo

@@ -91,8 +91,9 @@ SIGNATURES = {
                        C.POINTER(RunInfo)]),
     "svo_batch_run": (_i32, [_vp, C.POINTER(Job), C.c_uint32, C.POINTER(JobResult),
                              C.POINTER(RunInfo)]),
-    "svo_ctx_autotune": (_i32, [_vp, C.POINTER(Job), C.c_uint32, C.POINTER(C.c_int * 3)]),
-    "svo_ctx_get_tuning": (_i32, [_vp, C.POINTER(C.c_int * 3)]),
+    "svo_ctx_autotune": (_i32, [_vp, C.POINTER(Job), C.c_uint32, C.POINTER(C.c_int * 4)]),
+    "svo_ctx_get_tuning": (_i32, [_vp, C.POINTER(C.c_int * 4)]),
+    "svo_ctx_set_knobs": (_i32, [_vp, C.POINTER(C.c_int * 4)]),
     "svo_server_create": (_i32, [_i32, _u64, C.c_uint32, C.c_uint32, C.POINTER(_vp)]),
     "svo_server_destroy": (None, [_vp]),
     "svo_server_submit": (_i32, [_vp, C.POINTER(Job), C.POINTER(_u64)]),
@@ -376,14 +377,19 @@ class SvoContext:
         n = len(jobs)
         arr = (Job * n)()
         keep = [fill_job(arr[i], j) for i, j in enumerate(jobs)]  # noqa: F841
-        out = (C.c_int * 3)()
+        out = (C.c_int * 4)()
         _check(load_library().svo_ctx_autotune(self._h, arr, n, C.byref(out)))
         return tuple(out)
 
     def get_tuning(self) -> tuple:
-        out = (C.c_int * 3)()
+        """(small_cols, direct_cols, tile_blocks, large_blocks)"""
+        out = (C.c_int * 4)()
         _check(load_library().svo_ctx_get_tuning(self._h, C.byref(out)))
         return tuple(out)
+
+    def set_knobs(self, small_cols: int, direct_cols: int, tile_blocks: int, large_blocks: int):
+        k = (C.c_int * 4)(small_cols, direct_cols, tile_blocks, large_blocks)
+        _check(load_library().svo_ctx_set_knobs(self._h, C.byref(k)))
 
     def nodes_range(self, first: int, count: int) -> np.ndarray:
         out = np.empty(count, NODE_DTYPE)

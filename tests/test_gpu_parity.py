@@ -277,16 +277,21 @@ def test_cli_files_identical(tmp_path):
         assert files_identical(str(tmp_path / "gpu" / base), str(tmp_path / "ref" / base)), name
 
 
-@pytest.mark.parametrize("small_cols,dense_mb", [(1024, None), (4, None), (1024, 0), (16, 0)])
-def test_voxelizer_paths_identical(small_cols, dense_mb):
-    """Dense byte grid and touched-brick pool, small and large tier thresholds: the same
-    bytes as the oracle (these knobs change the work split, never the output)."""
-    _say("voxelizer paths small_cols=%d dense_mb=%s" % (small_cols, dense_mb))
+@pytest.mark.parametrize("small_cols,dense_mb,direct", [(1024, None, 16), (4, None, 0),
+                                                      (1024, 0, 0), (16, 0, 8),
+                                                      (1024, None, 1 << 20)])
+def test_voxelizer_paths_identical(small_cols, dense_mb, direct):
+    """Dense byte grid and touched-brick pool, small and large tier thresholds, walks by
+    the triangle's own thread or shared by the block: the same bytes as the oracle (these
+    knobs change the work split, never the output)."""
+    _say("voxelizer paths small_cols=%d dense_mb=%s direct=%d" % (small_cols, dense_mb, direct))
     rng = np.random.default_rng(123)
     tris = np.concatenate([generators.icosphere(subdivisions=4),
                            adversarial_soup(rng, 500, 128)])
     with SvoContext(0) as ctx:
         ctx.set_tuning(small_cols=small_cols, dense_limit_mb=dense_mb)
+        k = ctx.get_tuning()
+        ctx.set_knobs(k[0], direct, k[2], k[3])
         for G, L in ((128, 2048), (256, 2)):
             _run_both(ctx, tris, False, G, L)
         _run_both(ctx, generators.terrain(n=49, seed=8, payload=True), True, 128, 2048)
