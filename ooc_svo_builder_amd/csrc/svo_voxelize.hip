@@ -48,6 +48,7 @@ struct VoxParams {
   float unit, unit_div;
   float coord_limit;  // |vertex| beyond this walks full columns
   double slab_margin; // voxels added on each side of the exact plane slab
+  double inv_unit;    // 1 / unit in double
   int small_cols;
   int direct_cols;     // triangles with at most this many columns: walked by their thread
   uint32_t job;        // tag of the unit's codes (bits kJobShift and up)
@@ -235,7 +236,7 @@ __device__ __forceinline__ void make_walk(const Setup& S, const VoxParams& P, ui
   K.f[F_NVA] = nva;
   // plane slab in index space along w: c in [ka*a + kb*b + alpha_lo, ... + alpha_hi]
   if (S.slab) {
-    const double inw = 1.0 / (double)nw, iu = 1.0 / (double)P.unit;
+    const double inw = 1.0 / (double)nw, iu = P.inv_unit;
     const double a1 = -(S.nc - S.nv0) * inw * iu, a2 = -(S.nm - S.nv0) * inw * iu;
     K.f[F_KA] = (float)(-(double)nua * inw);
     K.f[F_KB] = (float)(-(double)nva * inw);
@@ -296,10 +297,19 @@ __device__ __forceinline__ void touch_column(const Walk& K, const VoxParams& P, 
   for (int cb = lo >> 4; cb <= (hi >> 4); cb++) act.touch(bbase | (spread3((uint32_t)cb) << w));
 }
 
-// Test the voxels of column (a, b) and hand every accepted one to act.add(code, tri).
+// Every bit of Morton axis k: a coordinate spread onto these bits advances by one with a
+// masked add, (m | ~mask) + (1 << k), then & mask.
+constexpr uint64_t kAxis0Bits = 0x1249249249249249ull;
+__device__ __forceinline__ uint64_t morton_step(uint64_t m, int k) {
+  const uint64_t mask = kAxis0Bits << k;
+  return ((m | ~mask) + (1ull << k)) & mask;
+}
+
+// Test the voxels of column (a, b) and hand every accepted one to act.add(code, tri);
+// cbase holds a and b spread onto their Morton axes.
 template <typename Act>
 __device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, int a, int b,
-                                            Act& act) {
+                                            uint64_t cbase, Act& act) {
   const float u = P.unit;
   const float pa = (float)a * u;
   const float pb = (float)b * u;
@@ -317,11 +327,9 @@ __device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, i
     KA[k] = K.f[F_AC + k] * cA;
     KB[k] = K.f[F_BC + k] * cB;
   }
-  const int ua = (w == 0) ? 1 : 0;
-  const int va = (w == 2) ? 1 : 2;
-  const uint64_t cbase = (spread3((uint32_t)a) << ua) | (spread3((uint32_t)b) << va);
   const float nw = K.f[F_NW], d1 = K.f[F_D1], d2 = K.f[F_D2];
-  for (int c = lo; c <= hi; c++) {
+  uint64_t mc = spread3((uint32_t)lo) << w;
+  for (int c = lo; c <= hi; c++, mc = morton_step(mc, w)) {
     const float pc = (float)c * u;
     const float t = (nw * pc + Ka) + Kb;
     if ((t + d1) * (t + d2) > 0.0f) continue;
@@ -332,7 +340,35 @@ __device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, i
       if ((K.f[F_BV + k] * pc + KB[k]) + K.f[F_BD + k] < 0.0f) ok = false;
     }
     if (!ok) continue;
-    act.add(cbase | (spread3((uint32_t)c) << w), (uint32_t)K.i[I_TRI]);
+    act.add(cbase | mc, (uint32_t)K.i[I_TRI]);
+  }
+}
+
+// Column (a, b) of a walk, for callers that do not step the Morton parts themselves.
+template <typename Act>
+__device__ __forceinline__ void walk_column_ab(const Walk& K, const VoxParams& P, int a, int b,
+                                               Act& act) {
+  const int w = K.w();
+  const int ua = (w == 0) ? 1 : 0;
+  const int va = (w == 2) ? 1 : 2;
+  walk_column(K, P, a, b, (spread3((uint32_t)a) << ua) | (spread3((uint32_t)b) << va), act);
+}
+
+// All columns of a walk, row by row, the Morton parts of a and b stepped incrementally.
+template <typename Act>
+__device__ __forceinline__ void walk_all_columns(const Walk& K, const VoxParams& P, Act& act) {
+  const int w = K.w();
+  const int ua = (w == 0) ? 1 : 0;
+  const int va = (w == 2) ? 1 : 2;
+  uint64_t mb = spread3((uint32_t)K.i[I_G0V]) << va;
+  for (int b = K.i[I_G0V]; b <= K.i[I_G1V]; b++, mb = morton_step(mb, va)) {
+    uint64_t ma = spread3((uint32_t)K.i[I_G0U]) << ua;
+    for (int a = K.i[I_G0U]; a <= K.i[I_G1U]; a++, ma = morton_step(ma, ua)) {
+      if constexpr (Act::kTouch)
+        touch_column(K, P, a, b, act);
+      else
+        walk_column(K, P, a, b, ma | mb, act);
+    }
   }
 }
 
@@ -471,7 +507,7 @@ __device__ __forceinline__ uint32_t upper_index(const T* __restrict__ a, uint32_
   return lo;
 }
 
-// Small tier, persistent: blocks take tiles of 256 consecutive triangles of one unit from
+// Small tier, persistent: blocks take tiles of kTileTris consecutive triangles of one unit from
 // a device work queue (tile_off[u] = first tile of unit u). For a tile, each thread builds
 // its triangle's walk constants into LDS; a block scan lays out the (triangle, column)
 // items; then the threads share the items, one column each, so the work per lane is a few
@@ -481,7 +517,9 @@ __device__ __forceinline__ uint32_t upper_index(const T* __restrict__ a, uint32_
 // thread walks their columns from registers (the per-item lookup and constant loads
 // would cost more than the walk). Units the pass does not apply to (skip mask)
 // contribute no items.
-constexpr int kTile = 256;
+constexpr int kTile = 256;      // threads per block
+constexpr int kTileTris = 192;  // triangles per tile: 35 KB of walk constants in LDS, so
+                                // that four blocks fit a CU
 constexpr int kSkipDense = 1;   // units voxelized in a dense grid (touch pass)
 constexpr int kSkipBinary = 2;  // geometry-only units (payload winner pass)
 
@@ -497,8 +535,8 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
                                                     uint32_t* large_unit, uint32_t* large_tri,
                                                     uint32_t* large_cols, int collect,
                                                     int skip) {
-  __shared__ uint32_t sk[kWalkWords][kTile];
-  __shared__ uint32_t soff[kTile + 1];
+  __shared__ uint32_t sk[kWalkWords][kTileTris];
+  __shared__ uint32_t soff[kTileTris + 1];
   __shared__ uint32_t wsum[kTile / kWave];
   __shared__ uint32_t s_tile;
   ull* ctr = A.ctr;
@@ -512,11 +550,9 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
     const uint32_t u = upper_index<uint32_t>(tile_off, n_units + 1, tile);
     const VoxParams P = units[u];
     const bool active = !skipped(P, skip);
-    const uint32_t i = (tile - tile_off[u]) * kTile + tid;
+    const uint32_t i = (tile - tile_off[u]) * kTileTris + tid;
     uint32_t items = 0;
-    bool direct = false;
-    Walk D;  // the walk of a triangle this thread handles alone
-    if (active && i < P.n) {
+    if (active && tid < (uint32_t)kTileTris && i < P.n) {
       const uint32_t t = P.list ? P.list[i] : i;
       Setup S;
       setup_tri(P.tris + (uint64_t)t * P.stride, P, S);
@@ -532,8 +568,10 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
           large_cols[idx] = (uint32_t)ncols;
         }
       } else if (ncols <= (uint64_t)P.direct_cols) {
-        direct = true;
-        D = K;
+        // few columns: this thread walks them now, from registers
+        Act act = Act::make(A, P);
+        walk_all_columns(K, P, act);
+        act.flush();
       } else {
         items = (uint32_t)ncols;
 #pragma unroll
@@ -544,25 +582,13 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
     }
     uint32_t total;
     const uint32_t inc = block_incl_scan<uint32_t, kTile>(items, wsum, total);
-    soff[tid] = inc - items;
-    if (tid == kTile - 1) soff[kTile] = total;
+    if (tid < (uint32_t)kTileTris) soff[tid] = inc - items;
+    if (tid == kTileTris - 1) soff[kTileTris] = total;
     __syncthreads();
     Act act = Act::make(A, P);
-    if (direct) {  // few columns: this thread walks them from registers
-      const uint32_t eu = (uint32_t)(D.i[I_G1U] - D.i[I_G0U] + 1);
-      const uint32_t nc = (uint32_t)D.columns();
-      for (uint32_t local = 0; local < nc; local++) {
-        const int a = D.i[I_G0U] + (int)(local % eu);
-        const int b = D.i[I_G0V] + (int)(local / eu);
-        if constexpr (Act::kTouch)
-          touch_column(D, P, a, b, act);
-        else
-          walk_column(D, P, a, b, act);
-      }
-    }
     for (uint32_t it = tid; it < total; it += kTile) {
       // owner: last j with soff[j] <= it (triangles without items have empty ranges)
-      const uint32_t lo = upper_index<uint32_t>(soff, kTile, it);
+      const uint32_t lo = upper_index<uint32_t>(soff, kTileTris, it);
       Walk K;
 #pragma unroll
       for (int f = 0; f < kWalkF; f++) K.f[f] = __uint_as_float(sk[f][lo]);
@@ -575,7 +601,7 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
       if constexpr (Act::kTouch)
         touch_column(K, P, a, b, act);
       else
-        walk_column(K, P, a, b, act);
+        walk_column_ab(K, P, a, b, act);
     }
     act.flush();
     __syncthreads();  // LDS (setups, offsets, s_tile) is reused by the next tile
@@ -611,7 +637,7 @@ __global__ __launch_bounds__(256) void k_vox_large(const VoxParams* __restrict__
     if constexpr (Act::kTouch)
       touch_column(K, P, a, b, act);
     else
-      walk_column(K, P, a, b, act);
+      walk_column_ab(K, P, a, b, act);
     act.flush();
   }
 }
@@ -1128,6 +1154,7 @@ static void fill_params(const VoxUnit& h, const Context& c, VoxParams& P) {
   // less than ~3e-6 * coord_limit; along the column axis (|n_w| >= 1/2 here) that is
   // below 2.1e-5 * G voxels. The margin is 0.5 + 4e-5 * G voxels, at least twice that.
   P.slab_margin = 0.5 + 4e-5 * (double)h.gridsize;
+  P.inv_unit = 1.0 / (double)h.unit;
   P.small_cols = c.tune.small_cols;
   P.direct_cols = c.tune.direct_cols;
   P.job = h.job;
@@ -1165,7 +1192,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     ubase[u] = ntab;
     toff[u] = (uint32_t)ntiles;
     ntab += unit_bricks(units[u]);
-    ntiles += (units[u].n + kTile - 1) / kTile;
+    ntiles += (units[u].n + kTileTris - 1) / kTileTris;
     ntri += units[u].n;
   }
   ubase[nu] = ntab;
