@@ -232,6 +232,7 @@ struct ParentStore {
 __global__ __launch_bounds__(256) void k_job_bases(JobDev* jobs, uint32_t n_jobs,
                                                    const unsigned long long* root_size,
                                                    const unsigned long long* root_isize,
+                                                   const unsigned long long* brick_hazard,
                                                    const uint64_t* __restrict__ codes,
                                                    Tags T, LevelMeta M, uint64_t node_cap,
                                                    uint64_t* __restrict__ nodes,
@@ -261,6 +262,8 @@ __global__ __launch_bounds__(256) void k_job_bases(JobDev* jobs, uint32_t n_jobs
       uint64_t hz = 0;
       if (J.nvox == 0) {
         hz = (J.gridsize == 1) ? 1 : 0;
+      } else if (brick_hazard) {  // found in the brick stream (codes not written)
+        hz = brick_hazard[j];
       } else {
         const uint64_t last = codes[J.voff + J.nvox - 1] & ~T.tag_mask;
         // OctreeBuilder.cpp:36-42 leaves the root undefined when the last voxel is the
@@ -477,10 +480,19 @@ __global__ __launch_bounds__(256) void k_brick_level(Arena A, LevelMeta M, Tags 
                                                      const uint64_t* __restrict__ bwords,
                                                      const uint64_t* __restrict__ bcode,
                                                      uint64_t nb,
-                                                     unsigned long long* root_size) {
+                                                     unsigned long long* root_size,
+                                                     unsigned long long* hazard) {
   const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   if (w >= nb) return;  // uniform per wave
   const BrickShape B = brick_shape(bwords, w);
+  if (lane_id() == kWave - 1) {
+    // the job's last voxel is code G^3 - 2 (the case k_job_bases flags) when that voxel
+    // is set and G^3 - 1 is not: bits 62 and 63 of the last word of the grid's last brick
+    const uint64_t c0 = bcode[w];
+    const uint32_t j = T.job(c0);
+    const uint64_t G = jobs[j].gridsize;
+    if ((c0 & ~T.tag_mask) == G * G * G - kBrickVox && (B.word >> 62) == 1ull) hazard[j] = 1;
+  }
   if (lane_id() == 0) {
     const uint64_t c0 = bcode[w];
     const uint64_t code = (c0 & T.tag_mask) | ((c0 & ~T.tag_mask) >> 12);
@@ -692,6 +704,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
   bool fast = c.bricks_ok && !levels && c.n_bricks > 0;
   for (uint32_t j = 0; j < n_jobs; j++)
     if (jd[j].nvox && jd[j].depth < 4) fast = false;
+  if (!fast) SVO_TRY(ensure_codes(c));  // the level scans start from the codes
   const uint64_t NB = fast ? c.n_bricks : 0;
   const int h0 = fast ? 4 : 0;  // lowest height the level scans start from
   // per-height upper bounds (host) size the launch grids
@@ -726,7 +739,11 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     SVO_TRY(ensure(c.lvl_first, cap * 4));
     SVO_TRY(ensure(c.lvl_start, cap * 8));
     SVO_TRY(ensure(c.lvl_pidx, cap * 4));
-    SVO_TRY(ensure(c.lvl_aux, (2 * 64 + 8) * 8 + (uint64_t)n_jobs * 16));
+    // meta | per job: root size, root internal size, hazard | single-pass scan status
+    const uint64_t meta_words = 2 * 64 + 8 + 3 * (uint64_t)n_jobs;
+    uint64_t st_words = 0;
+    for (int h = h0; h < H; h++) st_words += scan_single_words(bound[h]);
+    SVO_TRY(ensure(c.lvl_aux, (meta_words + st_words) * 8));
     if (levels) {
       SVO_TRY(ensure(c.lvl_isize, cap * 8));
       SVO_TRY(ensure(c.lvl_cache, cap * 24));
@@ -738,7 +755,6 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     if (!fast) SVO_TRY(ensure(c.lvl_leafp, std::max<uint64_t>(V, 1) * 4));
     const uint64_t node_cap = cap + V + in_brick + n_jobs;
     SVO_TRY(ensure(c.nodes, node_cap * 24));
-    SVO_TRY(ensure(c.scan_scratch, scan_scratch_elems(std::max<uint64_t>(V, 1)) * 4 + 256));
     Arena A{c.lvl_code.as<uint64_t>(), c.lvl_size.as<uint64_t>(), c.lvl_mask.as<uint8_t>(),
             c.lvl_first.as<uint32_t>(), c.lvl_start.as<uint64_t>(), c.lvl_pidx.as<uint32_t>(),
             cap, levels ? c.lvl_isize.as<uint64_t>() : nullptr,
@@ -748,8 +764,9 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     LevelMeta M{meta, meta + 64, meta + 128};
     unsigned long long* root_size = meta + 136;
     unsigned long long* root_isize = root_size + n_jobs;
-    // n[0] = V, off[1] = 0, flags = 0, root sizes = 0
-    SVO_TRY(fill_u64((uint64_t*)meta, 0, 2 * 64 + 8 + 2 * (uint64_t)n_jobs, c.stream));
+    unsigned long long* brick_hazard = root_isize + n_jobs;
+    // n[0] = V, off[1] = 0, flags = 0, root sizes = 0, hazard flags = 0, scan status = 0
+    SVO_TRY(fill_u64((uint64_t*)meta, 0, meta_words + st_words, c.stream));
     SVO_TRY(fill_u64((uint64_t*)meta, V, 1, c.stream));
     if (fast) {  // n[4] = NB, off[4] = 0, off[5] = NB
       SVO_TRY(fill_u64((uint64_t*)meta + 4, NB, 1, c.stream));
@@ -757,21 +774,25 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
       ktimer_begin(c, "brick_level");
       hipLaunchKernelGGL(k_brick_level, dim3((unsigned)((NB * kWave + 255) / 256)), dim3(256), 0,
                          c.stream, A, M, T, (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),
-                         c.bcode.as<uint64_t>(), NB, root_size);
+                         c.bcode.as<uint64_t>(), NB, root_size, brick_hazard);
       ktimer_end(c);
       SVO_TRY(hipGetLastError());
     }
 
     // ---- bottom-up: heights 1 .. H
+    unsigned long long* status = meta + meta_words;
     for (int h = h0; h < H; h++) {
       HeadLoad ld{A, M, T, d_jobs, c.codes.as<uint64_t>(), h};
       ParentStore st{A, M, T, d_jobs, c.codes.as<uint64_t>(), c.lvl_leafp.as<uint32_t>(),
                      root_size, root_isize, c.records.as<float>() + 8, h};
-      SVO_TRY((scan_exclusive<uint32_t>(ld, st, bound[h], c.scan_scratch.as<uint32_t>(),
-                                        c.stream, M.n + h)));
+      SVO_TRY((scan_single<uint32_t>(ld, st, bound[h], status,
+                                     c.counters.as<unsigned long long>() + kCtrError, c.stream,
+                                     M.n + h)));
+      status += scan_single_words(bound[h]);
     }
     hipLaunchKernelGGL(k_job_bases, dim3(1), dim3(256), 0, c.stream, d_jobs, n_jobs, root_size,
-                       root_isize, c.codes.as<uint64_t>(), T, M, node_cap,
+                       root_isize, fast ? brick_hazard : nullptr, c.codes.as<uint64_t>(), T, M,
+                       node_cap,
                        c.nodes.as<uint64_t>(), c.records.as<float>(), out_rec);
     SVO_TRY(hipGetLastError());
     // ---- top-down: heights H .. 1, then the leaves

@@ -196,6 +196,7 @@ static void reset_plain(Context& c) {
   c.data_in_records2 = false;
   c.n_bricks = 0;
   c.bricks_ok = true;
+  c.codes_lazy = false;
 }
 
 // The 6-significant-digit text round trip a bbox value takes through the .trip header
@@ -509,6 +510,7 @@ int svo_add_voxels(svo_ctx* ctx, const uint64_t* codes, const void* records32, u
     return SVO_ERR_INVALID;
   }
   uint64_t last = 0;
+  TRYC(ensure_codes(c), "svo_add_voxels");
   if (c.n_vox)
     TRYC(copy_sync(c, &last, c.codes.as<uint64_t>() + c.n_vox - 1, 8),
          "svo_add_voxels");
@@ -604,6 +606,7 @@ int svo_copy_codes(svo_ctx* ctx, uint64_t* host_dst, uint64_t first, uint64_t co
     g_err = "code range out of bounds";
     return SVO_ERR_INVALID;
   }
+  TRYC(ensure_codes(c), "svo_copy_codes");
   if (count)
     TRYC(copy_sync(c, host_dst, c.codes.as<uint64_t>() + first, count * 8),
          "svo_copy_codes");

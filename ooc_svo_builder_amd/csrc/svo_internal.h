@@ -156,6 +156,7 @@ struct Context {
   DevBuf large_unit, large_tri, large_cols, large_off;
   DevBuf occ_list, bcount, wprefix, winner;
   DevBuf scan_scratch;
+  DevBuf scan_status;  // status words of the single-pass scans of a voxelize pass
 
   // brick stream: every occupied 16^3 brick of the voxel stream, in stream order: its 64
   // occupancy words, its first voxel's code (tagged) and its first voxel's stream index.
@@ -163,6 +164,9 @@ struct Context {
   DevBuf bwords, bcode, bvoff;
   uint64_t n_bricks = 0;
   bool bricks_ok = true;  // every voxel of the stream came with its brick, bricks unique
+  // Geometry-only passes leave the codes of their voxels unwritten (the build reads the
+  // brick stream); ensure_codes expands the brick stream into them when a reader needs them.
+  bool codes_lazy = false;
 
   // voxel stream across partitions and jobs (sorted by construction)
   DevBuf codes;    // u64 Morton codes (job tag in the high bits for batches)
@@ -237,6 +241,8 @@ hipError_t partition_job(Context& c, uint32_t j, float unitlength, uint64_t grid
 // batch = true: the units hold the whole of their jobs, whose ranges are laid out here.
 hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& units, bool batch,
                           uint64_t* nfilled);
+// Write the codes of every voxel that only the brick stream holds (no-op when none).
+hipError_t ensure_codes(Context& c);
 // Build the octrees of jobs [0, n_jobs) from the voxel stream; 'tagged' = the stream
 // carries job tags (a batch). levels (-levels, OctreeBuilder.cpp:82-99) is supported for
 // a single job.

@@ -125,4 +125,126 @@ hipError_t scan_exclusive_items(Load ld, Store st, uint64_t n, T* scratch, hipSt
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- single-pass scan
+// One launch (decoupled look-back, Merrill & Garland 2016): each block takes the next tile
+// ticket, publishes its tile's sum, then its wave 0 walks back over the predecessors'
+// status words until it meets an inclusive prefix, publishes its own inclusive prefix and
+// the block applies the stores. A status word is the flag (bits 63..62: 0 = not yet
+// published, 1 = the tile's own sum, 2 = inclusive prefix through the tile) and the value
+// (bits 61..0; every prefix must stay below 2^62) in one word, written and read with
+// device-scope atomics: those bypass the per-XCD L2 caches, so the word is coherent
+// across the chip without a fence. Tiles are taken in ticket order, so every tile a block
+// waits on belongs to a block that is already running. The wait is bounded: after
+// kScanSpinLimit polls the block counts an error in *err and goes on with a zero prefix.
+//
+// Blocks are 1024 threads (tiles of 8192 elements for large inputs), so that the walk
+// back is short; tiles at or beyond the actual element count exit at once (no later tile
+// waits on them).
+//
+// status: 1 + tiles words, zero before the launch ([0] is the ticket counter).
+constexpr int kSingleBlock = 1024;
+constexpr uint64_t kSingleSmall = 1ull << 16;  // below: one item per thread
+constexpr unsigned long long kStAgg = 1ull << 62;
+constexpr unsigned long long kStIncl = 2ull << 62;
+constexpr unsigned long long kStVal = (1ull << 62) - 1;
+constexpr uint32_t kScanSpinLimit = 1u << 22;
+
+__device__ __forceinline__ void status_put(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long status_get(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T, typename Load, typename Store, int ITEMS>
+__global__ __launch_bounds__(kSingleBlock) void k_scan_single(Load ld, Store st, uint64_t n,
+                                                              const unsigned long long* n_dev,
+                                                              unsigned long long* status,
+                                                              unsigned long long* err) {
+  __shared__ T sm[kSingleBlock / kWave];
+  __shared__ uint32_t s_tile;
+  __shared__ T s_prefix;
+  if (threadIdx.x == 0) s_tile = atomicAdd((unsigned int*)status, 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  unsigned long long* tiles = status + 1;
+  if (n_dev) n = min(n, (uint64_t)*n_dev);
+  if ((uint64_t)tile * kSingleBlock * ITEMS >= n) return;  // uniform per block
+  const uint64_t base = ((uint64_t)tile * kSingleBlock + threadIdx.x) * ITEMS;
+  T v[ITEMS];
+#pragma unroll
+  for (int k = 0; k < ITEMS; k++) v[k] = (base + k < n) ? ld(base + k) : (T)0;
+  T acc = 0;
+#pragma unroll
+  for (int k = 0; k < ITEMS; k++) acc += v[k];
+  T tot;
+  const T inc = block_incl_scan<T, kSingleBlock>(acc, sm, tot);
+  if (threadIdx.x < kWave) {  // wave 0: publish, look back
+    T prefix = 0;
+    if (tile == 0) {
+      if (threadIdx.x == 0) status_put(&tiles[0], kStIncl | (unsigned long long)tot);
+    } else {
+      if (threadIdx.x == 0) status_put(&tiles[tile], kStAgg | (unsigned long long)tot);
+      int64_t j = (int64_t)tile - 1;  // lane l looks at tile j - l
+      uint32_t polls = 0;
+      for (;;) {
+        const int64_t idx = j - (int64_t)lane_id();
+        unsigned long long s = (idx >= 0) ? status_get(&tiles[idx]) : kStIncl;
+        for (;;) {
+          const bool pend = (s >> 62) == 0ull;
+          if (!__ballot(pend)) break;
+          if (++polls > kScanSpinLimit) {
+            if (lane_id() == 0) atomicAdd(err, 1ull);
+            if (pend) s = kStIncl;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          if (pend) s = status_get(&tiles[idx]);
+        }
+        const unsigned long long incl = __ballot((s >> 62) == 2ull);
+        const int first = incl ? __ffsll((long long)incl) - 1 : kWave;
+        const T val = (lane_id() <= first) ? (T)(s & kStVal) : (T)0;
+        prefix += shfl_t<T>(wave_incl_scan<T>(val), kWave - 1);
+        if (incl) break;
+        j -= kWave;
+      }
+      if (threadIdx.x == 0) status_put(&tiles[tile], kStIncl | (unsigned long long)(prefix + tot));
+    }
+    if (threadIdx.x == 0) s_prefix = prefix;
+  }
+  __syncthreads();
+  T run = s_prefix + inc - acc;
+#pragma unroll
+  for (int k = 0; k < ITEMS; k++) {
+    if (base + k < n) st(base + k, run, v[k]);
+    run += v[k];
+  }
+}
+
+// Items per thread of the single-pass scan for n elements, and its status words.
+inline int scan_single_items(uint64_t n) { return n >= kSingleSmall ? kScanItems : 1; }
+inline uint64_t scan_single_words(uint64_t n) {
+  const uint64_t tile = (uint64_t)kSingleBlock * scan_single_items(n);
+  return 1 + (n + tile - 1) / tile;
+}
+
+// Exclusive scan in one launch; 'status' holds scan_single_words(n) zero words. n is an
+// upper bound when n_dev is given (as for scan_exclusive).
+template <typename T, typename Load, typename Store>
+hipError_t scan_single(Load ld, Store st, uint64_t n, unsigned long long* status,
+                       unsigned long long* err, hipStream_t s,
+                       const unsigned long long* n_dev = nullptr) {
+  if (n == 0) return hipSuccess;
+  const int items = scan_single_items(n);
+  const uint64_t tile = (uint64_t)kSingleBlock * items;
+  const unsigned nb = (unsigned)((n + tile - 1) / tile);
+  if (items == kScanItems)
+    hipLaunchKernelGGL((k_scan_single<T, Load, Store, kScanItems>), dim3(nb), dim3(kSingleBlock),
+                       0, s, ld, st, n, n_dev, status, err);
+  else
+    hipLaunchKernelGGL((k_scan_single<T, Load, Store, 1>), dim3(nb), dim3(kSingleBlock), 0, s,
+                       ld, st, n, n_dev, status, err);
+  return hipGetLastError();
+}
+
 }  // namespace svo

@@ -699,12 +699,15 @@ __global__ __launch_bounds__(256) void k_brick_count(const uint32_t* __restrict_
         m |= (ull)nib << (4 * q);
       }
     } else {
+      // bytes: load q of lane l is the 16 bytes at q * 1024 + 16 l (each load of the wave
+      // one contiguous KiB), i.e. bits 16 (l % 4) .. + 15 of word 16 q + l / 4
       uint8_t* base = units[u].dense ? units[u].grid8 + (ti - ubase[u]) * kBrickVox
                                      : pool8 + wv * kBrickVox;
-      uint4* src = (uint4*)(base + lane_id() * 64);
+      uint4* src = (uint4*)base + lane_id();
       uint4 v[4];
 #pragma unroll
-      for (int q = 0; q < 4; q++) v[q] = src[q];
+      for (int q = 0; q < 4; q++) v[q] = src[q * kWave];
+      ull parts = 0;  // 16 bits per load
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const uint32_t w4[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
@@ -712,9 +715,18 @@ __global__ __launch_bounds__(256) void k_brick_count(const uint32_t* __restrict_
         for (int d = 0; d < 4; d++) {
           const uint32_t x = w4[d];  // four bytes, each 0 or 1
           const uint32_t nib = (x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u);
-          m |= (ull)nib << (16 * q + 4 * d);
+          parts |= (ull)nib << (16 * q + 4 * d);
         }
-        if (v[q].x | v[q].y | v[q].z | v[q].w) src[q] = make_uint4(0u, 0u, 0u, 0u);
+        if (v[q].x | v[q].y | v[q].z | v[q].w) src[q * kWave] = make_uint4(0u, 0u, 0u, 0u);
+      }
+      // word L = 16 q + r gathers the parts of load q from lanes 4 r .. 4 r + 3
+      const int L = lane_id(), q = L >> 4, r = L & 15;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)parts, 4 * r + k, kWave);
+        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(parts >> 32), 4 * r + k, kWave);
+        const ull p = ((ull)hi << 32) | lo;
+        m |= ((p >> (16 * q)) & 0xFFFFull) << (16 * k);
       }
     }
     pool[wv * kBrickWords + lane_id()] = m;
@@ -741,7 +753,10 @@ struct ColLoad {
   __device__ uint64_t operator()(uint64_t i) const { return a[i]; }
 };
 
-// Sorted codes of every flagged brick, tagged with the unit's job in bits kJobShift+.
+// The brick stream entry of every non-empty flagged brick and, with kCodes, the sorted
+// codes of its voxels, tagged with the unit's job in bits kJobShift+ (and the payload
+// winners of dense payload units).
+template <bool kCodes>
 __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__ occ,
                                                     const ull* __restrict__ ctr,
                                                     const VoxParams* __restrict__ units,
@@ -761,11 +776,7 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
     const uint32_t u = upper_index<uint64_t>(ubase, n_units + 1, ti);
     const uint64_t slot = wv * kBrickWords + lane_id();
     ull word = pool[slot];
-    const uint32_t c = (uint32_t)__popcll(word);
-    const uint32_t inc = wave_incl_scan<uint32_t>(c);
     const uint64_t bo = boff[wv];
-    uint64_t pos = (bo & kVoxMask) + (inc - c);
-    if (wprefix) wprefix[slot] = (uint32_t)pos;
     const uint64_t tag = (uint64_t)units[u].job << kJobShift;
     const uint64_t b0 = units[u].bstart + ((ti - ubase[u]) << kBrickBits);
     // the brick stream (non-empty bricks, appended in stream order)
@@ -777,6 +788,11 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
         bvoff[sb] = s0 + (bo & kVoxMask);
       }
     }
+    if constexpr (!kCodes) continue;
+    const uint32_t c = (uint32_t)__popcll(word);
+    const uint32_t inc = wave_incl_scan<uint32_t>(c);
+    uint64_t pos = (bo & kVoxMask) + (inc - c);
+    if (wprefix) wprefix[slot] = (uint32_t)pos;
     const uint64_t cb = (b0 + ((uint64_t)lane_id() << 6)) | tag;
     // dense payload unit: each voxel's winner from its cell, which is cleared
     uint32_t* cells = units[u].wide ? (uint32_t*)units[u].grid8 + (ti - ubase[u]) * kBrickVox +
@@ -788,6 +804,26 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
         winner[pos] = ~cells[bit];
         cells[bit] = 0u;
       }
+      codes[pos++] = cb + (uint64_t)bit;
+      word &= word - 1;
+    }
+  }
+}
+
+// Codes of the voxels of bricks [0, nb) of the stream, from their occupancy words: one
+// wave per brick, lane l = word l, written at the brick's stream offset.
+__global__ __launch_bounds__(256) void k_bricks_codes(const uint64_t* __restrict__ bwords,
+                                                      const uint64_t* __restrict__ bcode,
+                                                      const uint64_t* __restrict__ bvoff,
+                                                      uint64_t nb, uint64_t* __restrict__ codes) {
+  for (uint64_t b = wave_global(); b < nb; b += waves_total()) {
+    ull word = bwords[b * kBrickWords + lane_id()];
+    const uint32_t c = (uint32_t)__popcll(word);
+    const uint32_t inc = wave_incl_scan<uint32_t>(c);
+    uint64_t pos = bvoff[b] + (inc - c);
+    const uint64_t cb = bcode[b] + ((uint64_t)lane_id() << 6);
+    while (word) {
+      const int bit = __ffsll(word) - 1;
       codes[pos++] = cb + (uint64_t)bit;
       word &= word - 1;
     }
@@ -1296,28 +1332,43 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     ktimer_end(c);
     return hipGetLastError();
   };
+  // single-pass scans, each with its own zeroed status words: large-triangle columns,
+  // flagged bricks, brick counts (the last only while its packed prefixes stay below 2^62:
+  // at most bricks * (2^kNonEmptyShift + 4096))
+  const uint64_t w_col = scan_single_words(ntri), w_occ = scan_single_words(ntab);
+  ull* st_col = nullptr;
+  ull* st_occ = nullptr;
+  ull* st_bc = nullptr;
   auto occ_scan = [&]() {
-    return scan_exclusive<uint32_t>(
+    return scan_single<uint32_t>(
         OccLoad{c.table.as<uint32_t>()},
-        OccStore{c.occ_list.as<uint32_t>(), c.table.as<uint32_t>(), ctr, ntab}, ntab,
-        c.scan_scratch.as<uint32_t>(), c.stream);
+        OccStore{c.occ_list.as<uint32_t>(), c.table.as<uint32_t>(), ctr, ntab}, ntab, st_occ,
+        ctr + kCtrError, c.stream);
   };
 
   // ---- phase A: mark, compact counts; one synchronization at its end
   std::vector<uint64_t> uvoff(nu + 1);
   for (int attempt = 0;; attempt++) {
     const uint64_t bc_bound = dense ? ntab : std::min<uint64_t>(ntab, c.pool_cap);
+    const bool bc_single = bc_bound <= (3ull << 20);
+    const uint64_t w_bc = bc_single ? scan_single_words(bc_bound) : 0;
     SVO_TRY(ensure(c.bcount, bc_bound * 8 + 8));
+    SVO_TRY(ensure(c.scan_status, (w_col + w_occ + w_bc) * 8));
+    st_col = c.scan_status.as<ull>();
+    st_occ = st_col + w_col;
+    st_bc = st_occ + w_occ;
     SVO_TRY(fill_u64((uint64_t*)ctr, 0, kCtrPassClear, c.stream));
+    SVO_TRY(fill_u64(c.scan_status.as<uint64_t>(), 0, w_col + w_occ + w_bc, c.stream));
     // first pass: dense units mark directly, touch units flag their bricks; both collect
     // the large triangles
     if (dense)
       SVO_TRY(tile_pass(MarkAction{}, "vox_small_mark", ctr + kCtrQueueA, 1, 0));
     else
       SVO_TRY(tile_pass(TouchAction{}, "vox_small_touch", ctr + kCtrQueueA, 1, 0));
-    SVO_TRY((scan_exclusive<uint64_t>(
-        ColLoad{l_cols}, CountStore{l_off, ctr + kCtrCols, ctr + kCtrLargeCount}, ntri,
-        c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrLargeCount)));
+    SVO_TRY((scan_single<uint64_t>(ColLoad{l_cols},
+                                   CountStore{l_off, ctr + kCtrCols, ctr + kCtrLargeCount},
+                                   ntri, st_col, ctr + kCtrError, c.stream,
+                                   ctr + kCtrLargeCount)));
     if (dense) {
       SVO_TRY(large_pass(MarkAction{}, "vox_large_mark", 0));
       // number the flagged bricks in Morton order: table entry = rank + 1, occ = list
@@ -1336,10 +1387,16 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
                        c.pool8.as<uint8_t>(), c.pool.as<ull>(), c.bcount.as<uint64_t>());
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
-    SVO_TRY((scan_exclusive<uint64_t>(
-        ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
-        CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox, ctr + kCtrNumOcc}, bc_bound,
-        c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrNumOcc)));
+    if (bc_single)
+      SVO_TRY((scan_single<uint64_t>(
+          ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
+          CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox, ctr + kCtrNumOcc}, bc_bound,
+          st_bc, ctr + kCtrError, c.stream, ctr + kCtrNumOcc)));
+    else
+      SVO_TRY((scan_exclusive<uint64_t>(
+          ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
+          CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox, ctr + kCtrNumOcc}, bc_bound,
+          c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrNumOcc)));
     hipLaunchKernelGGL(k_unit_voxels, dim3((nu + 256) / 256), dim3(256), 0, c.stream,
                        c.occ_list.as<uint32_t>(), ctr, d_ubase, nu, c.bcount.as<uint64_t>(),
                        d_uvoff);
@@ -1433,6 +1490,9 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     }
     if (winner_walk) SVO_TRY(ensure(c.wprefix, n_occ * kBrickWords * 4));
     if (any_payload) SVO_TRY(ensure(c.winner, nvox * 4));
+    // geometry-only passes write no codes: the build reads the brick stream, and
+    // ensure_codes writes the codes for readers that want them
+    const bool codes_now = any_payload;
     SVO_TRY(ensure_keep(c.codes, (s0 + nvox) * 8, s0 * 8, c.stream));
     const uint64_t nb0 = c.n_bricks;
     SVO_TRY(ensure_keep(c.bwords, (nb0 + n_occ) * kBrickWords * 8, nb0 * kBrickWords * 8,
@@ -1440,7 +1500,8 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     SVO_TRY(ensure_keep(c.bcode, (nb0 + n_occ) * 8, nb0 * 8, c.stream));
     SVO_TRY(ensure_keep(c.bvoff, (nb0 + n_occ) * 8, nb0 * 8, c.stream));
     ktimer_begin(c, "brick_emit");
-    hipLaunchKernelGGL(k_brick_emit, dim3(brick_grid(n_occ)), dim3(256), 0, c.stream,
+    hipLaunchKernelGGL(codes_now ? k_brick_emit<true> : k_brick_emit<false>,
+                       dim3(brick_grid(n_occ)), dim3(256), 0, c.stream,
                        c.occ_list.as<uint32_t>(), ctr, d_units, d_ubase, nu, c.pool.as<ull>(),
                        c.bcount.as<uint64_t>(), c.codes.as<uint64_t>() + s0,
                        winner_walk ? c.wprefix.as<uint32_t>() : (uint32_t*)nullptr, s0,
@@ -1448,6 +1509,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
                        c.bvoff.as<uint64_t>() + nb0,
                        any_payload ? c.winner.as<uint32_t>() : (uint32_t*)nullptr);
     c.n_bricks = nb0 + n_nonempty;
+    if (!codes_now) c.codes_lazy = true;
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
 
@@ -1476,6 +1538,21 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   }
   c.n_vox = s0 + nvox;
   *nfilled = nvox;
+  return hipSuccess;
+}
+
+hipError_t ensure_codes(Context& c) {
+  if (!c.codes_lazy) return hipSuccess;
+  // every brick again (rewriting codes already there with the same values)
+  if (c.n_bricks) {
+    const unsigned grid =
+        (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((c.n_bricks * kWave + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_bricks_codes, dim3(grid), dim3(256), 0, c.stream,
+                       c.bwords.as<uint64_t>(), c.bcode.as<uint64_t>(), c.bvoff.as<uint64_t>(),
+                       c.n_bricks, c.codes.as<uint64_t>());
+    SVO_TRY(hipGetLastError());
+  }
+  c.codes_lazy = false;
   return hipSuccess;
 }
 

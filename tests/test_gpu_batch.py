@@ -7,7 +7,7 @@ import pytest
 import oracle
 from ooc_svo_builder_amd import SvoContext, generators
 
-from test_gpu_parity import adversarial_soup
+from test_gpu_parity import _corner_mesh, adversarial_soup
 
 pytestmark = pytest.mark.gpu
 
@@ -41,6 +41,7 @@ def _mix(seed):
     jobs.append(dict(records=generators.icosphere(subdivisions=1), payload=False, gridsize=2))
     jobs.append(dict(records=generators.icosphere(subdivisions=2) * np.float32(2.0),
                      payload=False, gridsize=128, bbox=(0, 0, 0, 2, 2, 2)))
+    jobs.append(dict(records=_corner_mesh(32), payload=False, gridsize=32))  # hazard
     return jobs
 
 
@@ -73,6 +74,12 @@ def test_batch_matches_plain():
         sub = [jobs[i] for i in (9, 3, 0, 12, 7)]
         _say("second batch of %d jobs" % len(sub))
         _check_batch(b, p, sub)
+        # grids of 16^3 and up only: the build takes the brick stream (with and without a
+        # payload job in the pass), the hazard job included
+        for idx in ((0, 1, 7, 10, 11, 15), (15, 3, 1)):
+            deep = [jobs[i] for i in idx]
+            _say("brick-stream batch %s" % (idx,))
+            _check_batch(b, p, deep)
         # knobs picked by the autotuner, and extreme ones: the same bytes
         _say("autotune")
         knobs = b.autotune(sub)

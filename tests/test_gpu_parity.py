@@ -176,6 +176,58 @@ def test_builder_payload_and_hazard(gpu_ctx):
         assert gpu_ctx.data(nd).tobytes() == data_w.tobytes()
 
 
+def _corner_mesh(G, far=True):
+    """A tiny triangle inside voxel (G-2, G-1, G-1), Morton code G^3 - 2 (the last voxel
+    is the second-to-last code: the builder's hazard case), plus a terrain elsewhere; far
+    = False leaves the corner triangle out."""
+    u = 1.0 / G
+    c = np.array([G - 1.5, G - 0.5, G - 0.5]) * u
+    tri = (c + np.array([[-0.2, -0.1, 0.0], [0.2, -0.1, 0.1], [0.0, 0.2, -0.1]]) * u)
+    rest = generators.terrain(n=17, seed=4, payload=False) * np.float32(0.5)
+    parts = [rest] + ([tri.reshape(1, 9).astype(np.float32)] if far else [])
+    return np.concatenate(parts).astype(np.float32)
+
+
+@pytest.mark.parametrize("G", [16, 64])
+def test_hazard_from_voxelizer(gpu_ctx, G):
+    """The hazard flag where the build reads the brick stream (no codes written): the
+    voxelized mesh's last voxel is code G^3 - 2."""
+    _say("hazard from voxelizer G=%d" % G)
+    for far in (True, False):
+        tris = _corner_mesh(G, far)
+        info = _run_both(gpu_ctx, tris, False, G, 2048)
+        assert info.hazard == (1 if far else 0)
+        codes = gpu_ctx.codes()
+        assert (codes[-1] == G ** 3 - 2) == far
+
+
+def test_codes_after_build_then_added_voxels(gpu_ctx):
+    """Codes of a geometry-only pass are written on demand: read after the build, and
+    before voxels added from outside (which then build with them)."""
+    _say("codes on demand")
+    G = 64
+    tris = generators.terrain(n=33, seed=9, payload=False) * np.float32(0.5)
+    want, _ = oracle_codes(tris, False, G, 1.0 / G)
+    gpu_ctx.load_mesh(tris, False)
+    gpu_ctx.voxelize(0, 0, G ** 3, 1.0 / G, G)
+    nn, nd, hz = gpu_ctx.build(G)
+    nodes_w, data_w, hz_w = oracle.build_octree(want, G, False)
+    assert gpu_ctx.nodes(nn).tobytes() == nodes_w.tobytes()
+    assert np.array_equal(gpu_ctx.codes(), want)
+    gpu_ctx.load_mesh(tris, False)
+    gpu_ctx.voxelize(0, 0, G ** 3, 1.0 / G, G)
+    extra = np.array([G ** 3 - 9, G ** 3 - 1], np.uint64)
+    assert want[-1] < extra[0]
+    gpu_ctx.add_voxels(extra)
+    allc = np.concatenate([want, extra])
+    assert np.array_equal(gpu_ctx.codes(), allc)
+    nn, nd, hz = gpu_ctx.build(G)
+    nodes_w, data_w, hz_w = oracle.build_octree(allc, G, False)
+    assert (nn, nd, hz) == (len(nodes_w), len(data_w), hz_w)
+    assert gpu_ctx.nodes(nn).tobytes() == nodes_w.tobytes()
+    assert gpu_ctx.data(nd).tobytes() == data_w.tobytes()
+
+
 def _run_both(ctx, tris, payload, G, L, color_mode=0, levels=False):
     info = ctx.run(tris, payload, G, L, color_mode=color_mode, levels=levels)
     nodes_w, data_w, hz_w, np_w, nf_w = oracle.pipeline(tris, payload, G, L, color_mode=color_mode,
