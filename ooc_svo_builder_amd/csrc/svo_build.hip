@@ -85,6 +85,9 @@ struct Arena {
   uint64_t* isize;
   float* cache;  // 6 floats per node
   uint64_t* pstart;
+  // sum of (size - 1) over the node's preceding siblings: its start is its parent's start
+  // plus this (written by the bottom-up pass that creates the parent)
+  uint64_t* sib;
 };
 
 // -levels data record of an internal node (OctreeBuilder.cpp:82-99): the caches of its
@@ -193,7 +196,12 @@ struct ParentStore {
           const uint64_t cj = code[j];
           if (T.parent(cj) != pc) break;
           m |= 1u << (cj & 7);
-          sz += csize ? csize[j] : 1ull;
+          if (csize) {
+            A.sib[M.off[h] + j] = sz - 1 - (j - i);
+            sz += csize[j];
+          } else {
+            sz += 1;
+          }
           je = j + 1;
         }
         A.code[base + ex] = pc;
@@ -373,6 +381,48 @@ __global__ __launch_bounds__(256) void k_emit_level(Arena A, LevelMeta M, Tags T
     put_data(out_rec, J.dbase + idx, A.cache + (co + i) * 6);
   }
   put_node(nodes, gpos, data, st + sz - 1 - (uint32_t)__popc(cm), offsets_of(cm));
+}
+
+// Every node of heights hs..H in one launch (geometry records, no -levels data): one
+// thread per node, flattened over the contiguous levels of the arena. A node's start is
+// the sum of the sibling offsets along its path to the root (start(P) plus its own sib),
+// which needs only the bottom-up results, so all levels are written at once.
+__global__ __launch_bounds__(256) void k_emit_upper(Arena A, LevelMeta M, Tags T,
+                                                    const JobDev* __restrict__ jobs, int hs,
+                                                    int H, uint64_t node_cap,
+                                                    uint64_t* __restrict__ nodes) {
+  if (M.flag[0]) return;
+  const uint64_t g = M.off[hs] + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int h = hs;
+  while (h <= H && g >= M.off[h] + M.n[h]) h++;
+  if (h > H) return;
+  const uint64_t i = g - M.off[h];
+  const uint64_t code = A.code[g];
+  const JobDev& J = jobs[T.job(code)];
+  const uint64_t sz = A.size[g];
+  const uint32_t cm = A.mask[g];
+  uint64_t st, pos;
+  if (J.depth == h) {  // the job's root: start 0, last record
+    st = 0;
+    pos = sz - 1;
+  } else {
+    const uint64_t po = M.off[h + 1] + A.pidx[g];
+    uint64_t sp = 0;  // start of the parent
+    uint64_t cur = po;
+    for (int l = h + 1; l < J.depth; l++) {
+      sp += A.sib[cur];
+      cur = M.off[l + 1] + A.pidx[cur];
+    }
+    st = sp + A.sib[g];
+    pos = sp + A.size[po] - 1 - (uint32_t)__popc((uint32_t)A.mask[po]) + (i - A.first[po]);
+  }
+  const uint64_t gpos = J.nbase + pos;
+  if (pos >= J.n_nodes || gpos >= node_cap) {
+    M.flag[1] = 1;
+    return;
+  }
+  A.start[g] = st;
+  put_node(nodes, gpos, 0ull, st + sz - 1 - (uint32_t)__popc(cm), offsets_of(cm));
 }
 
 // One thread per leaf: position = children_base(parent) + rank among its siblings.
@@ -739,6 +789,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     SVO_TRY(ensure(c.lvl_first, cap * 4));
     SVO_TRY(ensure(c.lvl_start, cap * 8));
     SVO_TRY(ensure(c.lvl_pidx, cap * 4));
+    SVO_TRY(ensure(c.lvl_sib, cap * 8));
     // meta | per job: root size, root internal size, hazard | single-pass scan status
     const uint64_t meta_words = 2 * 64 + 8 + 3 * (uint64_t)n_jobs;
     uint64_t st_words = 0;
@@ -759,7 +810,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
             c.lvl_first.as<uint32_t>(), c.lvl_start.as<uint64_t>(), c.lvl_pidx.as<uint32_t>(),
             cap, levels ? c.lvl_isize.as<uint64_t>() : nullptr,
             levels ? c.lvl_cache.as<float>() : nullptr,
-            levels ? c.lvl_pstart.as<uint64_t>() : nullptr};
+            levels ? c.lvl_pstart.as<uint64_t>() : nullptr, c.lvl_sib.as<uint64_t>()};
     unsigned long long* meta = c.lvl_aux.as<unsigned long long>();
     LevelMeta M{meta, meta + 64, meta + 128};
     unsigned long long* root_size = meta + 136;
@@ -795,8 +846,19 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
                        node_cap,
                        c.nodes.as<uint64_t>(), c.records.as<float>(), out_rec);
     SVO_TRY(hipGetLastError());
-    // ---- top-down: heights H .. 1, then the leaves
-    for (int h = H; h >= std::max(h0, 1); h--) {
+    // ---- top-down: heights H .. 1 (one launch without -levels), then the leaves
+    const int hs = std::max(h0, 1);
+    if (!levels && H >= hs) {
+      uint64_t nup = 0;
+      for (int h = hs; h <= H; h++) nup += bound[h];
+      ktimer_begin(c, "emit_children");
+      hipLaunchKernelGGL(k_emit_upper, dim3((unsigned)std::max<uint64_t>((nup + 255) / 256, 1)),
+                         dim3(256), 0, c.stream, A, M, T, (const JobDev*)d_jobs, hs, H, node_cap,
+                         c.nodes.as<uint64_t>());
+      ktimer_end(c);
+      SVO_TRY(hipGetLastError());
+    }
+    for (int h = H; levels && h >= hs; h--) {
       ktimer_begin(c, "emit_children");
       hipLaunchKernelGGL(k_emit_level, dim3((unsigned)std::max<uint64_t>((bound[h] + 255) / 256, 1)),
                          dim3(256), 0, c.stream, A, M, T, (const JobDev*)d_jobs, h, node_cap,
