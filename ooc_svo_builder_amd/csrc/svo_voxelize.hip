@@ -674,18 +674,28 @@ __device__ __forceinline__ uint64_t waves_total() {
 // brick's rank, count its voxels, and zero the bytes that were set. The bytes live in the
 // unit's dense grid (at the brick's index) or in the slot pool (at its rank).
 // ubase[u] = first table entry of unit u (n_units + 1 entries).
+// Dense passes flag only bricks that hold voxels, so the rank is the brick's place in the
+// brick stream: there 'pool' is the stream's word array, bcode (non-null) receives the
+// tagged brick codes and the table entries are cleared here.
 __global__ __launch_bounds__(256) void k_brick_count(const uint32_t* __restrict__ occ,
                                                      const ull* __restrict__ ctr, uint64_t cap,
                                                      const VoxParams* __restrict__ units,
                                                      const uint64_t* __restrict__ ubase,
                                                      uint32_t n_units, uint8_t* __restrict__ pool8,
                                                      ull* __restrict__ pool,
-                                                     uint64_t* __restrict__ bcount) {
+                                                     uint64_t* __restrict__ bcount,
+                                                     uint64_t* __restrict__ bcode,
+                                                     uint32_t* __restrict__ table) {
   // more flagged bricks than slots (touch path): the pass is redone with a larger pool
   const uint64_t n_occ = min((uint64_t)ctr[kCtrNumOcc], cap);
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
     const uint64_t ti = occ[wv];
-    const uint32_t u = upper_index<uint64_t>(ubase, n_units + 1, ti);
+    const uint32_t u = n_units == 1 ? 0u : upper_index<uint64_t>(ubase, n_units + 1, ti);
+    if (bcode && lane_id() == 0) {
+      bcode[wv] = (units[u].bstart + ((ti - ubase[u]) << kBrickBits)) |
+                  ((uint64_t)units[u].job << kJobShift);
+      table[ti] = 0u;
+    }
     ull m = 0;
     if (units[u].wide) {
       // 4-byte cells (~winner, 0 = unmarked): kept for k_brick_emit, which clears them
@@ -745,6 +755,19 @@ struct CountStore {
   const ull* n_dev;  // actual element count (device)
   __device__ void operator()(uint64_t i, uint64_t ex, uint64_t v) const {
     off[i] = ex;
+    if (i == *n_dev - 1) *total = (ull)(ex + v);
+  }
+};
+// Brick counts of a dense pass: also each brick's first voxel in the stream (s0 + offset).
+struct BrickOffStore {
+  uint64_t* off;
+  ull* total;
+  const ull* n_dev;
+  uint64_t* bvoff;
+  uint64_t s0;
+  __device__ void operator()(uint64_t i, uint64_t ex, uint64_t v) const {
+    off[i] = ex;
+    bvoff[i] = s0 + (ex & kVoxMask);
     if (i == *n_dev - 1) *total = (ull)(ex + v);
   }
 };
@@ -1259,7 +1282,6 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
       SVO_TRY(fill_u64(c.grid8.as<uint64_t>(), 0, dense_bytes / 8, c.stream));
       c.grid_zeroed = dense_bytes;
     }
-    SVO_TRY(ensure(c.pool, ntab * kBrickWords * 8));
     for (uint32_t u = 0; u < nu; u++) {
       P[u].dense = 1;
       P[u].wide = units[u].stride == 21 ? 1 : 0;
@@ -1288,6 +1310,17 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     SVO_TRY(upload_async(c, d_units, img.data(), img.size()));
   }
 
+  // dense passes write their bricks straight into the brick stream (every flagged brick
+  // holds voxels): room for all of the pass's bricks after the nb0 already there
+  const uint64_t s0 = c.n_vox;
+  const uint64_t nb0 = c.n_bricks;
+  if (dense) {
+    SVO_TRY(ensure_keep(c.bwords, (nb0 + ntab) * kBrickWords * 8, nb0 * kBrickWords * 8,
+                        c.stream));
+    SVO_TRY(ensure_keep(c.bcode, (nb0 + ntab) * 8, nb0 * 8, c.stream));
+    SVO_TRY(ensure_keep(c.bvoff, (nb0 + ntab) * 8, nb0 * 8, c.stream));
+  }
+  ull* pool_words = dense ? c.bwords.as<ull>() + nb0 * kBrickWords : c.pool.as<ull>();
   SVO_TRY(ensure(c.large_unit, ntri * 4));
   SVO_TRY(ensure(c.large_tri, ntri * 4));
   SVO_TRY(ensure(c.large_cols, ntri * 4));
@@ -1305,7 +1338,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     ActArgs A;
     A.table = c.table.as<uint32_t>();
     A.pool8 = c.pool8.as<uint8_t>();
-    A.pool = c.pool.as<ull>();
+    A.pool = pool_words;
     A.wprefix = c.wprefix.as<uint32_t>();
     A.winner = c.winner.as<uint32_t>();
     A.ctr = ctr;
@@ -1384,14 +1417,28 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     ktimer_begin(c, "brick_count");
     hipLaunchKernelGGL(k_brick_count, dim3(brick_grid(bc_bound)), dim3(256), 0, c.stream,
                        c.occ_list.as<uint32_t>(), ctr, bc_bound, d_units, d_ubase, nu,
-                       c.pool8.as<uint8_t>(), c.pool.as<ull>(), c.bcount.as<uint64_t>());
+                       c.pool8.as<uint8_t>(), pool_words, c.bcount.as<uint64_t>(),
+                       dense ? c.bcode.as<uint64_t>() + nb0 : (uint64_t*)nullptr,
+                       dense ? c.table.as<uint32_t>() : (uint32_t*)nullptr);
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
-    if (bc_single)
+    if (bc_single && dense)
+      SVO_TRY((scan_single<uint64_t>(
+          ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
+          BrickOffStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox, ctr + kCtrNumOcc,
+                        c.bvoff.as<uint64_t>() + nb0, s0},
+          bc_bound, st_bc, ctr + kCtrError, c.stream, ctr + kCtrNumOcc)));
+    else if (bc_single)
       SVO_TRY((scan_single<uint64_t>(
           ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
           CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox, ctr + kCtrNumOcc}, bc_bound,
           st_bc, ctr + kCtrError, c.stream, ctr + kCtrNumOcc)));
+    else if (dense)
+      SVO_TRY((scan_exclusive<uint64_t>(
+          ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
+          BrickOffStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox, ctr + kCtrNumOcc,
+                        c.bvoff.as<uint64_t>() + nb0, s0},
+          bc_bound, c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrNumOcc)));
     else
       SVO_TRY((scan_exclusive<uint64_t>(
           ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
@@ -1432,7 +1479,6 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   }
 
   // ---- job layout: voxel and data-record ranges
-  const uint64_t s0 = c.n_vox;
   bool any_payload = false;
   if (batch) {
     // the passes of a batch take its jobs' partitions in job order: a job's range starts
@@ -1494,24 +1540,29 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     // ensure_codes writes the codes for readers that want them
     const bool codes_now = any_payload;
     SVO_TRY(ensure_keep(c.codes, (s0 + nvox) * 8, s0 * 8, c.stream));
-    const uint64_t nb0 = c.n_bricks;
-    SVO_TRY(ensure_keep(c.bwords, (nb0 + n_occ) * kBrickWords * 8, nb0 * kBrickWords * 8,
-                        c.stream));
-    SVO_TRY(ensure_keep(c.bcode, (nb0 + n_occ) * 8, nb0 * 8, c.stream));
-    SVO_TRY(ensure_keep(c.bvoff, (nb0 + n_occ) * 8, nb0 * 8, c.stream));
-    ktimer_begin(c, "brick_emit");
-    hipLaunchKernelGGL(codes_now ? k_brick_emit<true> : k_brick_emit<false>,
+    if (!dense) {
+      SVO_TRY(ensure_keep(c.bwords, (nb0 + n_occ) * kBrickWords * 8, nb0 * kBrickWords * 8,
+                          c.stream));
+      SVO_TRY(ensure_keep(c.bcode, (nb0 + n_occ) * 8, nb0 * 8, c.stream));
+      SVO_TRY(ensure_keep(c.bvoff, (nb0 + n_occ) * 8, nb0 * 8, c.stream));
+    }
+    // a dense pass has its brick stream entries already (k_brick_count); the emit pass
+    // writes codes and winners (rewriting those entries with the same values)
+    if (!dense || codes_now) {
+      ktimer_begin(c, "brick_emit");
+      hipLaunchKernelGGL(codes_now ? k_brick_emit<true> : k_brick_emit<false>,
                        dim3(brick_grid(n_occ)), dim3(256), 0, c.stream,
-                       c.occ_list.as<uint32_t>(), ctr, d_units, d_ubase, nu, c.pool.as<ull>(),
+                       c.occ_list.as<uint32_t>(), ctr, d_units, d_ubase, nu, pool_words,
                        c.bcount.as<uint64_t>(), c.codes.as<uint64_t>() + s0,
                        winner_walk ? c.wprefix.as<uint32_t>() : (uint32_t*)nullptr, s0,
                        c.bwords.as<uint64_t>() + nb0 * kBrickWords, c.bcode.as<uint64_t>() + nb0,
                        c.bvoff.as<uint64_t>() + nb0,
                        any_payload ? c.winner.as<uint32_t>() : (uint32_t*)nullptr);
+      ktimer_end(c);
+      SVO_TRY(hipGetLastError());
+    }
     c.n_bricks = nb0 + n_nonempty;
     if (!codes_now) c.codes_lazy = true;
-    ktimer_end(c);
-    SVO_TRY(hipGetLastError());
 
     if (any_payload) {
       if (winner_walk) {
@@ -1529,12 +1580,15 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
       ktimer_end(c);
       SVO_TRY(hipGetLastError());
     }
-    ktimer_begin(c, "brick_clear");
-    hipLaunchKernelGGL(k_brick_clear, dim3((unsigned)std::min<uint64_t>((n_occ + 255) / 256, 4096)),
-                       dim3(256), 0, c.stream, c.occ_list.as<uint32_t>(), ctr,
-                       c.table.as<uint32_t>());
-    ktimer_end(c);
-    SVO_TRY(hipGetLastError());
+    if (!dense) {  // (dense passes cleared their table entries in k_brick_count)
+      ktimer_begin(c, "brick_clear");
+      hipLaunchKernelGGL(k_brick_clear,
+                         dim3((unsigned)std::min<uint64_t>((n_occ + 255) / 256, 4096)),
+                         dim3(256), 0, c.stream, c.occ_list.as<uint32_t>(), ctr,
+                         c.table.as<uint32_t>());
+      ktimer_end(c);
+      SVO_TRY(hipGetLastError());
+    }
   }
   c.n_vox = s0 + nvox;
   *nfilled = nvox;
