@@ -576,11 +576,6 @@ __device__ __forceinline__ int last_le8(const uint32_t* a, uint32_t v) {
   if (a[k + 1] <= v) k += 1;
   return k;
 }
-// position of the n-th set bit of m (n < popcount(m))
-__device__ __forceinline__ int nth_bit(uint32_t m, uint32_t n) {
-  for (uint32_t q = 0; q < n; q++) m &= m - 1;
-  return __ffs(m) - 1;
-}
 
 __global__ __launch_bounds__(kWave * kEmitWaves) void k_emit_bricks(
     Arena A, LevelMeta M, Tags T, const JobDev* __restrict__ jobs,
@@ -596,6 +591,18 @@ __global__ __launch_bounds__(kWave * kEmitWaves) void k_emit_bricks(
   __shared__ uint32_t s_vw[kEmitWaves][64];   // voxels before the word in the brick
   __shared__ uint64_t s_wd[kEmitWaves][64];   // the words
   __shared__ uint32_t s_bo[kEmitWaves][512];  // byte voxel ranges (relative to S)
+  __shared__ uint64_t s_offs[256];            // offsets_of(m) for every child mask m
+  __shared__ uint8_t s_nth[256][8];           // nth_bit(m, n)
+  {
+    const uint32_t m = threadIdx.x;  // the block has 256 threads: one mask each
+    s_offs[m] = offsets_of(m);
+    uint32_t x = m;
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+      s_nth[m][n] = (uint8_t)(x ? __ffs(x) - 1 : 0);
+      x &= x - 1;
+    }
+  }
   const int ws = threadIdx.x / kWave;
   const int lane = lane_id();
   const int g = lane >> 3, q = lane & 7;
@@ -653,25 +660,25 @@ __global__ __launch_bounds__(kWave * kEmitWaves) void k_emit_bricks(
     return;
   }
   uint64_t* out = nodes + 3 * (J.nbase + S);
+  const int payload = J.payload;
   for (uint32_t r = lane; r < R; r += kWave) {
-    uint64_t d0, d1, d2;
+    // find the record's node: its child mask m (internal nodes) and the first two words
+    uint64_t d0 = 0, d1;
+    uint32_t m;
+    bool leaf = false;
     if (r >= RG) {  // a group node
-      const int gg = nth_bit(B.mask4, r - RG);
-      const uint32_t m3 = s_gm3[ws][gg];
-      d0 = 0;
-      d1 = S + s_go[ws][gg] + s_gs3[ws][gg] - 1u - (uint32_t)__popc(m3);
-      d2 = offsets_of(m3);
+      const int gg = s_nth[B.mask4][r - RG];
+      m = s_gm3[ws][gg];
+      d1 = S + s_go[ws][gg] + s_gs3[ws][gg] - 1u - (uint32_t)__popc(m);
     } else {
       const int gg = last_le8(s_go[ws], r);
       const uint32_t m3 = s_gm3[ws][gg];
       const uint32_t r1 = r - s_go[ws][gg];
       const uint32_t l3 = s_gs3[ws][gg] - 1u, nw = (uint32_t)__popc(m3);
       if (r1 >= l3 - nw) {  // a word node
-        const int ww = gg * 8 + nth_bit(m3, r1 - (l3 - nw));
-        const uint32_t m2 = s_m2[ws][ww];
-        d0 = 0;
-        d1 = S + s_wo[ws][ww] + s_s2[ws][ww] - 1u - (uint32_t)__popc(m2);
-        d2 = offsets_of(m2);
+        const int ww = gg * 8 + s_nth[m3][r1 - (l3 - nw)];
+        m = s_m2[ws][ww];
+        d1 = S + s_wo[ws][ww] + s_s2[ws][ww] - 1u - (uint32_t)__popc(m);
       } else {
         const int ww = gg * 8 + last_le8(&s_wo[ws][gg * 8], r);
         const uint32_t m2 = s_m2[ws][ww];
@@ -679,20 +686,21 @@ __global__ __launch_bounds__(kWave * kEmitWaves) void k_emit_bricks(
         const uint32_t l2 = s_s2[ws][ww] - 1u, nbz = (uint32_t)__popc(m2);
         const uint64_t word = s_wd[ws][ww];
         if (r2 >= l2 - nbz) {  // a byte node: its voxels start at its own range start
-          const int j = nth_bit(m2, r2 - (l2 - nbz));
-          d0 = 0;
+          const int j = s_nth[m2][r2 - (l2 - nbz)];
+          m = (uint32_t)(word >> (8 * j)) & 0xFFu;
           d1 = S + s_bo[ws][ww * 8 + j];
-          d2 = offsets_of((uint32_t)(word >> (8 * j)) & 0xFFu);
         } else {  // a voxel
           const int j = last_le8(&s_bo[ws][ww * 8], r);
           const uint32_t k = r - s_bo[ws][ww * 8 + j];
           const uint32_t before = (uint32_t)__popcll(word & ((1ull << (8 * j)) - 1ull));
-          d0 = J.payload ? 1ull + vox0 + s_vw[ws][ww] + before + k : 1ull;
+          d0 = payload ? 1ull + vox0 + s_vw[ws][ww] + before + k : 1ull;
           d1 = 0;
-          d2 = ~0ull;
+          m = 0;
+          leaf = true;
         }
       }
     }
+    const uint64_t d2 = leaf ? ~0ull : s_offs[m];
     out[3 * (uint64_t)r] = d0;
     out[3 * (uint64_t)r + 1] = d1;
     out[3 * (uint64_t)r + 2] = d2;
