@@ -232,6 +232,18 @@ struct ParentStore {
   }
 };
 
+// Level bookkeeping before a build: all zero, n[0] = V and, when the four lowest levels
+// come from the brick stream, n[4] = NB and off[5] = NB.
+__global__ __launch_bounds__(256) void k_build_init(unsigned long long* meta, uint64_t words,
+                                                    uint64_t V, uint64_t NB, int fast) {
+  for (uint64_t i = threadIdx.x; i < words; i += blockDim.x) {
+    unsigned long long v = 0;
+    if (i == 0) v = V;
+    if (fast && (i == 4 || i == 64 + 5)) v = NB;
+    meta[i] = v;
+  }
+}
+
 // Per job, after the bottom-up passes: node count (a single null root for an empty
 // grid, finalizeTree pads it; the single leaf of a 1^3 grid), node base (exclusive scan
 // over the jobs), data count (-levels adds one record per internal node), the records
@@ -778,8 +790,13 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     bound[h] = std::min(bound[h - 1], b);
     if (fast && h == 4) bound[h] = NB;
   }
-  SVO_TRY(ensure(c.jobdev, n_jobs * sizeof(JobDev)));
-  SVO_TRY(upload_async(c, c.jobdev.p, jd.data(), n_jobs * sizeof(JobDev)));
+  // the voxelizer's last upload usually holds these already
+  if (c.jobdev_img.size() != jd.size() ||
+      memcmp(c.jobdev_img.data(), jd.data(), n_jobs * sizeof(JobDev)) != 0) {
+    SVO_TRY(ensure(c.jobdev, n_jobs * sizeof(JobDev)));
+    SVO_TRY(upload_async(c, c.jobdev.p, jd.data(), n_jobs * sizeof(JobDev)));
+  }
+  c.jobdev_img.clear();  // k_job_bases writes into it
   // data records: payload records were written by the voxelizer at dbase + 1 ..
   SVO_TRY(ensure_keep(c.records, n_records * 32, n_records * 32, c.stream));
   JobDev* d_jobs = c.jobdev.as<JobDev>();
@@ -825,11 +842,11 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     unsigned long long* root_isize = root_size + n_jobs;
     unsigned long long* brick_hazard = root_isize + n_jobs;
     // n[0] = V, off[1] = 0, flags = 0, root sizes = 0, hazard flags = 0, scan status = 0
-    SVO_TRY(fill_u64((uint64_t*)meta, 0, meta_words + st_words, c.stream));
-    SVO_TRY(fill_u64((uint64_t*)meta, V, 1, c.stream));
-    if (fast) {  // n[4] = NB, off[4] = 0, off[5] = NB
-      SVO_TRY(fill_u64((uint64_t*)meta + 4, NB, 1, c.stream));
-      SVO_TRY(fill_u64((uint64_t*)meta + 64 + 5, NB, 1, c.stream));
+    // (fast: n[4] = NB, off[4] = 0, off[5] = NB)
+    hipLaunchKernelGGL(k_build_init, dim3(1), dim3(256), 0, c.stream, meta,
+                       meta_words + st_words, V, NB, fast ? 1 : 0);
+    SVO_TRY(hipGetLastError());
+    if (fast) {
       ktimer_begin(c, "brick_level");
       hipLaunchKernelGGL(k_brick_level, dim3((unsigned)((NB * kWave + 255) / 256)), dim3(256), 0,
                          c.stream, A, M, T, (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),

@@ -1089,9 +1089,28 @@ hipError_t upload_async(Context& c, void* dst, const void* src, size_t bytes) {
   return hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, c.stream);
 }
 
+// Small read-backs: one kernel copies every request into the pinned host buffer (mapped
+// into the device's address space), instead of one DMA copy per request.
+constexpr int kGatherMax = 4;
+struct GatherArgs {
+  const uint64_t* src[kGatherMax];
+  uint64_t* dst[kGatherMax];
+  uint32_t words[kGatherMax];
+  int n;
+};
+__global__ __launch_bounds__(256) void k_gather_host(GatherArgs g) {
+  for (int k = 0; k < g.n; k++)
+    for (uint32_t i = threadIdx.x; i < g.words[k]; i += blockDim.x) g.dst[k][i] = g.src[k][i];
+}
+
 hipError_t read_back(Context& c, const ReadReq* reqs, int n) {
   size_t total = 0;
-  for (int k = 0; k < n; k++) total += (reqs[k].bytes + 255) & ~(size_t)255;
+  bool small = n <= kGatherMax;
+  for (int k = 0; k < n; k++) {
+    total += (reqs[k].bytes + 255) & ~(size_t)255;
+    small = small && reqs[k].bytes % 8 == 0 && (uintptr_t)reqs[k].src % 8 == 0;
+  }
+  small = small && total <= (64u << 10);
   if (total > c.hread_cap) {
     SVO_TRY(hipStreamSynchronize(c.stream));
     if (c.hread) (void)hipHostFree(c.hread);
@@ -1102,11 +1121,24 @@ hipError_t read_back(Context& c, const ReadReq* reqs, int n) {
     c.hread_cap = cap;
   }
   size_t off = 0;
-  for (int k = 0; k < n; k++) {
-    if (reqs[k].bytes)
-      SVO_TRY(hipMemcpyAsync((char*)c.hread + off, reqs[k].src, reqs[k].bytes,
-                             hipMemcpyDeviceToHost, c.stream));
-    off += (reqs[k].bytes + 255) & ~(size_t)255;
+  if (small) {
+    GatherArgs g{};
+    g.n = n;
+    for (int k = 0; k < n; k++) {
+      g.src[k] = (const uint64_t*)reqs[k].src;
+      g.dst[k] = (uint64_t*)((char*)c.hread + off);
+      g.words[k] = (uint32_t)(reqs[k].bytes / 8);
+      off += (reqs[k].bytes + 255) & ~(size_t)255;
+    }
+    hipLaunchKernelGGL(k_gather_host, dim3(1), dim3(256), 0, c.stream, g);
+    SVO_TRY(hipGetLastError());
+  } else {
+    for (int k = 0; k < n; k++) {
+      if (reqs[k].bytes)
+        SVO_TRY(hipMemcpyAsync((char*)c.hread + off, reqs[k].src, reqs[k].bytes,
+                               hipMemcpyDeviceToHost, c.stream));
+      off += (reqs[k].bytes + 255) & ~(size_t)255;
+    }
   }
   SVO_TRY(hipStreamSynchronize(c.stream));
   off = 0;
@@ -1524,6 +1556,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   }
   SVO_TRY(ensure(c.jobdev, jd.size() * sizeof(JobDev)));
   SVO_TRY(upload_async(c, c.jobdev.p, jd.data(), jd.size() * sizeof(JobDev)));
+  c.jobdev_img = jd;
 
   // ---- phase B: emit sorted codes, payload, clear; no synchronization
   if (n_occ) {
