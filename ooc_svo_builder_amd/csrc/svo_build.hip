@@ -249,20 +249,21 @@ __global__ __launch_bounds__(256) void k_build_init(unsigned long long* meta, ui
 // over the jobs), data count (-levels adds one record per internal node), the records
 // the builder writes itself (OctreeBuilder.cpp:25-29: the null record, plus the white
 // record of geometry-only builds), and the hazard flag. One block.
-__global__ __launch_bounds__(256) void k_job_bases(JobDev* jobs, uint32_t n_jobs,
-                                                   const unsigned long long* root_size,
-                                                   const unsigned long long* root_isize,
-                                                   const unsigned long long* brick_hazard,
-                                                   const uint64_t* __restrict__ codes,
-                                                   Tags T, LevelMeta M, uint64_t node_cap,
-                                                   uint64_t* __restrict__ nodes,
-                                                   const float* __restrict__ in_rec,
-                                                   float* __restrict__ records) {
-  __shared__ uint64_t wsum[256 / kWave];
-  __shared__ uint64_t carry;
+// BLOCK threads of one block; wsum holds BLOCK / 64 words.
+template <int BLOCK>
+__device__ __forceinline__ void job_bases(JobDev* jobs, uint32_t n_jobs,
+                                          const unsigned long long* root_size,
+                                          const unsigned long long* root_isize,
+                                          const unsigned long long* brick_hazard,
+                                          const uint64_t* __restrict__ codes, Tags T,
+                                          LevelMeta M, uint64_t node_cap,
+                                          uint64_t* __restrict__ nodes,
+                                          const float* __restrict__ in_rec,
+                                          float* __restrict__ records, uint64_t* wsum,
+                                          uint64_t& carry) {
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
-  for (uint32_t j0 = 0; j0 < n_jobs; j0 += 256) {
+  for (uint32_t j0 = 0; j0 < n_jobs; j0 += BLOCK) {
     const uint32_t j = j0 + threadIdx.x;
     uint64_t nn = 0;
     if (j < n_jobs) {
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(256) void k_job_bases(JobDev* jobs, uint32_t n_jobs
       nn = (J.nvox == 0 || J.depth == 0) ? 1ull : root_size[j];
     }
     uint64_t tot;
-    const uint64_t inc = block_incl_scan<uint64_t, 256>(nn, wsum, tot);
+    const uint64_t inc = block_incl_scan<uint64_t, BLOCK>(nn, wsum, tot);
     const uint64_t nb = carry + inc - nn;
     if (j < n_jobs) {
       JobDev& J = jobs[j];
@@ -318,6 +319,40 @@ __global__ __launch_bounds__(256) void k_job_bases(JobDev* jobs, uint32_t n_jobs
     __syncthreads();
   }
   if (threadIdx.x == 0) M.flag[2] = carry;
+}
+
+// The bottom-up levels h_lo .. H-1 whose element counts are small (bounded on the host),
+// in one block, level after level (a level's scan in chunks of the block with a running
+// carry, a barrier before the next level reads it), then the per-job bases.
+constexpr int kTailBlock = 1024;
+constexpr uint64_t kTailLevelMax = 8192;  // largest level bound the tail block takes
+__global__ __launch_bounds__(kTailBlock) void k_levels_tail(
+    Arena A, LevelMeta M, Tags T, JobDev* jobs, uint32_t n_jobs, const uint64_t* leaves,
+    uint32_t* leaf_parent, unsigned long long* root_size, unsigned long long* root_isize,
+    const float* leaf_rec, int h_lo, int H, const unsigned long long* brick_hazard,
+    uint64_t node_cap, uint64_t* __restrict__ nodes, const float* __restrict__ in_rec,
+    float* __restrict__ records) {
+  __shared__ uint32_t wsum32[kTailBlock / kWave];
+  __shared__ uint64_t wsum[kTailBlock / kWave];
+  __shared__ uint64_t carry;
+  for (int h = h_lo; h < H; h++) {
+    HeadLoad ld{A, M, T, jobs, leaves, h};
+    ParentStore st{A, M, T, jobs, leaves, leaf_parent, root_size, root_isize, leaf_rec, h};
+    __syncthreads();  // the previous level's nodes and counts are written
+    const uint64_t n = M.n[h];
+    uint32_t run = 0;
+    for (uint64_t b = 0; b < n; b += kTailBlock) {
+      const uint64_t i = b + threadIdx.x;
+      const uint32_t v = i < n ? ld(i) : 0u;
+      uint32_t tot;
+      const uint32_t inc = block_incl_scan<uint32_t, kTailBlock>(v, wsum32, tot);
+      if (i < n) st(i, run + inc - v, v);
+      run += tot;
+    }
+  }
+  __syncthreads();
+  job_bases<kTailBlock>(jobs, n_jobs, root_size, root_isize, brick_hazard, leaves, T, M,
+                        node_cap, nodes, in_rec, records, wsum, carry);
 }
 
 // -levels: the data records of a subtree are written in post-order (a node completes
@@ -817,8 +852,15 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     SVO_TRY(ensure(c.lvl_sib, cap * 8));
     // meta | per job: root size, root internal size, hazard | single-pass scan status
     const uint64_t meta_words = 2 * 64 + 8 + 3 * (uint64_t)n_jobs;
+    // heights from h_tail up go to the single-block tail kernel
+    int h_tail = H;
+    for (int h = h0; h < H; h++)
+      if (bound[h] <= kTailLevelMax) {
+        h_tail = h;
+        break;
+      }
     uint64_t st_words = 0;
-    for (int h = h0; h < H; h++) st_words += scan_single_words(bound[h]);
+    for (int h = h0; h < h_tail; h++) st_words += scan_single_words(bound[h]);
     SVO_TRY(ensure(c.lvl_aux, (meta_words + st_words) * 8));
     if (levels) {
       SVO_TRY(ensure(c.lvl_isize, cap * 8));
@@ -857,7 +899,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
 
     // ---- bottom-up: heights 1 .. H
     unsigned long long* status = meta + meta_words;
-    for (int h = h0; h < H; h++) {
+    for (int h = h0; h < h_tail; h++) {
       HeadLoad ld{A, M, T, d_jobs, c.codes.as<uint64_t>(), h};
       ParentStore st{A, M, T, d_jobs, c.codes.as<uint64_t>(), c.lvl_leafp.as<uint32_t>(),
                      root_size, root_isize, c.records.as<float>() + 8, h};
@@ -866,9 +908,10 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
                                      M.n + h)));
       status += scan_single_words(bound[h]);
     }
-    hipLaunchKernelGGL(k_job_bases, dim3(1), dim3(256), 0, c.stream, d_jobs, n_jobs, root_size,
-                       root_isize, fast ? brick_hazard : nullptr, c.codes.as<uint64_t>(), T, M,
-                       node_cap,
+    hipLaunchKernelGGL(k_levels_tail, dim3(1), dim3(kTailBlock), 0, c.stream, A, M, T, d_jobs,
+                       n_jobs, c.codes.as<uint64_t>(), c.lvl_leafp.as<uint32_t>(), root_size,
+                       root_isize, c.records.as<float>() + 8, h_tail, H,
+                       fast ? brick_hazard : nullptr, node_cap,
                        c.nodes.as<uint64_t>(), c.records.as<float>(), out_rec);
     SVO_TRY(hipGetLastError());
     // ---- top-down: heights H .. 1 (one launch without -levels), then the leaves
