@@ -156,8 +156,11 @@ struct HeadLoad {
     if (h > 0 && M.off[h] + M.n[h] > A.cap) return 0u;
     const uint64_t* code = (h == 0) ? leaves : A.code + M.off[h];
     const uint64_t c = code[i];
-    if (jobs[T.job(c)].depth <= h) return 0u;
-    return (i == 0 || T.parent(c) != T.parent(code[i - 1])) ? 1u : 0u;
+    const uint64_t cp = i ? code[i - 1] : 0ull;
+    // one job (no tags): its depth does not wait for the code
+    const int depth = T.tag_mask ? jobs[T.job(c)].depth : jobs[0].depth;
+    if (depth <= h) return 0u;
+    return (i == 0 || T.parent(c) != T.parent(cp)) ? 1u : 0u;
   }
 };
 struct ParentStore {
@@ -189,20 +192,26 @@ struct ParentStore {
       if (base + ex >= A.cap) {
         M.flag[0] = 1;
       } else {
-        const uint64_t pc = T.parent(code[i]);
+        // the (at most 8) children follow the head: all their loads issued at once
+        uint64_t cj[8], sj[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          cj[k] = (i + k < nch) ? code[i + k] : ~0ull;  // ~0: no node has this parent
+          sj[k] = (csize && i + k < nch) ? csize[i + k] : 1ull;
+        }
+        const uint64_t pc = T.parent(cj[0]);
         uint32_t m = 0;
         uint64_t sz = 1, je = i;
-        for (uint64_t j = i; j < nch && j < i + 8; j++) {
-          const uint64_t cj = code[j];
-          if (T.parent(cj) != pc) break;
-          m |= 1u << (cj & 7);
-          if (csize) {
-            A.sib[M.off[h] + j] = sz - 1 - (j - i);
-            sz += csize[j];
-          } else {
-            sz += 1;
+        bool more = true;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          more = more && T.parent(cj[k]) == pc;
+          if (more) {
+            m |= 1u << (cj[k] & 7);
+            if (csize) A.sib[M.off[h] + i + k] = sz - 1 - (uint64_t)k;
+            sz += sj[k];
+            je = i + k + 1;
           }
-          je = j + 1;
         }
         A.code[base + ex] = pc;
         A.size[base + ex] = sz;
@@ -488,7 +497,7 @@ __global__ __launch_bounds__(256) void k_emit_leaves(Arena A, LevelMeta M, Tags 
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nl) return;
   const JobDev& J = jobs[T.job(codes[i])];
-  if (J.depth == 0) return;  // 1^3 grid: its leaf is the root, written by k_job_bases
+  if (J.depth == 0) return;  // 1^3 grid: its leaf is the root, written with the job bases
   const uint64_t po = M.off[1] + leaf_parent[i];
   const uint32_t cnt = (uint32_t)__popc((uint32_t)A.mask[po]);
   const uint64_t cb = A.start[po] + A.size[po] - 1 - cnt;
@@ -581,17 +590,16 @@ __global__ __launch_bounds__(256) void k_brick_level(Arena A, LevelMeta M, Tags 
                                                      unsigned long long* hazard) {
   const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   if (w >= nb) return;  // uniform per wave
+  const uint64_t c0 = bcode[w];  // (loaded with the words)
   const BrickShape B = brick_shape(bwords, w);
   if (lane_id() == kWave - 1) {
-    // the job's last voxel is code G^3 - 2 (the case k_job_bases flags) when that voxel
+    // the job's last voxel is code G^3 - 2 (the case the job bases flag) when that voxel
     // is set and G^3 - 1 is not: bits 62 and 63 of the last word of the grid's last brick
-    const uint64_t c0 = bcode[w];
     const uint32_t j = T.job(c0);
     const uint64_t G = jobs[j].gridsize;
     if ((c0 & ~T.tag_mask) == G * G * G - kBrickVox && (B.word >> 62) == 1ull) hazard[j] = 1;
   }
   if (lane_id() == 0) {
-    const uint64_t c0 = bcode[w];
     const uint64_t code = (c0 & T.tag_mask) | ((c0 & ~T.tag_mask) >> 12);
     const uint64_t k = M.off[4] + w;
     if (k >= A.cap) {
@@ -831,7 +839,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     SVO_TRY(ensure(c.jobdev, n_jobs * sizeof(JobDev)));
     SVO_TRY(upload_async(c, c.jobdev.p, jd.data(), n_jobs * sizeof(JobDev)));
   }
-  c.jobdev_img.clear();  // k_job_bases writes into it
+  c.jobdev_img.clear();  // the job bases are written into it
   // data records: payload records were written by the voxelizer at dbase + 1 ..
   SVO_TRY(ensure_keep(c.records, n_records * 32, n_records * 32, c.stream));
   JobDev* d_jobs = c.jobdev.as<JobDev>();
