@@ -44,6 +44,31 @@ WORKLOADS = {
 }
 
 
+TRAFFIC_FILES = {"terrain2048": "hbm_traffic_terrain2048_binary.md"}
+
+
+def measured_traffic(workload, timer):
+    """(HBM bytes per launch, SIMD VALU issue share) of the kernel behind 'timer' from the
+    committed rocprofv3 counter summary of this workload (None where there is none)."""
+    name = TRAFFIC_FILES.get(workload)
+    path = os.path.join(ROOT, "profiles", name) if name else None
+    if not path or not os.path.exists(path):
+        return None, None
+    traffic = share = None
+    kernel = None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        try:
+            if len(cells) == 8 and cells[1] == timer:
+                kernel = cells[0]
+                traffic = float(cells[7]) * 1e6
+            elif len(cells) == 9 and kernel and cells[0] == kernel and cells[8] != "nan":
+                share = float(cells[8])
+        except ValueError:
+            pass
+    return traffic, share
+
+
 def make_mesh(kind, payload):
     from ooc_svo_builder_amd import generators
     if kind == "terrain":
@@ -163,6 +188,7 @@ def main():
         algo_bytes_per_launch = build_bytes / per_step_launches
     avg_launch_ms = dom_ms / launches
     achieved = algo_bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    traffic, valu_share = measured_traffic(args.workload, dom)
 
     value = N * args.steps / (ms_total * 1e-3)
     stage_ms = {k: v / args.steps for k, v in st.items()}
@@ -193,7 +219,8 @@ def main():
         "knobs": dict(zip(("small_cols", "direct_cols", "tile_blocks", "large_blocks"),
                           ctx.get_tuning())),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "simd_valu_share": valu_share,
                      "avg_launch_ms": avg_launch_ms,
                      "algo_bytes_per_launch": algo_bytes_per_launch},
         "wall_s_timed": wall,
