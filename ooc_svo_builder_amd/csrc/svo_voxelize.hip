@@ -677,20 +677,20 @@ __device__ __forceinline__ uint64_t waves_total() {
 // Dense passes flag only bricks that hold voxels, so the rank is the brick's place in the
 // brick stream: there 'pool' is the stream's word array, bcode (non-null) receives the
 // tagged brick codes and the table entries are cleared here.
-__global__ __launch_bounds__(256) void k_brick_count(const uint32_t* __restrict__ occ,
-                                                     const ull* __restrict__ ctr, uint64_t cap,
-                                                     const VoxParams* __restrict__ units,
-                                                     const uint64_t* __restrict__ ubase,
-                                                     uint32_t n_units, uint8_t* __restrict__ pool8,
-                                                     ull* __restrict__ pool,
-                                                     uint64_t* __restrict__ bcount,
-                                                     uint64_t* __restrict__ bcode,
-                                                     uint32_t* __restrict__ table) {
-  // more flagged bricks than slots (touch path): the pass is redone with a larger pool
-  const uint64_t n_occ = min((uint64_t)ctr[kCtrNumOcc], cap);
+// ONE: a single unit (its parameters are uniform, read with scalar loads).
+template <bool ONE>
+__device__ __forceinline__ void brick_count_body(const uint32_t* __restrict__ occ,
+                                                 uint64_t n_occ,
+                                                 const VoxParams* __restrict__ units,
+                                                 const uint64_t* __restrict__ ubase,
+                                                 uint32_t n_units, uint8_t* __restrict__ pool8,
+                                                 ull* __restrict__ pool,
+                                                 uint64_t* __restrict__ bcount,
+                                                 uint64_t* __restrict__ bcode,
+                                                 uint32_t* __restrict__ table) {
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
     const uint64_t ti = occ[wv];
-    const uint32_t u = n_units == 1 ? 0u : upper_index<uint64_t>(ubase, n_units + 1, ti);
+    const uint32_t u = ONE ? 0u : upper_index<uint64_t>(ubase, n_units + 1, ti);
     if (bcode && lane_id() == 0) {
       bcode[wv] = (units[u].bstart + ((ti - ubase[u]) << kBrickBits)) |
                   ((uint64_t)units[u].job << kJobShift);
@@ -701,12 +701,18 @@ __global__ __launch_bounds__(256) void k_brick_count(const uint32_t* __restrict_
       // 4-byte cells (~winner, 0 = unmarked): kept for k_brick_emit, which clears them
       const uint4* src = (const uint4*)((const uint32_t*)units[u].grid8 +
                                         (ti - ubase[u]) * kBrickVox + lane_id() * 64);
+      // four loads at a time: few registers, so that many waves stay resident
+#pragma unroll 1
+      for (int g = 0; g < 16; g += 4) {
+        uint4 v[4];
 #pragma unroll
-      for (int q = 0; q < 16; q++) {
-        const uint4 v = src[q];
-        const uint32_t nib = (v.x != 0u ? 1u : 0u) | (v.y != 0u ? 2u : 0u) |
-                             (v.z != 0u ? 4u : 0u) | (v.w != 0u ? 8u : 0u);
-        m |= (ull)nib << (4 * q);
+        for (int q = 0; q < 4; q++) v[q] = src[g + q];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const uint32_t nib = (v[q].x != 0u ? 1u : 0u) | (v[q].y != 0u ? 2u : 0u) |
+                               (v[q].z != 0u ? 4u : 0u) | (v[q].w != 0u ? 8u : 0u);
+          m |= (ull)nib << (4 * (g + q));
+        }
       }
     } else {
       // bytes: load q of lane l is the 16 bytes at q * 1024 + 16 l (each load of the wave
@@ -747,6 +753,23 @@ __global__ __launch_bounds__(256) void k_brick_count(const uint32_t* __restrict_
     // superset of the bricks that end up holding voxels)
     if (lane_id() == kWave - 1) bcount[wv] = tot + (tot ? (1ull << kNonEmptyShift) : 0ull);
   }
+}
+
+__global__ __launch_bounds__(256) void k_brick_count(const uint32_t* __restrict__ occ,
+                                                     const ull* __restrict__ ctr, uint64_t cap,
+                                                     const VoxParams* __restrict__ units,
+                                                     const uint64_t* __restrict__ ubase,
+                                                     uint32_t n_units, uint8_t* __restrict__ pool8,
+                                                     ull* __restrict__ pool,
+                                                     uint64_t* __restrict__ bcount,
+                                                     uint64_t* __restrict__ bcode,
+                                                     uint32_t* __restrict__ table) {
+  // more flagged bricks than slots (touch path): the pass is redone with a larger pool
+  const uint64_t n_occ = min((uint64_t)ctr[kCtrNumOcc], cap);
+  if (n_units == 1)
+    brick_count_body<true>(occ, n_occ, units, ubase, n_units, pool8, pool, bcount, bcode, table);
+  else
+    brick_count_body<false>(occ, n_occ, units, ubase, n_units, pool8, pool, bcount, bcode, table);
 }
 
 struct CountStore {
