@@ -12,9 +12,9 @@
 //     out by an exclusive scan of their column counts.
 // Columns run along the axis of the triangle's dominant normal component. Per column the
 // projection test in the plane across that axis is evaluated once (it does not depend
-// on the position along the column), and only the voxels within two voxels of the
-// plane slab are tested; every voxel outside that range fails the reference's plane
-// test (the margin is far above the FP32 error of the test). Triangles with a
+// on the position along the column), and only the voxels of the plane slab, widened by
+// a margin, are tested; every voxel outside that range fails the reference's plane
+// test (the margin is over four times the FP32 error of the test). Triangles with a
 // non-finite normal or far-away vertices walk full columns.
 // Marked voxels go to a sparse brick grid; compaction of the occupied bricks in Morton
 // order yields the sorted unique codes the builder needs (the reference sorts its side
@@ -276,7 +276,7 @@ __device__ __forceinline__ bool column_range(const Walk& K, float pa, float pb, 
     const float base = K.f[F_KA] * (float)a + K.f[F_KB] * (float)b;
     const float l = fminf(fmaxf(base + K.f[F_ALO], (float)lo - 1.0f), (float)hi + 1.0f);
     const float h = fmaxf(fminf(base + K.f[F_AHI], (float)hi + 1.0f), (float)lo - 1.0f);
-    lo = max(lo, (int)floorf(l));
+    lo = max(lo, (int)ceilf(l));
     hi = min(hi, (int)floorf(h));
   }
   return lo <= hi;
@@ -1264,10 +1264,15 @@ static void fill_params(const VoxUnit& h, const Context& c, VoxParams& P) {
   P.unit = h.unit;
   P.unit_div = 1.0f / h.unit;  // voxelizer.cpp:164, IEEE on the host as on the reference
   P.coord_limit = 4.0f * (float)h.gridsize * h.unit;
-  // The reference's FP32 plane test (t + d1) * (t + d2) differs from its exact value by
-  // less than ~3e-6 * coord_limit; along the column axis (|n_w| >= 1/2 here) that is
-  // below 2.1e-5 * G voxels. The margin is 0.5 + 4e-5 * G voxels, at least twice that.
-  P.slab_margin = 0.5 + 4e-5 * (double)h.gridsize;
+  // Along a column the voxels c that pass the reference's plane test lie, in exact
+  // arithmetic, in [l, h] (the two zeros of (t + d1) and (t + d2)). Its FP32 evaluation
+  // moves them by at most: t = n.p with |p| <= G u: 3 roundings of terms <= |n|_1 G u,
+  // ~3.1e-7 G u; d1, d2 with |v0| <= coord_limit = 4 G u: ~1.7e-6 G u; so ~2e-6 G u, i.e.
+  // 3.5e-6 G voxels along the column axis (|n_w| >= 1/sqrt(3)); p = fl(x u) adds ~3e-7 G.
+  // This range is computed from float coefficients (KA, KB, alpha rounded to float, base
+  // summed in float): below 1.8e-6 G voxels. The margin, 0.03 + 2.5e-5 G voxels, is over
+  // four times the sum; the candidates are the integers in [l - margin, h + margin].
+  P.slab_margin = 0.03 + 2.5e-5 * (double)h.gridsize;
   P.inv_unit = 1.0 / (double)h.unit;
   P.small_cols = c.tune.small_cols;
   P.direct_cols = c.tune.direct_cols;

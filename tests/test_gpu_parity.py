@@ -87,6 +87,33 @@ def test_adversarial_codes(gpu_ctx, seed, G):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("G", [2048, 16384])
+def test_far_corner_codes(gpu_ctx, G):
+    """Large grids, small triangles at the far corner (the largest coordinates, where the
+    float rounding of the plane test and of the per-column voxel range is largest): the
+    top 256^3 Morton window of the grid, including triangles crossing its faces, vertices
+    on voxel corners and triangles in voxel-aligned planes."""
+    _say("far corner codes G=%d" % G)
+    rng = np.random.default_rng(G)
+    u = 1.0 / G
+    S = 256
+    lo = G - S
+    parts = []
+    c = rng.uniform(lo - 2, G + 2, (3000, 1, 3))
+    parts.append(c + rng.normal(0, 1.5, (3000, 3, 3)))
+    k = rng.integers(lo - 1, G + 1, (800, 1, 3)) + rng.integers(-2, 3, (800, 3, 3))
+    parts.append(k.astype(np.float64))
+    a = rng.uniform(lo, G, (400, 1, 3)) + rng.normal(0, 2.0, (400, 3, 3))
+    a[:, :, rng.integers(0, 3)] = rng.integers(lo, G, (400, 1))
+    parts.append(a)
+    tris = (np.concatenate(parts) * u).astype(np.float32).reshape(-1, 9)
+    mstart, mend = G ** 3 - S ** 3, G ** 3
+    got = gpu_codes(gpu_ctx, tris, False, G, u, mstart, mend)
+    want, _ = oracle_codes(tris, False, G, u, mstart, mend)
+    assert len(want) > 10000
+    assert np.array_equal(got, want)
+
+
 def test_partition_window_codes(gpu_ctx):
     """One Morton-range partition: the clamp to the partition box (voxelizer.cpp:198-204)."""
     _say("partition window codes")
