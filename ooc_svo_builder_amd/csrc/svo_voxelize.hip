@@ -1068,6 +1068,16 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t v, uint64_t n) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
+// two small ranges zeroed by one launch
+__global__ void k_zero2_u64(uint64_t* a, uint64_t na, uint64_t* b, uint64_t nb) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += stride) {
+    if (i < na)
+      a[i] = 0;
+    else
+      b[i - na] = 0;
+  }
+}
 static unsigned fill_grid(uint64_t n) {
   uint64_t b = (n + 255) / 256;
   return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b, 8192));
@@ -1450,8 +1460,13 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     st_col = c.scan_status.as<ull>();
     st_occ = st_col + w_col;
     st_bc = st_occ + w_occ;
-    SVO_TRY(fill_u64((uint64_t*)ctr, 0, kCtrPassClear, c.stream));
-    SVO_TRY(fill_u64(c.scan_status.as<uint64_t>(), 0, w_col + w_occ + w_bc, c.stream));
+    {  // pass counters and scan status words
+      const uint64_t nz = kCtrPassClear + w_col + w_occ + w_bc;
+      hipLaunchKernelGGL(k_zero2_u64, dim3(fill_grid(nz)), dim3(256), 0, c.stream,
+                         (uint64_t*)ctr, (uint64_t)kCtrPassClear, c.scan_status.as<uint64_t>(),
+                         w_col + w_occ + w_bc);
+      SVO_TRY(hipGetLastError());
+    }
     // first pass: dense units mark directly, touch units flag their bricks; both collect
     // the large triangles
     if (dense)
