@@ -961,6 +961,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
       SVO_TRY(hipGetLastError());
     }
     unsigned long long flags[3] = {0, 0, 0};
+    if (c.build_done_ev) SVO_TRY(hipEventRecord(c.build_done_ev, c.stream));
     {
       const ReadReq rq[3] = {{flags, M.flag, 24},
                              {jd.data(), d_jobs, n_jobs * sizeof(JobDev)},

@@ -682,9 +682,10 @@ int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payloa
   TRYC(hipEventRecord(c.ev2, c.stream), "svo_run");
   uint64_t nn = 0, nd = 0;
   int hz = 0;
+  c.build_done_ev = c.ev3;  // recorded by the build before its final read-back
   r = svo_build(ctx, gridsize, generate_levels, &nn, &nd, &hz);
+  c.build_done_ev = nullptr;
   if (r) return r;
-  TRYC(hipEventRecord(c.ev3, c.stream), "svo_run");
   TRYC(hipEventSynchronize(c.ev3), "svo_run");
   TRYC(hipEventElapsedTime(&ms_p, c.ev0, c.ev1), "svo_run");
   TRYC(hipEventElapsedTime(&ms_v, c.ev1, c.ev2), "svo_run");
@@ -816,9 +817,10 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
   }
   c.nfilled = nf;
   TRYC(hipEventRecord(c.ev2, c.stream), "svo_batch_run");
+  c.build_done_ev = c.ev3;  // recorded by the build before its final read-back
   hipError_t e = build_jobs(c, n_jobs, true, 0);
+  c.build_done_ev = nullptr;
   if (e != hipSuccess) return fail(e, "svo_batch_run");
-  TRYC(hipEventRecord(c.ev3, c.stream), "svo_batch_run");
   TRYC(hipEventSynchronize(c.ev3), "svo_batch_run");
   TRYC(hipEventElapsedTime(&ms_p, c.ev0, c.ev1), "svo_batch_run");
   TRYC(hipEventElapsedTime(&ms_v, c.ev1, c.ev2), "svo_batch_run");

@@ -195,12 +195,15 @@ struct Context {
   // pinned host staging for small uploads (bump allocated; wraps after a stream sync)
   void* stage = nullptr;
   size_t stage_cap = 0, stage_used = 0;
-  // pinned host landing area for device-to-host reads (counters, layouts)
+  // pinned host landing area for device-to-host reads (counters, layouts); its first
+  // 256 bytes hold the completion flag the gather kernel raises
   void* hread = nullptr;
   size_t hread_cap = 0;
+  uint32_t hread_seq = 0;
 
   StageTimes times;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  hipEvent_t build_done_ev = nullptr;  // when set, the build records it before its read-back
 
   bool ktiming = false;
   std::vector<PendingTimer> kpending;

@@ -22,6 +22,8 @@
 // partition order that covers them (voxelizer.cpp:263) by a second pass doing
 // atomicMin on the triangle index per voxel rank.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1131,9 +1133,15 @@ struct GatherArgs {
   uint32_t words[kGatherMax];
   int n;
 };
-__global__ __launch_bounds__(256) void k_gather_host(GatherArgs g) {
+// Then it raises *flag = seq (after a system-scope fence), which the host polls: a
+// shorter wait than a stream synchronization.
+__global__ __launch_bounds__(256) void k_gather_host(GatherArgs g, volatile uint32_t* flag,
+                                                     uint32_t seq) {
   for (int k = 0; k < g.n; k++)
     for (uint32_t i = threadIdx.x; i < g.words[k]; i += blockDim.x) g.dst[k][i] = g.src[k][i];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) *flag = seq;
 }
 
 hipError_t read_back(Context& c, const ReadReq* reqs, int n) {
@@ -1144,6 +1152,7 @@ hipError_t read_back(Context& c, const ReadReq* reqs, int n) {
     small = small && reqs[k].bytes % 8 == 0 && (uintptr_t)reqs[k].src % 8 == 0;
   }
   small = small && total <= (64u << 10);
+  total += 256;  // the completion flag
   if (total > c.hread_cap) {
     SVO_TRY(hipStreamSynchronize(c.stream));
     if (c.hread) (void)hipHostFree(c.hread);
@@ -1153,7 +1162,8 @@ hipError_t read_back(Context& c, const ReadReq* reqs, int n) {
     SVO_TRY(hipHostMalloc(&c.hread, cap, hipHostMallocDefault));
     c.hread_cap = cap;
   }
-  size_t off = 0;
+  const size_t base = 256;
+  size_t off = base;
   if (small) {
     GatherArgs g{};
     g.n = n;
@@ -1163,8 +1173,27 @@ hipError_t read_back(Context& c, const ReadReq* reqs, int n) {
       g.words[k] = (uint32_t)(reqs[k].bytes / 8);
       off += (reqs[k].bytes + 255) & ~(size_t)255;
     }
-    hipLaunchKernelGGL(k_gather_host, dim3(1), dim3(256), 0, c.stream, g);
+    volatile uint32_t* flag = (volatile uint32_t*)c.hread;
+    const uint32_t seq = ++c.hread_seq;
+    hipLaunchKernelGGL(k_gather_host, dim3(1), dim3(256), 0, c.stream, g, flag, seq);
     SVO_TRY(hipGetLastError());
+    // poll the flag for a short while (the usual case: the stream is almost done), then
+    // fall back to a blocking synchronization (long kernels ahead, or a failed launch)
+    bool seen = false;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t k = 0;; k++) {
+      if (*flag == seq) {
+        seen = true;
+        break;
+      }
+      if ((k & 255) == 255 &&
+          std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(2000))
+        break;
+    }
+    if (seen)
+      std::atomic_thread_fence(std::memory_order_acquire);
+    else
+      SVO_TRY(hipStreamSynchronize(c.stream));
   } else {
     for (int k = 0; k < n; k++) {
       if (reqs[k].bytes)
@@ -1172,9 +1201,9 @@ hipError_t read_back(Context& c, const ReadReq* reqs, int n) {
                                hipMemcpyDeviceToHost, c.stream));
       off += (reqs[k].bytes + 255) & ~(size_t)255;
     }
+    SVO_TRY(hipStreamSynchronize(c.stream));
   }
-  SVO_TRY(hipStreamSynchronize(c.stream));
-  off = 0;
+  off = base;
   for (int k = 0; k < n; k++) {
     memcpy(reqs[k].dst, (char*)c.hread + off, reqs[k].bytes);
     off += (reqs[k].bytes + 255) & ~(size_t)255;
