@@ -762,6 +762,244 @@ __global__ __launch_bounds__(kWave * kEmitWaves) void k_emit_bricks(
   }
 }
 
+// ---------------------------------------------------------------- upper levels at once
+// Fast path: every height >= 5 straight from the sorted brick nodes (height 4) in four
+// launches, instead of one scan per height. Brick i heads its ancestor at height l (is the
+// first brick below it) iff l <= T(i): T(i) = the job's depth D for the first brick of a
+// job, else 4 + (highest bit where its code differs from brick i-1's) / 3, capped at D.
+// So the index of brick i's ancestor at height l is incl_l(i) - 1, incl_l(i) = the number
+// of bricks j <= i with T(j) >= l. With S4(i) = the brick subtree sizes before brick i and
+//   A_l(i) = S4(i) + sum_{m=5..l} (bricks before i with T >= m)
+// (the nodes below height l+1 that precede brick i in the stream), a node P of height l
+// whose bricks are [i0, i1) has size(P) = A_l(i1) - A_l(i0), and the sizes of P's
+// preceding siblings sum to A_l(head of P) - A_l(head of its parent's first child).
+// k_up_count counts per tile of 256 bricks, k_up_scan scans the counts per height (one
+// block), k_up_apply writes every node's code, first child, parent index and A_l at its
+// head brick (and A_l at every job's end), k_up_finish turns them into sizes, child masks
+// and sibling offsets. No atomics.
+constexpr int kUpTile = 256;
+constexpr int kUpLevels = 17;  // heights 5 .. 21
+
+__device__ __forceinline__ int brick_top(const uint64_t* __restrict__ code4, uint64_t i, Tags T,
+                                         int D) {
+  if (D <= 4) return 4;
+  const uint64_t c = code4[i];
+  if (i == 0) return D;
+  const uint64_t p = code4[i - 1];
+  if ((c & T.tag_mask) != (p & T.tag_mask)) return D;
+  const uint64_t x = (c ^ p) & ~T.tag_mask;
+  if (!x) return 4;
+  return min(D, 4 + (63 - __clzll((long long)x)) / 3);
+}
+__device__ __forceinline__ int job_depth(const JobDev* jobs, Tags T, uint64_t code) {
+  return T.tag_mask ? jobs[T.job(code)].depth : jobs[0].depth;
+}
+
+// per tile: counts of T >= l for l = 5 .. H, and the sum of the brick subtree sizes
+__global__ __launch_bounds__(kUpTile) void k_up_count(Arena A, Tags T, const JobDev* jobs,
+                                                      uint64_t NB, int H,
+                                                      uint32_t* __restrict__ tile_cnt,
+                                                      uint64_t* __restrict__ tile_s4) {
+  __shared__ uint32_t wcnt[kUpTile / kWave][kUpLevels];
+  __shared__ uint64_t ws4[kUpTile / kWave];
+  const uint64_t i = (uint64_t)blockIdx.x * kUpTile + threadIdx.x;
+  int t = 4;
+  uint64_t s4 = 0;
+  if (i < NB) {
+    t = brick_top(A.code, i, T, job_depth(jobs, T, A.code[i]));
+    s4 = A.size[i];
+  }
+  const int w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < kUpLevels; k++) {
+    const uint64_t b = __ballot(t >= 5 + k);
+    if (lane_id() == 0) wcnt[w][k] = (uint32_t)__popcll(b);
+  }
+  const uint64_t sw = shfl_t<uint64_t>(wave_incl_scan<uint64_t>(s4), kWave - 1);
+  if (lane_id() == 0) ws4[w] = sw;
+  __syncthreads();
+  if (threadIdx.x < kUpLevels && 5 + (int)threadIdx.x <= H) {
+    uint32_t sum = 0;
+    for (int q = 0; q < kUpTile / kWave; q++) sum += wcnt[q][threadIdx.x];
+    tile_cnt[(uint64_t)blockIdx.x * kUpLevels + threadIdx.x] = sum;
+  }
+  if (threadIdx.x == 0) {
+    uint64_t sum = 0;
+    for (int q = 0; q < kUpTile / kWave; q++) sum += ws4[q];
+    tile_s4[blockIdx.x] = sum;
+  }
+}
+
+// one block, a wave per height (and one for the sizes): the exclusive prefix of the tile
+// counts (in place, 64 tiles at a time); then the node counts and arena offsets of all
+// heights and A_l at the end of the stream; an arena too small is flagged
+__global__ __launch_bounds__(1024) void k_up_scan(Arena A, LevelMeta M, uint64_t NB, int H,
+                                                  uint32_t ntiles,
+                                                  uint32_t* __restrict__ tile_cnt,
+                                                  uint64_t* __restrict__ tile_s4,
+                                                  uint64_t* __restrict__ a_end) {
+  __shared__ uint64_t s_tot[kUpLevels + 1];
+  const int w = threadIdx.x / kWave;
+  for (int k = w; k <= kUpLevels; k += 1024 / kWave) {
+    if (k < kUpLevels && 5 + k > H) continue;  // uniform per wave
+    uint64_t carry = 0;
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += kWave) {
+      const uint32_t t = t0 + lane_id();
+      if (k == kUpLevels) {  // the brick subtree sizes
+        const uint64_t v = t < ntiles ? tile_s4[t] : 0ull;
+        const uint64_t inc = wave_incl_scan<uint64_t>(v);
+        if (t < ntiles) tile_s4[t] = carry + inc - v;
+        carry += shfl_t<uint64_t>(inc, kWave - 1);
+      } else {
+        uint32_t* cell = tile_cnt + (uint64_t)t * kUpLevels + k;
+        const uint32_t v = t < ntiles ? *cell : 0u;
+        const uint32_t inc = wave_incl_scan<uint32_t>(v);
+        if (t < ntiles) *cell = (uint32_t)carry + inc - v;
+        carry += shfl_t<uint32_t>(inc, kWave - 1);
+      }
+    }
+    if (lane_id() == 0) s_tot[k] = carry;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t off = NB;  // off[5]
+    uint64_t a = s_tot[kUpLevels];  // S4 at the end
+    for (int l = 5; l <= H; l++) {
+      const uint64_t n = s_tot[l - 5];
+      M.n[l] = n;
+      M.off[l] = off;
+      off += n;
+      M.off[l + 1] = off;
+      a += n;
+      a_end[l - 5] = a;
+    }
+    if (off > A.cap) M.flag[0] = 1;
+  }
+}
+
+// Per brick: the parent index of its brick node, the fields of every node it heads, A_l at
+// each of those nodes (kept in the node's start slot until the records are written), and
+// at a job's first brick A_l as the previous job's end (the last brick: its job's end is
+// the end of the stream).
+__global__ __launch_bounds__(kUpTile) void k_up_apply(Arena A, LevelMeta M, Tags T,
+                                                      const JobDev* jobs, uint64_t NB, int H,
+                                                      const uint32_t* __restrict__ tile_pre,
+                                                      const uint64_t* __restrict__ tile_s4,
+                                                      const uint64_t* __restrict__ a_end,
+                                                      uint64_t* __restrict__ job_end) {
+  __shared__ uint32_t wpre[kUpTile / kWave][kUpLevels];
+  __shared__ uint64_t ws4[kUpTile / kWave];
+  if (M.flag[0]) return;  // arena too small: the build is redone with a larger one
+  const uint64_t i = (uint64_t)blockIdx.x * kUpTile + threadIdx.x;
+  const bool live = i < NB;
+  uint64_t c = 0, s4 = 0, cp = 0;
+  int D = 4, t = 4;
+  if (live) {
+    c = A.code[i];
+    s4 = A.size[i];
+    if (i) cp = A.code[i - 1];
+    D = job_depth(jobs, T, c);
+    t = brick_top(A.code, i, T, D);
+  }
+  const int w = threadIdx.x / kWave;
+  const uint64_t le = (2ull << lane_id()) - 1ull;  // lanes <= this one
+  uint64_t bal[kUpLevels];
+#pragma unroll
+  for (int k = 0; k < kUpLevels; k++) {
+    bal[k] = __ballot(t >= 5 + k);
+    if (lane_id() == 0) wpre[w][k] = (uint32_t)__popcll(bal[k]);
+  }
+  const uint64_t s4i = wave_incl_scan<uint64_t>(s4);
+  if (lane_id() == kWave - 1) ws4[w] = s4i;
+  __syncthreads();
+  if (threadIdx.x < kUpLevels) {  // exclusive prefix over the waves, per height
+    uint32_t run = 0;
+    for (int q = 0; q < kUpTile / kWave; q++) {
+      const uint32_t x = wpre[q][threadIdx.x];
+      wpre[q][threadIdx.x] = run;
+      run += x;
+    }
+  }
+  __syncthreads();
+  uint64_t S4 = tile_s4[blockIdx.x] + s4i - s4;
+  for (int q = 0; q < w; q++) S4 += ws4[q];
+  if (!live) return;
+  const uint64_t tag = c & T.tag_mask, rest = c & ~T.tag_mask;
+  const bool job_first = i == 0 || (cp & T.tag_mask) != tag;
+  const uint32_t jb = T.job(c);
+  A.start[i] = S4;  // A_4 at the brick
+  if (D >= 5)
+    A.pidx[i] = tile_pre[(uint64_t)blockIdx.x * kUpLevels] + wpre[w][0] +
+                (uint32_t)__popcll(bal[0] & le) - 1u;
+  uint64_t a = S4;
+  uint32_t prev_incl = 0;
+#pragma unroll
+  for (int k = 0; k < kUpLevels; k++) {
+    const int l = 5 + k;
+    if (l > H) break;
+    const uint32_t incl = tile_pre[(uint64_t)blockIdx.x * kUpLevels + k] + wpre[w][k] +
+                          (uint32_t)__popcll(bal[k] & le);
+    const bool head = t >= l;
+    a += incl - (head ? 1u : 0u);  // A_l(i)
+    if (head && l <= D) {
+      const uint64_t g = M.off[l] + incl - 1u;
+      A.code[g] = tag | (rest >> (3 * (l - 4)));
+      A.first[g] = l == 5 ? (uint32_t)i : prev_incl - 1u;
+      if (k + 1 < kUpLevels && l < D) {
+        const uint32_t nxt = tile_pre[(uint64_t)blockIdx.x * kUpLevels + k + 1] +
+                             wpre[w][k + 1] + (uint32_t)__popcll(bal[k + 1] & le);
+        A.pidx[g] = nxt - 1u;
+      }
+      A.start[g] = a;
+    }
+    if (job_first && i) job_end[(uint64_t)T.job(cp) * kUpLevels + k] = a;
+    if (i == NB - 1) job_end[(uint64_t)jb * kUpLevels + k] = a_end[k];
+    prev_incl = incl;
+  }
+}
+
+// Every node of heights 4 .. H: size and child mask (heights >= 5), sibling offset (below
+// its job's root); each job: its tree size.
+__global__ __launch_bounds__(256) void k_up_finish(Arena A, LevelMeta M, Tags T,
+                                                   const JobDev* jobs, int H,
+                                                   const uint64_t* __restrict__ job_end,
+                                                   unsigned long long* root_size) {
+  if (M.flag[0]) return;
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int l = 4;
+  while (l <= H && g >= M.off[l] + M.n[l]) l++;
+  if (l > H) return;
+  const uint64_t co = M.off[l], i = g - co;
+  const uint64_t code = A.code[g];
+  const uint32_t jb = T.job(code);
+  const int D = job_depth(jobs, T, code);
+  const uint64_t a0 = A.start[g];
+  if (l >= 5) {
+    const bool last = i + 1 == M.n[l] || T.job(A.code[g + 1]) != jb;
+    const uint64_t a1 = last ? job_end[(uint64_t)jb * kUpLevels + (l - 5)] : A.start[g + 1];
+    const uint64_t size = a1 - a0;
+    A.size[g] = size;
+    // children: the nodes of height l-1 from the first child on whose parent is this node
+    const uint64_t cco = M.off[l - 1], nch = M.n[l - 1];
+    const uint64_t f = A.first[g];
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (f + k < nch) {
+        const uint64_t cc = A.code[cco + f + k];
+        if (T.parent(cc) == code) m |= 1u << (cc & 7);
+      }
+    }
+    A.mask[g] = (uint8_t)m;
+    if (l == D) root_size[jb] = size;
+  }
+  if (l < D) {
+    const uint64_t p = M.off[l + 1] + A.pidx[g];
+    const uint64_t f = A.first[p];
+    A.sib[g] = a0 - A.start[co + f] - (i - f);
+  }
+}
+
 #define SVO_TRY(x)                   \
   do {                               \
     hipError_t _e = (x);             \
@@ -853,22 +1091,25 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
   for (int attempt = 0;; attempt++) {
     SVO_TRY(ensure(c.lvl_code, cap * 8));
     SVO_TRY(ensure(c.lvl_size, cap * 8));
-    SVO_TRY(ensure(c.lvl_mask, cap));
+    SVO_TRY(ensure(c.lvl_mask, cap + 8));  // (+: masks are or-ed as 32-bit words)
     SVO_TRY(ensure(c.lvl_first, cap * 4));
     SVO_TRY(ensure(c.lvl_start, cap * 8));
     SVO_TRY(ensure(c.lvl_pidx, cap * 4));
     SVO_TRY(ensure(c.lvl_sib, cap * 8));
     // meta | per job: root size, root internal size, hazard | single-pass scan status
     const uint64_t meta_words = 2 * 64 + 8 + 3 * (uint64_t)n_jobs;
-    // heights from h_tail up go to the single-block tail kernel
+    // heights from h_tail up go to the single-block tail kernel; on the fast path all
+    // heights above the bricks come from the brick codes at once (no level scans)
+    const bool up_once = fast && H >= 5;
     int h_tail = H;
-    for (int h = h0; h < H; h++)
+    for (int h = h0; h < H && !up_once; h++)
       if (bound[h] <= kTailLevelMax) {
         h_tail = h;
         break;
       }
+    const int h_scan_end = up_once ? h0 : h_tail;  // level scans for h0 .. h_scan_end-1
     uint64_t st_words = 0;
-    for (int h = h0; h < h_tail; h++) st_words += scan_single_words(bound[h]);
+    for (int h = h0; h < h_scan_end; h++) st_words += scan_single_words(bound[h]);
     SVO_TRY(ensure(c.lvl_aux, (meta_words + st_words) * 8));
     if (levels) {
       SVO_TRY(ensure(c.lvl_isize, cap * 8));
@@ -905,9 +1146,37 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
       SVO_TRY(hipGetLastError());
     }
 
+    if (up_once) {
+      const uint32_t ntiles = (uint32_t)((NB + kUpTile - 1) / kUpTile);
+      const uint64_t b_cnt = ((uint64_t)ntiles * kUpLevels * 4 + 255) & ~255ull;
+      const uint64_t b_s4 = ((uint64_t)ntiles * 8 + 255) & ~255ull;
+      const uint64_t b_end = ((uint64_t)kUpLevels * 8 + 255) & ~255ull;
+      SVO_TRY(ensure(c.up_tiles, b_cnt + b_s4 + b_end + (uint64_t)n_jobs * kUpLevels * 8));
+      char* ub = (char*)c.up_tiles.p;
+      uint32_t* tile_cnt = (uint32_t*)ub;
+      uint64_t* tile_s4 = (uint64_t*)(ub + b_cnt);
+      uint64_t* a_end = (uint64_t*)(ub + b_cnt + b_s4);
+      uint64_t* job_end = (uint64_t*)(ub + b_cnt + b_s4 + b_end);
+      ktimer_begin(c, "upper_levels");
+      hipLaunchKernelGGL(k_up_count, dim3(ntiles), dim3(kUpTile), 0, c.stream, A, T,
+                         (const JobDev*)d_jobs, NB, H, tile_cnt, tile_s4);
+      hipLaunchKernelGGL(k_up_scan, dim3(1), dim3(1024), 0, c.stream, A, M, NB, H, ntiles,
+                         tile_cnt, tile_s4, a_end);
+      hipLaunchKernelGGL(k_up_apply, dim3(ntiles), dim3(kUpTile), 0, c.stream, A, M, T,
+                         (const JobDev*)d_jobs, NB, H, (const uint32_t*)tile_cnt,
+                         (const uint64_t*)tile_s4, (const uint64_t*)a_end, job_end);
+      uint64_t nup = 0;
+      for (int h = 4; h <= H; h++) nup += bound[h];
+      hipLaunchKernelGGL(k_up_finish, dim3((unsigned)((nup + 255) / 256)), dim3(256), 0,
+                         c.stream, A, M, T, (const JobDev*)d_jobs, H,
+                         (const uint64_t*)job_end, root_size);
+      ktimer_end(c);
+      SVO_TRY(hipGetLastError());
+    }
+
     // ---- bottom-up: heights 1 .. H
     unsigned long long* status = meta + meta_words;
-    for (int h = h0; h < h_tail; h++) {
+    for (int h = h0; h < h_scan_end; h++) {
       HeadLoad ld{A, M, T, d_jobs, c.codes.as<uint64_t>(), h};
       ParentStore st{A, M, T, d_jobs, c.codes.as<uint64_t>(), c.lvl_leafp.as<uint32_t>(),
                      root_size, root_isize, c.records.as<float>() + 8, h};

@@ -181,6 +181,7 @@ struct Context {
   unsigned long long err_count = 0;  // the sticky failure counter, read by the build
   DevBuf lvl_code, lvl_size, lvl_mask, lvl_first, lvl_start, lvl_aux, lvl_leafp, lvl_pidx;
   DevBuf lvl_sib;  // per arena node: sum of (size - 1) over its preceding siblings
+  DevBuf up_tiles; // upper levels at once: per tile and height counts, job root nodes
   std::vector<JobDev> jobdev_img;  // what jobdev holds on the device (empty: unknown)
   DevBuf lvl_isize, lvl_cache, lvl_pstart;  // -levels: internal counts, averages, data starts
   DevBuf records2;                          // -levels: data records in post-order
