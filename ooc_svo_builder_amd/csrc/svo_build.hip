@@ -531,88 +531,14 @@ __global__ __launch_bounds__(256) void k_emit_leaves(Arena A, LevelMeta M, Tags 
 //   height 2: a word, child mask = its non-zero bytes;
 //   height 3: 8 consecutive words, child mask = its non-zero words;
 //   height 4: the brick, child mask = its non-zero word groups.
-// One wave per brick (lane = word) computes the subtree sizes, and later writes every
-// record below the brick node at its post-order position. Heights >= 4 use the level
-// scans. Sizes: s1 = 1 + popc(byte), s2 = 1 + nnz bytes + popc(word), s3 = 1 + sum s2,
-// s4 = 1 + sum s3 (non-null children only).
-
-// sum over the 8 lanes of a lane's group (all lanes active)
-__device__ __forceinline__ uint32_t group8_sum(uint32_t v) {
-  v += (uint32_t)__shfl_xor((int)v, 1, kWave);
-  v += (uint32_t)__shfl_xor((int)v, 2, kWave);
-  v += (uint32_t)__shfl_xor((int)v, 4, kWave);
-  return v;
-}
-
-struct BrickShape {
-  uint64_t word;
-  uint32_t m2;      // non-zero bytes of this word
-  uint32_t s2;      // subtree size of this word's node (0 if empty)
-  uint32_t mask3;   // non-zero words of this lane's group
-  uint32_t s3;      // subtree size of the group's node (0 if empty)
-  uint32_t mask4;   // non-zero groups
-  uint32_t s4;      // subtree size of the brick node
-};
+// The voxelizer's brick kernels compute each brick node's subtree size and child mask
+// (brick_shape_of, one wave per brick, lane = word) into the brick stream; here one wave
+// per brick writes every record below the brick node at its post-order position, and
+// the heights above come from the brick codes at once (upper levels, above).
 
 __device__ __forceinline__ BrickShape brick_shape(const uint64_t* __restrict__ bwords,
                                                   uint64_t b) {
-  BrickShape B;
-  const int lane = lane_id();
-  B.word = bwords[b * kBrickWords + lane];
-  B.m2 = 0;
-#pragma unroll
-  for (int j = 0; j < 8; j++)
-    if ((B.word >> (8 * j)) & 0xFFull) B.m2 |= 1u << j;
-  B.s2 = B.word ? 1u + (uint32_t)__popc(B.m2) + (uint32_t)__popcll(B.word) : 0u;
-  const uint64_t nz = __ballot(B.word != 0ull);
-  const int g = lane >> 3;
-  B.mask3 = (uint32_t)(nz >> (8 * g)) & 0xFFu;
-  const uint32_t sum2 = group8_sum(B.s2);
-  B.s3 = B.mask3 ? 1u + sum2 : 0u;
-  B.mask4 = 0;
-#pragma unroll
-  for (int q = 0; q < 8; q++)
-    if ((nz >> (8 * q)) & 0xFFull) B.mask4 |= 1u << q;
-  // s4 = 1 + sum of the groups' s3: each group counted once (its first lane)
-  uint32_t v = ((lane & 7) == 0) ? B.s3 : 0u;
-  for (int d = 1; d < kWave; d <<= 1) v += (uint32_t)__shfl_xor((int)v, d, kWave);
-  B.s4 = 1u + v;
-  return B;
-}
-
-// Height-4 level of the arena from the brick stream: code, subtree size, child mask.
-__global__ __launch_bounds__(256) void k_brick_level(Arena A, LevelMeta M, Tags T,
-                                                     const JobDev* __restrict__ jobs,
-                                                     const uint64_t* __restrict__ bwords,
-                                                     const uint64_t* __restrict__ bcode,
-                                                     uint64_t nb,
-                                                     unsigned long long* root_size,
-                                                     unsigned long long* hazard) {
-  const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  if (w >= nb) return;  // uniform per wave
-  const uint64_t c0 = bcode[w];  // (loaded with the words)
-  const BrickShape B = brick_shape(bwords, w);
-  if (lane_id() == kWave - 1) {
-    // the job's last voxel is code G^3 - 2 (the case the job bases flag) when that voxel
-    // is set and G^3 - 1 is not: bits 62 and 63 of the last word of the grid's last brick
-    const uint32_t j = T.job(c0);
-    const uint64_t G = jobs[j].gridsize;
-    if ((c0 & ~T.tag_mask) == G * G * G - kBrickVox && (B.word >> 62) == 1ull) hazard[j] = 1;
-  }
-  if (lane_id() == 0) {
-    const uint64_t code = (c0 & T.tag_mask) | ((c0 & ~T.tag_mask) >> 12);
-    const uint64_t k = M.off[4] + w;
-    if (k >= A.cap) {
-      M.flag[0] = 1;
-      return;
-    }
-    A.code[k] = code;
-    A.size[k] = B.s4;
-    A.mask[k] = (uint8_t)B.mask4;
-    A.first[k] = 0;
-    const uint32_t j = T.job(code);
-    if (jobs[j].depth == 4) root_size[j] = B.s4;
-  }
+  return brick_shape_of(bwords[b * kBrickWords + lane_id()]);
 }
 
 // Every record below each brick node, written record by record: the brick's records form
@@ -780,34 +706,56 @@ __global__ __launch_bounds__(kWave * kEmitWaves) void k_emit_bricks(
 constexpr int kUpTile = 256;
 constexpr int kUpLevels = 17;  // heights 5 .. 21
 
-__device__ __forceinline__ int brick_top(const uint64_t* __restrict__ code4, uint64_t i, Tags T,
-                                         int D) {
+// T(i) from brick i's node code c and the previous brick's p (has_prev = i > 0)
+__device__ __forceinline__ int brick_top(uint64_t c, uint64_t p, bool has_prev, Tags T, int D) {
   if (D <= 4) return 4;
-  const uint64_t c = code4[i];
-  if (i == 0) return D;
-  const uint64_t p = code4[i - 1];
-  if ((c & T.tag_mask) != (p & T.tag_mask)) return D;
+  if (!has_prev || (c & T.tag_mask) != (p & T.tag_mask)) return D;
   const uint64_t x = (c ^ p) & ~T.tag_mask;
   if (!x) return 4;
   return min(D, 4 + (63 - __clzll((long long)x)) / 3);
+}
+// the height-4 node code of a brick (its first voxel's code without the 12 in-brick bits)
+__device__ __forceinline__ uint64_t brick_node_code(uint64_t c0, Tags T) {
+  return (c0 & T.tag_mask) | ((c0 & ~T.tag_mask) >> 12);
 }
 __device__ __forceinline__ int job_depth(const JobDev* jobs, Tags T, uint64_t code) {
   return T.tag_mask ? jobs[T.job(code)].depth : jobs[0].depth;
 }
 
-// per tile: counts of T >= l for l = 5 .. H, and the sum of the brick subtree sizes
+// Per brick: its height-4 node (code, subtree size and child mask from the voxelizer's
+// brick shape) and the hazard case (the job's last voxel is code G^3 - 2: bits 62 and 63
+// of the last word of the grid's last brick); per tile: counts of T >= l for l = 5 .. H,
+// and the sum of the brick subtree sizes.
 __global__ __launch_bounds__(kUpTile) void k_up_count(Arena A, Tags T, const JobDev* jobs,
+                                                      const uint64_t* __restrict__ bwords,
+                                                      const uint64_t* __restrict__ bcode,
+                                                      const uint32_t* __restrict__ bshape,
                                                       uint64_t NB, int H,
                                                       uint32_t* __restrict__ tile_cnt,
-                                                      uint64_t* __restrict__ tile_s4) {
+                                                      uint64_t* __restrict__ tile_s4,
+                                                      unsigned long long* hazard) {
   __shared__ uint32_t wcnt[kUpTile / kWave][kUpLevels];
   __shared__ uint64_t ws4[kUpTile / kWave];
   const uint64_t i = (uint64_t)blockIdx.x * kUpTile + threadIdx.x;
   int t = 4;
   uint64_t s4 = 0;
   if (i < NB) {
-    t = brick_top(A.code, i, T, job_depth(jobs, T, A.code[i]));
-    s4 = A.size[i];
+    const uint64_t c0 = bcode[i];
+    const uint64_t code = brick_node_code(c0, T);
+    const uint64_t prev = i ? brick_node_code(bcode[i - 1], T) : 0ull;
+    const uint32_t sh = bshape[i];
+    s4 = sh & 0xFFFFu;
+    A.code[i] = code;
+    A.size[i] = s4;
+    A.mask[i] = (uint8_t)(sh >> 16);
+    A.first[i] = 0;
+    const uint32_t jb = T.job(c0);
+    const JobDev& J = jobs[jb];
+    const uint64_t G = J.gridsize;
+    if ((c0 & ~T.tag_mask) == G * G * G - kBrickVox &&
+        (bwords[i * kBrickWords + kWave - 1] >> 62) == 1ull)
+      hazard[jb] = 1;
+    t = brick_top(code, prev, i > 0, T, J.depth);
   }
   const int w = threadIdx.x / kWave;
 #pragma unroll
@@ -899,7 +847,7 @@ __global__ __launch_bounds__(kUpTile) void k_up_apply(Arena A, LevelMeta M, Tags
     s4 = A.size[i];
     if (i) cp = A.code[i - 1];
     D = job_depth(jobs, T, c);
-    t = brick_top(A.code, i, T, D);
+    t = brick_top(c, cp, i > 0, T, D);
   }
   const int w = threadIdx.x / kWave;
   const uint64_t le = (2ull << lane_id()) - 1ull;  // lanes <= this one
@@ -993,6 +941,7 @@ __global__ __launch_bounds__(256) void k_up_finish(Arena A, LevelMeta M, Tags T,
     A.mask[g] = (uint8_t)m;
     if (l == D) root_size[jb] = size;
   }
+  if (l == 4 && D == 4) root_size[jb] = A.size[g];  // a 16^3 job: the brick is the root
   if (l < D) {
     const uint64_t p = M.off[l + 1] + A.pidx[g];
     const uint64_t f = A.first[p];
@@ -1100,7 +1049,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     const uint64_t meta_words = 2 * 64 + 8 + 3 * (uint64_t)n_jobs;
     // heights from h_tail up go to the single-block tail kernel; on the fast path all
     // heights above the bricks come from the brick codes at once (no level scans)
-    const bool up_once = fast && H >= 5;
+    const bool up_once = fast;
     int h_tail = H;
     for (int h = h0; h < H && !up_once; h++)
       if (bound[h] <= kTailLevelMax) {
@@ -1137,14 +1086,6 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     hipLaunchKernelGGL(k_build_init, dim3(1), dim3(256), 0, c.stream, meta,
                        meta_words + st_words, V, NB, fast ? 1 : 0);
     SVO_TRY(hipGetLastError());
-    if (fast) {
-      ktimer_begin(c, "brick_level");
-      hipLaunchKernelGGL(k_brick_level, dim3((unsigned)((NB * kWave + 255) / 256)), dim3(256), 0,
-                         c.stream, A, M, T, (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),
-                         c.bcode.as<uint64_t>(), NB, root_size, brick_hazard);
-      ktimer_end(c);
-      SVO_TRY(hipGetLastError());
-    }
 
     if (up_once) {
       const uint32_t ntiles = (uint32_t)((NB + kUpTile - 1) / kUpTile);
@@ -1159,7 +1100,9 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
       uint64_t* job_end = (uint64_t*)(ub + b_cnt + b_s4 + b_end);
       ktimer_begin(c, "upper_levels");
       hipLaunchKernelGGL(k_up_count, dim3(ntiles), dim3(kUpTile), 0, c.stream, A, T,
-                         (const JobDev*)d_jobs, NB, H, tile_cnt, tile_s4);
+                         (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),
+                         c.bcode.as<uint64_t>(), c.bshape.as<uint32_t>(), NB, H, tile_cnt,
+                         tile_s4, brick_hazard);
       hipLaunchKernelGGL(k_up_scan, dim3(1), dim3(1024), 0, c.stream, A, M, NB, H, ntiles,
                          tile_cnt, tile_s4, a_end);
       hipLaunchKernelGGL(k_up_apply, dim3(ntiles), dim3(kUpTile), 0, c.stream, A, M, T,

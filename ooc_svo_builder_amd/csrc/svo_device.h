@@ -127,4 +127,58 @@ __device__ __forceinline__ T block_incl_scan(T v, T* smem, T& total) {
   return before + inc;
 }
 
+// ------------------------------------------------------------------------------------
+// Shape of the subtree of one occupied 16^3 brick (an aligned subtree of height 4) from
+// its 64 occupancy words, lane l holding word l (voxels 64l..64l+63 in Morton order):
+// height 1 = a byte of a word, 2 = a word, 3 = 8 consecutive words, 4 = the brick.
+// Sizes count the subtree's nodes: s2 = 1 + non-zero bytes + voxels, s3 = 1 + sum s2,
+// s4 = 1 + sum s3 (non-null children only). All lanes of the wave must call it.
+// ------------------------------------------------------------------------------------
+// sum over the 8 lanes of a lane's group
+__device__ __forceinline__ uint32_t group8_sum(uint32_t v) {
+  v += (uint32_t)__shfl_xor((int)v, 1, kWave);
+  v += (uint32_t)__shfl_xor((int)v, 2, kWave);
+  v += (uint32_t)__shfl_xor((int)v, 4, kWave);
+  return v;
+}
+
+struct BrickShape {
+  uint64_t word;
+  uint32_t m2;      // non-zero bytes of this word
+  uint32_t s2;      // subtree size of this word's node (0 if empty)
+  uint32_t mask3;   // non-zero words of this lane's group
+  uint32_t s3;      // subtree size of the group's node (0 if empty)
+  uint32_t mask4;   // non-zero groups
+  uint32_t s4;      // subtree size of the brick node
+};
+
+__device__ __forceinline__ BrickShape brick_shape_of(uint64_t word) {
+  BrickShape B;
+  const int lane = lane_id();
+  B.word = word;
+  B.m2 = 0;
+#pragma unroll
+  for (int j = 0; j < 8; j++)
+    if ((B.word >> (8 * j)) & 0xFFull) B.m2 |= 1u << j;
+  B.s2 = B.word ? 1u + (uint32_t)__popc(B.m2) + (uint32_t)__popcll(B.word) : 0u;
+  const uint64_t nz = __ballot(B.word != 0ull);
+  const int g = lane >> 3;
+  B.mask3 = (uint32_t)(nz >> (8 * g)) & 0xFFu;
+  const uint32_t sum2 = group8_sum(B.s2);
+  B.s3 = B.mask3 ? 1u + sum2 : 0u;
+  B.mask4 = 0;
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+    if ((nz >> (8 * q)) & 0xFFull) B.mask4 |= 1u << q;
+  // s4 = 1 + sum of the groups' s3: each group counted once (its first lane)
+  uint32_t v = ((lane & 7) == 0) ? B.s3 : 0u;
+  for (int d = 1; d < kWave; d <<= 1) v += (uint32_t)__shfl_xor((int)v, d, kWave);
+  B.s4 = 1u + v;
+  return B;
+}
+// s4 (below 2^13) and the brick's child mask in one word
+__device__ __forceinline__ uint32_t brick_shape_word(const BrickShape& B) {
+  return B.s4 | (B.mask4 << 16);
+}
+
 }  // namespace svo

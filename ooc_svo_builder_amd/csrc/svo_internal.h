@@ -162,6 +162,7 @@ struct Context {
   // occupancy words, its first voxel's code (tagged) and its first voxel's stream index.
   // The build derives the four lowest tree levels from it.
   DevBuf bwords, bcode, bvoff;
+  DevBuf bshape;  // per brick: subtree size and child mask of its node (brick_shape_word)
   uint64_t n_bricks = 0;
   bool bricks_ok = true;  // every voxel of the stream came with its brick, bricks unique
   // Geometry-only passes leave the codes of their voxels unwritten (the build reads the

@@ -689,6 +689,7 @@ __device__ __forceinline__ void brick_count_body(const uint32_t* __restrict__ oc
                                                  ull* __restrict__ pool,
                                                  uint64_t* __restrict__ bcount,
                                                  uint64_t* __restrict__ bcode,
+                                                 uint32_t* __restrict__ bshape,
                                                  uint32_t* __restrict__ table) {
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
     const uint64_t ti = occ[wv];
@@ -748,6 +749,10 @@ __device__ __forceinline__ void brick_count_body(const uint32_t* __restrict__ oc
       }
     }
     pool[wv * kBrickWords + lane_id()] = m;
+    if (bcode) {  // the brick node's subtree size and child mask, for the build
+      const BrickShape B = brick_shape_of(m);
+      if (lane_id() == 0) bshape[wv] = brick_shape_word(B);
+    }
     const uint32_t c = (uint32_t)__popcll(m);
     const uint32_t tot = wave_incl_scan<uint32_t>(c);
     // voxel count, plus a flag in bit kNonEmptyShift: one scan gives every brick its
@@ -765,13 +770,16 @@ __global__ __launch_bounds__(256) void k_brick_count(const uint32_t* __restrict_
                                                      ull* __restrict__ pool,
                                                      uint64_t* __restrict__ bcount,
                                                      uint64_t* __restrict__ bcode,
+                                                     uint32_t* __restrict__ bshape,
                                                      uint32_t* __restrict__ table) {
   // more flagged bricks than slots (touch path): the pass is redone with a larger pool
   const uint64_t n_occ = min((uint64_t)ctr[kCtrNumOcc], cap);
   if (n_units == 1)
-    brick_count_body<true>(occ, n_occ, units, ubase, n_units, pool8, pool, bcount, bcode, table);
+    brick_count_body<true>(occ, n_occ, units, ubase, n_units, pool8, pool, bcount, bcode,
+                           bshape, table);
   else
-    brick_count_body<false>(occ, n_occ, units, ubase, n_units, pool8, pool, bcount, bcode, table);
+    brick_count_body<false>(occ, n_occ, units, ubase, n_units, pool8, pool, bcount, bcode,
+                            bshape, table);
 }
 
 struct CountStore {
@@ -817,6 +825,7 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
                                                     uint64_t s0, uint64_t* __restrict__ bwords,
                                                     uint64_t* __restrict__ bcode,
                                                     uint64_t* __restrict__ bvoff,
+                                                    uint32_t* __restrict__ bshape,
                                                     uint32_t* __restrict__ winner) {
   const uint64_t n_occ = ctr[kCtrNumOcc];
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
@@ -831,9 +840,11 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
     if (__ballot(word != 0ull)) {
       const uint64_t sb = bo >> kNonEmptyShift;
       bwords[sb * kBrickWords + lane_id()] = word;
+      const BrickShape B = brick_shape_of(word);
       if (lane_id() == 0) {
         bcode[sb] = b0 | tag;
         bvoff[sb] = s0 + (bo & kVoxMask);
+        bshape[sb] = brick_shape_word(B);
       }
     }
     if constexpr (!kCodes) continue;
@@ -1418,6 +1429,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
                         c.stream));
     SVO_TRY(ensure_keep(c.bcode, (nb0 + ntab) * 8, nb0 * 8, c.stream));
     SVO_TRY(ensure_keep(c.bvoff, (nb0 + ntab) * 8, nb0 * 8, c.stream));
+    SVO_TRY(ensure_keep(c.bshape, (nb0 + ntab) * 4, nb0 * 4, c.stream));
   }
   ull* pool_words = dense ? c.bwords.as<ull>() + nb0 * kBrickWords : c.pool.as<ull>();
   SVO_TRY(ensure(c.large_unit, ntri * 4));
@@ -1523,6 +1535,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
                        c.occ_list.as<uint32_t>(), ctr, bc_bound, d_units, d_ubase, nu,
                        c.pool8.as<uint8_t>(), pool_words, c.bcount.as<uint64_t>(),
                        dense ? c.bcode.as<uint64_t>() + nb0 : (uint64_t*)nullptr,
+                       dense ? c.bshape.as<uint32_t>() + nb0 : (uint32_t*)nullptr,
                        dense ? c.table.as<uint32_t>() : (uint32_t*)nullptr);
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
@@ -1650,6 +1663,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
                           c.stream));
       SVO_TRY(ensure_keep(c.bcode, (nb0 + n_occ) * 8, nb0 * 8, c.stream));
       SVO_TRY(ensure_keep(c.bvoff, (nb0 + n_occ) * 8, nb0 * 8, c.stream));
+      SVO_TRY(ensure_keep(c.bshape, (nb0 + n_occ) * 4, nb0 * 4, c.stream));
     }
     // a dense pass has its brick stream entries already (k_brick_count); the emit pass
     // writes codes and winners (rewriting those entries with the same values)
@@ -1661,7 +1675,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
                        c.bcount.as<uint64_t>(), c.codes.as<uint64_t>() + s0,
                        winner_walk ? c.wprefix.as<uint32_t>() : (uint32_t*)nullptr, s0,
                        c.bwords.as<uint64_t>() + nb0 * kBrickWords, c.bcode.as<uint64_t>() + nb0,
-                       c.bvoff.as<uint64_t>() + nb0,
+                       c.bvoff.as<uint64_t>() + nb0, c.bshape.as<uint32_t>() + nb0,
                        any_payload ? c.winner.as<uint32_t>() : (uint32_t*)nullptr);
       ktimer_end(c);
       SVO_TRY(hipGetLastError());
