@@ -30,9 +30,6 @@ enum Counter : int {
   kCtrNumVox = 4,     // voxels of the current pass
   kCtrLevelN = 5,     // node count of the level being built
   kCtrCols = 6,       // columns of the large tier
-  kCtrQueueA = 7,     // tile queues of the persistent voxelize launches
-  kCtrQueueB = 8,
-  kCtrQueueC = 9,
   kCtrError = 15,     // sticky: internal consistency failures (cleared per job)
   kCtrPartTotal = 16, // partition list length
   kNumCounters = 32

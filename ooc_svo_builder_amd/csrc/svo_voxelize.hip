@@ -522,6 +522,9 @@ __device__ __forceinline__ uint32_t upper_index(const T* __restrict__ a, uint32_
 constexpr int kTile = 256;      // threads per block
 constexpr int kTileTris = 192;  // triangles per tile: 35 KB of walk constants in LDS, so
                                 // that four blocks fit a CU
+constexpr uint32_t kQueues = 8;        // tile queues per persistent launch
+constexpr uint32_t kQueueStride = 16;  // words between their counters (own cache lines)
+constexpr uint32_t kQueueWords = kQueues * kQueueStride;
 constexpr int kSkipDense = 1;   // units voxelized in a dense grid (touch pass)
 constexpr int kSkipBinary = 2;  // geometry-only units (payload winner pass)
 
@@ -544,8 +547,25 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
   ull* ctr = A.ctr;
   const uint32_t total_tiles = tile_off[n_units];
   const uint32_t tid = threadIdx.x;
+  // kQueues interleaved tile classes (tile % kQueues), each with its own counter: one
+  // counter's atomics serialize (~14 ns each), which bounds a single queue at ~70 M
+  // tiles/s. A block starts on class blockIdx % kQueues and moves on when it is empty.
+  uint32_t cls = blockIdx.x % kQueues, tried = 0;
   for (;;) {
-    if (tid == 0) s_tile = (uint32_t)atomicAdd(queue, 1ull);
+    if (tid == 0) {
+      uint32_t tl = total_tiles;
+      while (tried < kQueues) {
+        const uint32_t k = (uint32_t)atomicAdd(queue + kQueueStride * cls, 1ull);
+        const uint32_t cand = cls + kQueues * k;
+        if (cand < total_tiles) {
+          tl = cand;
+          break;
+        }
+        cls = (cls + 1) % kQueues;
+        tried++;
+      }
+      s_tile = tl;
+    }
     __syncthreads();
     const uint32_t tile = s_tile;
     if (tile >= total_tiles) break;
@@ -1483,6 +1503,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   ull* st_col = nullptr;
   ull* st_occ = nullptr;
   ull* st_bc = nullptr;
+  ull* queues = nullptr;  // 3 x kQueueWords: the tile queues of the mark/touch, second and winner passes
   auto occ_scan = [&]() {
     return scan_single<uint32_t>(
         OccLoad{c.table.as<uint32_t>()},
@@ -1497,23 +1518,25 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     const bool bc_single = bc_bound <= (3ull << 20);
     const uint64_t w_bc = bc_single ? scan_single_words(bc_bound) : 0;
     SVO_TRY(ensure(c.bcount, bc_bound * 8 + 8));
-    SVO_TRY(ensure(c.scan_status, (w_col + w_occ + w_bc) * 8));
+    // status words of the scans, then the tile queues of the three tile passes
+    SVO_TRY(ensure(c.scan_status, (w_col + w_occ + w_bc + 3 * kQueueWords) * 8));
     st_col = c.scan_status.as<ull>();
     st_occ = st_col + w_col;
     st_bc = st_occ + w_occ;
-    {  // pass counters and scan status words
-      const uint64_t nz = kCtrPassClear + w_col + w_occ + w_bc;
-      hipLaunchKernelGGL(k_zero2_u64, dim3(fill_grid(nz)), dim3(256), 0, c.stream,
-                         (uint64_t*)ctr, (uint64_t)kCtrPassClear, c.scan_status.as<uint64_t>(),
-                         w_col + w_occ + w_bc);
+    queues = st_bc + w_bc;
+    {  // pass counters, scan status words and tile queues
+      const uint64_t ns = w_col + w_occ + w_bc + 3 * kQueueWords;
+      hipLaunchKernelGGL(k_zero2_u64, dim3(fill_grid(kCtrPassClear + ns)), dim3(256), 0,
+                         c.stream, (uint64_t*)ctr, (uint64_t)kCtrPassClear,
+                         c.scan_status.as<uint64_t>(), ns);
       SVO_TRY(hipGetLastError());
     }
     // first pass: dense units mark directly, touch units flag their bricks; both collect
     // the large triangles
     if (dense)
-      SVO_TRY(tile_pass(MarkAction{}, "vox_small_mark", ctr + kCtrQueueA, 1, 0));
+      SVO_TRY(tile_pass(MarkAction{}, "vox_small_mark", queues, 1, 0));
     else
-      SVO_TRY(tile_pass(TouchAction{}, "vox_small_touch", ctr + kCtrQueueA, 1, 0));
+      SVO_TRY(tile_pass(TouchAction{}, "vox_small_touch", queues, 1, 0));
     SVO_TRY((scan_single<uint64_t>(ColLoad{l_cols},
                                    CountStore{l_off, ctr + kCtrCols, ctr + kCtrLargeCount},
                                    ntri, st_col, ctr + kCtrError, c.stream,
@@ -1526,7 +1549,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
       SVO_TRY(large_pass(TouchAction{}, "vox_large_touch", 0));
       SVO_TRY(occ_scan());
       // mark pass: byte stores into the slots
-      SVO_TRY(tile_pass(MarkAction{}, "vox_small_mark", ctr + kCtrQueueB, 0, 0));
+      SVO_TRY(tile_pass(MarkAction{}, "vox_small_mark", queues + kQueueWords, 0, 0));
       SVO_TRY(large_pass(MarkAction{}, "vox_large_mark", 0));
     }
     // per-brick voxel counts (bytes packed to bits, bytes cleared) and their offsets
@@ -1686,7 +1709,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     if (any_payload) {
       if (winner_walk) {
         SVO_TRY(fill_u32(c.winner.as<uint32_t>(), 0xFFFFFFFFu, nvox, c.stream));
-        SVO_TRY(tile_pass(WinnerAction{}, "vox_small_winner", ctr + kCtrQueueC, 0, kSkipBinary));
+        SVO_TRY(tile_pass(WinnerAction{}, "vox_small_winner", queues + 2 * kQueueWords, 0, kSkipBinary));
         if (n_large) SVO_TRY(large_pass(WinnerAction{}, "vox_large_winner", kSkipBinary));
       }
       // records: each job's first record is its null record
