@@ -829,9 +829,9 @@ struct ColLoad {
   __device__ uint64_t operator()(uint64_t i) const { return a[i]; }
 };
 
-// The brick stream entry of every non-empty flagged brick and, with kCodes, the sorted
-// codes of its voxels, tagged with the unit's job in bits kJobShift+ (and the payload
-// winners of dense payload units).
+// The brick stream entry of every non-empty flagged brick and, with kCodes, for payload
+// units the sorted codes of its voxels, tagged with the unit's job in bits kJobShift+
+// (and the payload winners of dense payload units).
 template <bool kCodes>
 __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__ occ,
                                                     const ull* __restrict__ ctr,
@@ -868,6 +868,7 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
       }
     }
     if constexpr (!kCodes) continue;
+    if (units[u].stride == 9) continue;  // geometry-only unit: codes written on demand
     const uint32_t c = (uint32_t)__popcll(word);
     const uint32_t inc = wave_incl_scan<uint32_t>(c);
     uint64_t pos = (bo & kVoxMask) + (inc - c);
@@ -946,16 +947,22 @@ __global__ void k_unit_voxels(const uint32_t* __restrict__ occ, const ull* __res
 
 // ---------------------------------------------------------------- payload records
 // BarycentricCoords.h:4-32 for the winning triangle; colour modes of main.cpp:373-382.
-// Voxel i of the pass (stream index s0 + i) belongs to the job in its code's tag; its
-// record goes to dbase[job] + 1 + (s0 + i - voff[job]). Voxels of geometry-only jobs
-// have no record here.
+// Voxel i of the pass (stream index s0 + i) belongs to the unit whose voxel range holds
+// it, and to that unit's job; its record goes to dbase[job] + 1 + (s0 + i - voff[job]).
+// Voxels of geometry-only jobs have no record here.
 __global__ __launch_bounds__(256) void k_payload_records(
-    const JobDev* __restrict__ jobs, const uint64_t* __restrict__ codes, uint64_t s0,
-    const uint32_t* __restrict__ winner, uint64_t nvox, float* __restrict__ out, ull* ctr) {
+    const JobDev* __restrict__ jobs, const VoxParams* __restrict__ units,
+    const uint64_t* __restrict__ uvoff, uint32_t n_units, const uint64_t* __restrict__ codes,
+    uint64_t s0, const uint32_t* __restrict__ winner, uint64_t nvox, float* __restrict__ out,
+    ull* ctr) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nvox) return;
+  // the unit holding voxel i of the pass (uvoff: first voxel of each unit); only payload
+  // units have their codes written
+  const uint32_t u = upper_index<uint64_t>(uvoff, n_units + 1, i);
+  if (units[u].stride == 9) return;
   const uint64_t tagged = codes[s0 + i];
-  const JobDev& J = jobs[tagged >> kJobShift];
+  const JobDev& J = jobs[units[u].job];
   if (!J.payload) return;
   const uint64_t code = tagged & kCodeMask;
   const uint32_t t = winner[i];
@@ -1704,7 +1711,9 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
       SVO_TRY(hipGetLastError());
     }
     c.n_bricks = nb0 + n_nonempty;
-    if (!codes_now) c.codes_lazy = true;
+    bool any_binary = false;
+    for (uint32_t u = 0; u < nu; u++) any_binary |= units[u].stride == 9;
+    if (!codes_now || any_binary) c.codes_lazy = true;
 
     if (any_payload) {
       if (winner_walk) {
@@ -1717,7 +1726,8 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
       SVO_TRY(ensure_keep(c.records, n_records * 32, keep, c.stream));
       ktimer_begin(c, "payload_records");
       hipLaunchKernelGGL(k_payload_records, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
-                         c.stream, c.jobdev.as<JobDev>(), c.codes.as<uint64_t>(), s0,
+                         c.stream, c.jobdev.as<JobDev>(), (const VoxParams*)d_units,
+                         (const uint64_t*)d_uvoff, nu, c.codes.as<uint64_t>(), s0,
                          c.winner.as<uint32_t>(), nvox, c.records.as<float>(), ctr);
       ktimer_end(c);
       SVO_TRY(hipGetLastError());
