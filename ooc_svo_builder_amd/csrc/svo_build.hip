@@ -174,7 +174,7 @@ struct ParentStore {
   unsigned long long* root_isize; // per job: internal nodes in its tree (-levels)
   const float* leaf_rec;          // -levels, payload: voxel i's record at leaf_rec + 8 i
   int h;
-  __device__ void operator()(uint64_t i, uint32_t ex, uint32_t head) const {
+  __device__ void operator()(uint64_t i, uint32_t ex, uint32_t head, bool) const {
     const bool leaf = (h == 0);
     if (leaf)
       leaf_parent[i] = ex + head - 1;
@@ -355,7 +355,7 @@ __global__ __launch_bounds__(kTailBlock) void k_levels_tail(
       const uint32_t v = i < n ? ld(i) : 0u;
       uint32_t tot;
       const uint32_t inc = block_incl_scan<uint32_t, kTailBlock>(v, wsum32, tot);
-      if (i < n) st(i, run + inc - v, v);
+      if (i < n) st(i, run + inc - v, v, i == n - 1);
       run += tot;
     }
   }

@@ -1,7 +1,8 @@
 // Device-wide exclusive scan (reduce-then-scan), parameterised by load/store functors so
 // that producers and consumers fuse into the two passes: a Load functor yields element
-// i's value, a Store functor receives (i, exclusive prefix, value). Tiles are 256
-// threads x 8 consecutive items per thread for large inputs, 1 item below 2M elements.
+// i's value, a Store functor receives (i, exclusive prefix, value, i is the last element:
+// the count is known to the kernel, so a store need not read it back). Tiles are 256
+// threads x 8 items per thread (wave-striped) for large inputs, 1 item below 2M elements.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,6 +11,58 @@
 #include "svo_device.h"
 
 namespace svo {
+
+// Tiles are laid out wave-striped: wave w of a block owns the tile's elements
+// [seg, seg + 64 ITEMS), seg = tile start + 64 ITEMS w, and its lane l takes elements
+// seg + 64 k + l (k < ITEMS), so every load and store instruction of the wave touches 64
+// consecutive elements. Within the segment the order is k-major, so a lane's exclusive
+// prefix of item k is (rows before k) + (lanes before l in row k).
+template <typename T>
+__device__ __forceinline__ T wave_last(T v);
+template <>
+__device__ __forceinline__ uint32_t wave_last<uint32_t>(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, kWave - 1);
+}
+template <>
+__device__ __forceinline__ uint64_t wave_last<uint64_t>(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, kWave - 1);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), kWave - 1);
+  return ((uint64_t)hi << 32) | lo;
+}
+// First element of the lane's striped items (item k is at the result + 64 k).
+template <int ITEMS>
+__device__ __forceinline__ uint64_t striped_first(uint64_t tile, int block) {
+  const uint64_t w = threadIdx.x / kWave;
+  return (tile * (uint64_t)block + w * kWave) * ITEMS + lane_id();
+}
+// Exclusive prefixes (within the block's tile) of the lane's striped items; 'total'
+// receives the tile sum. smem holds BLOCK/64 elements; all threads must call it. One
+// barrier: smem is not read again after it.
+template <typename T, int BLOCK, int ITEMS>
+__device__ __forceinline__ void tile_excl_striped(const T (&v)[ITEMS], T (&ex)[ITEMS], T* smem,
+                                                  T& total) {
+  constexpr int NW = BLOCK / kWave;
+  const int w = threadIdx.x / kWave;
+  T run = 0;  // the wave's rows before row k
+#pragma unroll
+  for (int k = 0; k < ITEMS; k++) {
+    const T inc = wave_incl_scan<T>(v[k]);
+    ex[k] = run + inc - v[k];
+    run += wave_last<T>(inc);
+  }
+  if (lane_id() == 0) smem[w] = run;
+  __syncthreads();
+  T before = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < NW; i++) {
+    const T s = smem[i];
+    if (i < w) before += s;
+    tot += s;
+  }
+#pragma unroll
+  for (int k = 0; k < ITEMS; k++) ex[k] += before;
+  total = tot;
+}
 
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;  // items per thread for large inputs
@@ -20,22 +73,21 @@ constexpr uint64_t kScanSmall = 1ull << 21;
 
 // n_dev (optional) holds the actual element count on the device; n is then an upper
 // bound used only to size the grid, and elements at or beyond *n_dev are treated as
-// absent (loads give 0, stores are skipped). Each thread owns ITEMS consecutive
-// elements: all its loads are issued before the single block-wide scan.
+// absent (loads give 0, stores are skipped). Each thread owns ITEMS wave-striped
+// elements: all its loads are issued before the block-wide scan.
 template <typename T, typename Load, int ITEMS>
 __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(Load ld, uint64_t n,
                                                             const unsigned long long* n_dev,
                                                             T* sums) {
   __shared__ T sm[kScanBlock / kWave];
   if (n_dev) n = min(n, (uint64_t)*n_dev);
-  const uint64_t tile = (uint64_t)kScanBlock * ITEMS;
-  const uint64_t base = (uint64_t)blockIdx.x * tile + (uint64_t)threadIdx.x * ITEMS;
-  T v[ITEMS];
-#pragma unroll
-  for (int k = 0; k < ITEMS; k++) v[k] = (base + k < n) ? ld(base + k) : (T)0;
+  const uint64_t first = striped_first<ITEMS>(blockIdx.x, kScanBlock);
   T acc = 0;
 #pragma unroll
-  for (int k = 0; k < ITEMS; k++) acc += v[k];
+  for (int k = 0; k < ITEMS; k++) {
+    const uint64_t i = first + (uint64_t)kWave * k;
+    if (i < n) acc += ld(i);
+  }
   T tot;
   block_incl_scan<T, kScanBlock>(acc, sm, tot);
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
@@ -49,20 +101,20 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_apply(Load ld, Store st, ui
   if (n_dev) n = min(n, (uint64_t)*n_dev);
   const uint64_t tile = (uint64_t)kScanBlock * ITEMS;
   if ((uint64_t)blockIdx.x * tile >= n) return;  // uniform per block
-  const uint64_t base = (uint64_t)blockIdx.x * tile + (uint64_t)threadIdx.x * ITEMS;
-  T v[ITEMS];
-#pragma unroll
-  for (int k = 0; k < ITEMS; k++) v[k] = (base + k < n) ? ld(base + k) : (T)0;
-  T acc = 0;
-#pragma unroll
-  for (int k = 0; k < ITEMS; k++) acc += v[k];
-  T tot;
-  const T inc = block_incl_scan<T, kScanBlock>(acc, sm, tot);
-  T run = (prefix ? prefix[blockIdx.x] : (T)0) + inc - acc;
+  const uint64_t first = striped_first<ITEMS>(blockIdx.x, kScanBlock);
+  T v[ITEMS], ex[ITEMS];
 #pragma unroll
   for (int k = 0; k < ITEMS; k++) {
-    if (base + k < n) st(base + k, run, v[k]);
-    run += v[k];
+    const uint64_t i = first + (uint64_t)kWave * k;
+    v[k] = (i < n) ? ld(i) : (T)0;
+  }
+  T tot;
+  tile_excl_striped<T, kScanBlock, ITEMS>(v, ex, sm, tot);
+  const T p = prefix ? prefix[blockIdx.x] : (T)0;
+#pragma unroll
+  for (int k = 0; k < ITEMS; k++) {
+    const uint64_t i = first + (uint64_t)kWave * k;
+    if (i < n) st(i, p + ex[k], v[k], i == n - 1);
   }
 }
 
@@ -74,7 +126,7 @@ struct ArrayLoad {
 template <typename T>
 struct ArrayStoreExcl {
   T* a;
-  __device__ void operator()(uint64_t i, T ex, T) const { a[i] = ex; }
+  __device__ void operator()(uint64_t i, T ex, T, bool) const { a[i] = ex; }
 };
 
 // Number of T elements of scratch that scan_exclusive needs for n elements (sized for
@@ -150,6 +202,7 @@ constexpr unsigned long long kStAgg = 1ull << 62;
 constexpr unsigned long long kStIncl = 2ull << 62;
 constexpr unsigned long long kStVal = (1ull << 62) - 1;
 constexpr uint32_t kScanSpinLimit = 1u << 22;
+constexpr int kLookWindows = 4;  // 64-tile windows read per round of the walk back
 
 __device__ __forceinline__ void status_put(unsigned long long* p, unsigned long long v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -169,54 +222,67 @@ __global__ __launch_bounds__(kSingleBlock) void k_scan_single(Load ld, Store st,
   unsigned long long* tiles = status + 1;
   if (n_dev) n = min(n, (uint64_t)*n_dev);
   if ((uint64_t)tile * kSingleBlock * ITEMS >= n) return;  // uniform per block
-  const uint64_t base = ((uint64_t)tile * kSingleBlock + threadIdx.x) * ITEMS;
-  T v[ITEMS];
+  const uint64_t first = striped_first<ITEMS>(tile, kSingleBlock);
+  T v[ITEMS], ex[ITEMS];
 #pragma unroll
-  for (int k = 0; k < ITEMS; k++) v[k] = (base + k < n) ? ld(base + k) : (T)0;
-  T acc = 0;
-#pragma unroll
-  for (int k = 0; k < ITEMS; k++) acc += v[k];
+  for (int k = 0; k < ITEMS; k++) {
+    const uint64_t i = first + (uint64_t)kWave * k;
+    v[k] = (i < n) ? ld(i) : (T)0;
+  }
   T tot;
-  const T inc = block_incl_scan<T, kSingleBlock>(acc, sm, tot);
+  tile_excl_striped<T, kSingleBlock, ITEMS>(v, ex, sm, tot);
   if (threadIdx.x < kWave) {  // wave 0: publish, look back
     T prefix = 0;
     if (tile == 0) {
       if (threadIdx.x == 0) status_put(&tiles[0], kStIncl | (unsigned long long)tot);
     } else {
       if (threadIdx.x == 0) status_put(&tiles[tile], kStAgg | (unsigned long long)tot);
-      int64_t j = (int64_t)tile - 1;  // lane l looks at tile j - l
+      // lane l looks at tiles j - l - 64 r, r < kLookWindows: the windows' status words
+      // are read together, so a walk back over up to 256 tiles costs one round trip
+      int64_t j = (int64_t)tile - 1;
       uint32_t polls = 0;
-      for (;;) {
-        const int64_t idx = j - (int64_t)lane_id();
-        unsigned long long s = (idx >= 0) ? status_get(&tiles[idx]) : kStIncl;
-        for (;;) {
-          const bool pend = (s >> 62) == 0ull;
-          if (!__ballot(pend)) break;
-          if (++polls > kScanSpinLimit) {
-            if (lane_id() == 0) atomicAdd(err, 1ull);
-            if (pend) s = kStIncl;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          if (pend) s = status_get(&tiles[idx]);
+      bool found = false;
+      while (!found) {
+        unsigned long long s[kLookWindows];
+#pragma unroll
+        for (int r = 0; r < kLookWindows; r++) {
+          const int64_t idx = j - (int64_t)lane_id() - (int64_t)kWave * r;
+          s[r] = (idx >= 0) ? status_get(&tiles[idx]) : kStIncl;
         }
-        const unsigned long long incl = __ballot((s >> 62) == 2ull);
-        const int first = incl ? __ffsll((long long)incl) - 1 : kWave;
-        const T val = (lane_id() <= first) ? (T)(s & kStVal) : (T)0;
-        prefix += shfl_t<T>(wave_incl_scan<T>(val), kWave - 1);
-        if (incl) break;
-        j -= kWave;
+#pragma unroll
+        for (int r = 0; r < kLookWindows; r++) {
+          if (found) break;
+          const int64_t idx = j - (int64_t)lane_id() - (int64_t)kWave * r;
+          unsigned long long w = s[r];
+          for (;;) {
+            const bool pend = (w >> 62) == 0ull;
+            if (!__ballot(pend)) break;
+            if (++polls > kScanSpinLimit) {
+              if (lane_id() == 0) atomicAdd(err, 1ull);
+              if (pend) w = kStIncl;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            if (pend) w = status_get(&tiles[idx]);
+          }
+          const unsigned long long incl = __ballot((w >> 62) == 2ull);
+          const int first = incl ? __ffsll((long long)incl) - 1 : kWave;
+          const T val = (lane_id() <= first) ? (T)(w & kStVal) : (T)0;
+          prefix += shfl_t<T>(wave_incl_scan<T>(val), kWave - 1);
+          found = incl != 0ull;
+        }
+        j -= (int64_t)kWave * kLookWindows;
       }
       if (threadIdx.x == 0) status_put(&tiles[tile], kStIncl | (unsigned long long)(prefix + tot));
     }
     if (threadIdx.x == 0) s_prefix = prefix;
   }
   __syncthreads();
-  T run = s_prefix + inc - acc;
+  const T p = s_prefix;
 #pragma unroll
   for (int k = 0; k < ITEMS; k++) {
-    if (base + k < n) st(base + k, run, v[k]);
-    run += v[k];
+    const uint64_t i = first + (uint64_t)kWave * k;
+    if (i < n) st(i, p + ex[k], v[k], i == n - 1);
   }
 }
 

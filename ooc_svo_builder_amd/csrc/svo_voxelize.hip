@@ -342,7 +342,9 @@ __device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, i
       if ((K.f[F_BV + k] * pc + KB[k]) + K.f[F_BD + k] < 0.0f) ok = false;
     }
     if (!ok) continue;
-    act.add(cbase | mc, (uint32_t)K.i[I_TRI]);
+    // cbase and mc occupy disjoint bits: the sum is their union, and an action's
+    // loop-invariant offset from cbase folds out of the loop
+    act.add(cbase + mc, (uint32_t)K.i[I_TRI]);
   }
 }
 
@@ -459,6 +461,36 @@ struct MarkAction {
       return;
     }
     bytes[(uint64_t)(s - 1) * kBrickVox + (rel & (kBrickVox - 1))] = 1;
+  }
+  __device__ __forceinline__ void flush() {}
+};
+
+// MarkAction for passes whose units are all dense and geometry-only: the byte store and
+// the brick flag with no per-voxel mode tests, through global (not generic) pointers.
+typedef __attribute__((address_space(1))) uint8_t gbyte_t;
+typedef __attribute__((address_space(1))) uint32_t gword_t;
+struct MarkDenseAction {
+  static constexpr bool kTouch = false;
+  gword_t* table;
+  gbyte_t* bytes;
+  uint64_t base;
+  uint32_t cbrick = 0xFFFFFFFFu;
+  __device__ static MarkDenseAction make(const ActArgs& A, const VoxParams& P) {
+    MarkDenseAction m;
+    m.table = (gword_t*)(A.table + P.table_base);
+    m.bytes = (gbyte_t*)P.grid8;
+    m.base = P.bstart;
+    return m;
+  }
+  __device__ __forceinline__ void touch(uint64_t) {}
+  __device__ __forceinline__ void add(uint64_t code, uint32_t) {
+    const uint64_t rel = code - base;
+    const uint32_t b = (uint32_t)(rel >> kBrickBits);
+    bytes[rel] = 1;
+    if (b != cbrick) {
+      cbrick = b;
+      table[b] = 1u;
+    }
   }
   __device__ __forceinline__ void flush() {}
 };
@@ -673,13 +705,12 @@ struct OccStore {
   uint32_t* occ;
   uint32_t* table;  // flagged entries become slot + 1 (slots in Morton order)
   ull* ctr;
-  uint64_t n;
-  __device__ void operator()(uint64_t i, uint32_t ex, uint32_t v) const {
+  __device__ void operator()(uint64_t i, uint32_t ex, uint32_t v, bool last) const {
     if (v) {
       occ[ex] = (uint32_t)i;
       table[i] = ex + 1u;
     }
-    if (i == n - 1) ctr[kCtrNumOcc] = (ull)(ex + v);
+    if (last) ctr[kCtrNumOcc] = (ull)(ex + v);
   }
 };
 
@@ -805,23 +836,21 @@ __global__ __launch_bounds__(256) void k_brick_count(const uint32_t* __restrict_
 struct CountStore {
   uint64_t* off;
   ull* total;
-  const ull* n_dev;  // actual element count (device)
-  __device__ void operator()(uint64_t i, uint64_t ex, uint64_t v) const {
+  __device__ void operator()(uint64_t i, uint64_t ex, uint64_t v, bool last) const {
     off[i] = ex;
-    if (i == *n_dev - 1) *total = (ull)(ex + v);
+    if (last) *total = (ull)(ex + v);
   }
 };
 // Brick counts of a dense pass: also each brick's first voxel in the stream (s0 + offset).
 struct BrickOffStore {
   uint64_t* off;
   ull* total;
-  const ull* n_dev;
   uint64_t* bvoff;
   uint64_t s0;
-  __device__ void operator()(uint64_t i, uint64_t ex, uint64_t v) const {
+  __device__ void operator()(uint64_t i, uint64_t ex, uint64_t v, bool last) const {
     off[i] = ex;
     bvoff[i] = s0 + (ex & kVoxMask);
-    if (i == *n_dev - 1) *total = (ull)(ex + v);
+    if (last) *total = (ull)(ex + v);
   }
 };
 struct ColLoad {
@@ -1092,10 +1121,9 @@ struct PartLoad {
 struct PartStore {
   uint32_t* list;
   ull* total;
-  uint64_t n;
-  __device__ void operator()(uint64_t i, uint32_t ex, uint32_t v) const {
+  __device__ void operator()(uint64_t i, uint32_t ex, uint32_t v, bool last) const {
     if (v) list[ex] = (uint32_t)i;
-    if (i == n - 1) *total = (ull)(ex + v);
+    if (last) *total = (ull)(ex + v);
   }
 };
 
@@ -1304,7 +1332,7 @@ hipError_t partition_job(Context& c, uint32_t jid, float unitlength, uint64_t gr
     unmorton3(j, cx, cy, cz);
     if (n) {
       PartLoad ld{c.hitbox.as<uint64_t>(), cx, cy, cz};
-      PartStore st{c.part_lists.as<uint32_t>() + c.part_used, d_tot, n};
+      PartStore st{c.part_lists.as<uint32_t>() + c.part_used, d_tot};
       SVO_TRY((scan_exclusive<uint32_t>(ld, st, n, c.scan_scratch.as<uint32_t>(), c.stream)));
     }
     ull tot = 0;
@@ -1406,6 +1434,8 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
                                 (units[u].stride == 21 ? 4 : 1);
   const uint64_t dense_bytes = goff[nu];
   const bool dense = dense_bytes <= c.dense_limit;
+  bool geometry_only = true;  // no unit carries payload records
+  for (uint32_t u = 0; u < nu; u++) geometry_only = geometry_only && units[u].stride == 9;
   // the table, the dense grid and the byte pool are all-zero whenever no pass is running
   if (c.table_entries < ntab) {
     SVO_TRY(ensure(c.table, ntab * 4));
@@ -1514,7 +1544,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   auto occ_scan = [&]() {
     return scan_single<uint32_t>(
         OccLoad{c.table.as<uint32_t>()},
-        OccStore{c.occ_list.as<uint32_t>(), c.table.as<uint32_t>(), ctr, ntab}, ntab, st_occ,
+        OccStore{c.occ_list.as<uint32_t>(), c.table.as<uint32_t>(), ctr}, ntab, st_occ,
         ctr + kCtrError, c.stream);
   };
 
@@ -1540,16 +1570,21 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     }
     // first pass: dense units mark directly, touch units flag their bricks; both collect
     // the large triangles
-    if (dense)
+    if (dense && geometry_only)
+      SVO_TRY(tile_pass(MarkDenseAction{}, "vox_small_mark", queues, 1, 0));
+    else if (dense)
       SVO_TRY(tile_pass(MarkAction{}, "vox_small_mark", queues, 1, 0));
     else
       SVO_TRY(tile_pass(TouchAction{}, "vox_small_touch", queues, 1, 0));
     SVO_TRY((scan_single<uint64_t>(ColLoad{l_cols},
-                                   CountStore{l_off, ctr + kCtrCols, ctr + kCtrLargeCount},
+                                   CountStore{l_off, ctr + kCtrCols},
                                    ntri, st_col, ctr + kCtrError, c.stream,
                                    ctr + kCtrLargeCount)));
     if (dense) {
-      SVO_TRY(large_pass(MarkAction{}, "vox_large_mark", 0));
+      if (geometry_only)
+        SVO_TRY(large_pass(MarkDenseAction{}, "vox_large_mark", 0));
+      else
+        SVO_TRY(large_pass(MarkAction{}, "vox_large_mark", 0));
       // number the flagged bricks in Morton order: table entry = rank + 1, occ = list
       SVO_TRY(occ_scan());
     } else {
@@ -1572,24 +1607,24 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     if (bc_single && dense)
       SVO_TRY((scan_single<uint64_t>(
           ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
-          BrickOffStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox, ctr + kCtrNumOcc,
+          BrickOffStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox,
                         c.bvoff.as<uint64_t>() + nb0, s0},
           bc_bound, st_bc, ctr + kCtrError, c.stream, ctr + kCtrNumOcc)));
     else if (bc_single)
       SVO_TRY((scan_single<uint64_t>(
           ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
-          CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox, ctr + kCtrNumOcc}, bc_bound,
+          CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox}, bc_bound,
           st_bc, ctr + kCtrError, c.stream, ctr + kCtrNumOcc)));
     else if (dense)
       SVO_TRY((scan_exclusive<uint64_t>(
           ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
-          BrickOffStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox, ctr + kCtrNumOcc,
+          BrickOffStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox,
                         c.bvoff.as<uint64_t>() + nb0, s0},
           bc_bound, c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrNumOcc)));
     else
       SVO_TRY((scan_exclusive<uint64_t>(
           ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
-          CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox, ctr + kCtrNumOcc}, bc_bound,
+          CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox}, bc_bound,
           c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrNumOcc)));
     hipLaunchKernelGGL(k_unit_voxels, dim3((nu + 256) / 256), dim3(256), 0, c.stream,
                        c.occ_list.as<uint32_t>(), ctr, d_ubase, nu, c.bcount.as<uint64_t>(),
