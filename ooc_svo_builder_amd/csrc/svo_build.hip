@@ -591,9 +591,15 @@ __global__ __launch_bounds__(kWave * kEmitWaves) void k_emit_bricks(
   const bool live = w < nb && !M.flag[0];
   // every wave of the block reaches the barrier below; a wave without a brick works on
   // zeros and writes nothing
+  // the brick's own words first, then its job and offsets: all in flight while the
+  // offset tables are built
+  uint64_t bc = 0, bvo = 0, S = 0;
   BrickShape B;
   if (live) {
     B = brick_shape(bwords, w);
+    bc = bcode[w];
+    bvo = bvoff[w];
+    S = A.start[M.off[4] + w];
   } else {
     B.word = 0;
     B.m2 = B.s2 = B.mask3 = B.s3 = B.mask4 = 0;
@@ -631,9 +637,8 @@ __global__ __launch_bounds__(kWave * kEmitWaves) void k_emit_bricks(
   }
   __syncthreads();
   if (!live) return;
-  const JobDev& J = jobs[T.job(bcode[w])];
-  const uint64_t S = A.start[M.off[4] + w];
-  const uint64_t vox0 = bvoff[w] - J.voff;
+  const JobDev& J = jobs[T.job(bc)];
+  const uint64_t vox0 = bvo - J.voff;
   const uint32_t R = B.s4 - 1u;        // records below the brick node
   const uint32_t RG = s_go[ws][8];     // records inside the groups' ranges
   if (S + R > J.n_nodes || J.nbase + S + R > node_cap) {

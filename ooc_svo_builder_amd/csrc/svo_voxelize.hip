@@ -299,6 +299,19 @@ __device__ __forceinline__ void touch_column(const Walk& K, const VoxParams& P, 
   for (int cb = lo >> 4; cb <= (hi >> 4); cb++) act.touch(bbase | (spread3((uint32_t)cb) << w));
 }
 
+// Morton spreading in the walks from a table in LDS (s_spread[v] = spread3(v) for 8-bit
+// v): three lookups and a few shifts instead of the five-step bit spread. Every kernel
+// that walks columns fills the table with fill_spread_table before its first walk.
+__shared__ uint32_t s_spread[256];
+__device__ __forceinline__ void fill_spread_table() {
+  for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) s_spread[i] = (uint32_t)spread3(i);
+}
+// spread3(v) for v < 2^21
+__device__ __forceinline__ uint64_t spread3_s(uint32_t v) {
+  const uint32_t a = s_spread[v & 255u], b = s_spread[(v >> 8) & 255u], c = s_spread[(v >> 16) & 31u];
+  return (uint64_t)(a | (b << 24)) | ((uint64_t)((b >> 8) | (c << 16)) << 32);
+}
+
 // Every bit of Morton axis k: a coordinate spread onto these bits advances by one with a
 // masked add, (m | ~mask) + (1 << k), then & mask.
 constexpr uint64_t kAxis0Bits = 0x1249249249249249ull;
@@ -330,7 +343,7 @@ __device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, i
     KB[k] = K.f[F_BC + k] * cB;
   }
   const float nw = K.f[F_NW], d1 = K.f[F_D1], d2 = K.f[F_D2];
-  uint64_t mc = spread3((uint32_t)lo) << w;
+  uint64_t mc = spread3_s((uint32_t)lo) << w;
   for (int c = lo; c <= hi; c++, mc = morton_step(mc, w)) {
     const float pc = (float)c * u;
     const float t = (nw * pc + Ka) + Kb;
@@ -355,7 +368,7 @@ __device__ __forceinline__ void walk_column_ab(const Walk& K, const VoxParams& P
   const int w = K.w();
   const int ua = (w == 0) ? 1 : 0;
   const int va = (w == 2) ? 1 : 2;
-  walk_column(K, P, a, b, (spread3((uint32_t)a) << ua) | (spread3((uint32_t)b) << va), act);
+  walk_column(K, P, a, b, (spread3_s((uint32_t)a) << ua) | (spread3_s((uint32_t)b) << va), act);
 }
 
 // All columns of a walk, row by row, the Morton parts of a and b stepped incrementally.
@@ -364,9 +377,10 @@ __device__ __forceinline__ void walk_all_columns(const Walk& K, const VoxParams&
   const int w = K.w();
   const int ua = (w == 0) ? 1 : 0;
   const int va = (w == 2) ? 1 : 2;
-  uint64_t mb = spread3((uint32_t)K.i[I_G0V]) << va;
+  uint64_t mb = spread3_s((uint32_t)K.i[I_G0V]) << va;
+  const uint64_t ma0 = spread3_s((uint32_t)K.i[I_G0U]) << ua;
   for (int b = K.i[I_G0V]; b <= K.i[I_G1V]; b++, mb = morton_step(mb, va)) {
-    uint64_t ma = spread3((uint32_t)K.i[I_G0U]) << ua;
+    uint64_t ma = ma0;
     for (int a = K.i[I_G0U]; a <= K.i[I_G1U]; a++, ma = morton_step(ma, ua)) {
       if constexpr (Act::kTouch)
         touch_column(K, P, a, b, act);
@@ -576,6 +590,7 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
   __shared__ uint32_t soff[kTileTris + 1];
   __shared__ uint32_t wsum[kTile / kWave];
   __shared__ uint32_t s_tile;
+  fill_spread_table();  // read after the first barrier of the tile loop
   ull* ctr = A.ctr;
   const uint32_t total_tiles = tile_off[n_units];
   const uint32_t tid = threadIdx.x;
@@ -670,6 +685,8 @@ __global__ __launch_bounds__(256) void k_vox_large(const VoxParams* __restrict__
                                                    const uint32_t* __restrict__ large_tri,
                                                    const uint64_t* __restrict__ large_off,
                                                    int skip) {
+  fill_spread_table();
+  __syncthreads();
   const ull* ctr = A.ctr;
   const uint32_t n_large = (uint32_t)ctr[kCtrLargeCount];
   const uint64_t total = ctr[kCtrCols];
@@ -731,6 +748,74 @@ __device__ __forceinline__ uint64_t waves_total() {
 // brick stream: there 'pool' is the stream's word array, bcode (non-null) receives the
 // tagged brick codes and the table entries are cleared here.
 // ONE: a single unit (its parameters are uniform, read with scalar loads).
+
+// Brick wv (table entry ti, unit u): its tagged code, and its table entry cleared.
+__device__ __forceinline__ void brick_head(uint64_t wv, uint64_t ti, uint32_t u,
+                                           const VoxParams* __restrict__ units,
+                                           const uint64_t* __restrict__ ubase,
+                                           uint64_t* __restrict__ bcode,
+                                           uint32_t* __restrict__ table) {
+  if (bcode && lane_id() == 0) {
+    bcode[wv] = (units[u].bstart + ((ti - ubase[u]) << kBrickBits)) |
+                ((uint64_t)units[u].job << kJobShift);
+    table[ti] = 0u;
+  }
+}
+// The lane's 16-byte pieces of a brick's bytes: load q of lane l is the 16 bytes at
+// q * 1024 + 16 l (each load of the wave one contiguous KiB).
+__device__ __forceinline__ uint4* brick_bytes(uint64_t wv, uint64_t ti, uint32_t u,
+                                              const VoxParams* __restrict__ units,
+                                              const uint64_t* __restrict__ ubase,
+                                              uint8_t* __restrict__ pool8) {
+  uint8_t* base = units[u].dense ? units[u].grid8 + (ti - ubase[u]) * kBrickVox
+                                 : pool8 + wv * kBrickVox;
+  return (uint4*)base + lane_id();
+}
+// The lane's occupancy word from the loaded pieces (piece q of lane l holds bits
+// 16 (l % 4) .. + 15 of word 16 q + l / 4); pieces that held marks are zeroed.
+__device__ __forceinline__ ull pack_brick_bytes(uint4* src, const uint4 (&v)[4]) {
+  ull parts = 0;  // 16 bits per load
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const uint32_t w4[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      const uint32_t x = w4[d];  // four bytes, each 0 or 1
+      const uint32_t nib = (x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u);
+      parts |= (ull)nib << (16 * q + 4 * d);
+    }
+    if (v[q].x | v[q].y | v[q].z | v[q].w) src[q * kWave] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  // word L = 16 q + r gathers the parts of load q from lanes 4 r .. 4 r + 3
+  const int L = lane_id(), q = L >> 4, r = L & 15;
+  ull m = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)parts, 4 * r + k, kWave);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(parts >> 32), 4 * r + k, kWave);
+    const ull p = ((ull)hi << 32) | lo;
+    m |= ((p >> (16 * q)) & 0xFFFFull) << (16 * k);
+  }
+  return m;
+}
+// Brick wv's word array, its node shape (dense passes) and its voxel count.
+__device__ __forceinline__ void brick_tail(uint64_t wv, ull m, ull* __restrict__ pool,
+                                           uint64_t* __restrict__ bcount,
+                                           const uint64_t* __restrict__ bcode,
+                                           uint32_t* __restrict__ bshape) {
+  pool[wv * kBrickWords + lane_id()] = m;
+  if (bcode) {  // the brick node's subtree size and child mask, for the build
+    const BrickShape B = brick_shape_of(m);
+    if (lane_id() == 0) bshape[wv] = brick_shape_word(B);
+  }
+  const uint32_t c = (uint32_t)__popcll(m);
+  const uint32_t tot = wave_incl_scan<uint32_t>(c);
+  // voxel count, plus a flag in bit kNonEmptyShift: one scan gives every brick its
+  // voxel offset and its rank among the non-empty bricks (the touch pass flags a
+  // superset of the bricks that end up holding voxels)
+  if (lane_id() == kWave - 1) bcount[wv] = tot + (tot ? (1ull << kNonEmptyShift) : 0ull);
+}
+
 template <bool ONE>
 __device__ __forceinline__ void brick_count_body(const uint32_t* __restrict__ occ,
                                                  uint64_t n_occ,
@@ -745,11 +830,7 @@ __device__ __forceinline__ void brick_count_body(const uint32_t* __restrict__ oc
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
     const uint64_t ti = occ[wv];
     const uint32_t u = ONE ? 0u : upper_index<uint64_t>(ubase, n_units + 1, ti);
-    if (bcode && lane_id() == 0) {
-      bcode[wv] = (units[u].bstart + ((ti - ubase[u]) << kBrickBits)) |
-                  ((uint64_t)units[u].job << kJobShift);
-      table[ti] = 0u;
-    }
+    brick_head(wv, ti, u, units, ubase, bcode, table);
     ull m = 0;
     if (units[u].wide) {
       // 4-byte cells (~winner, 0 = unmarked): kept for k_brick_emit, which clears them
@@ -769,47 +850,13 @@ __device__ __forceinline__ void brick_count_body(const uint32_t* __restrict__ oc
         }
       }
     } else {
-      // bytes: load q of lane l is the 16 bytes at q * 1024 + 16 l (each load of the wave
-      // one contiguous KiB), i.e. bits 16 (l % 4) .. + 15 of word 16 q + l / 4
-      uint8_t* base = units[u].dense ? units[u].grid8 + (ti - ubase[u]) * kBrickVox
-                                     : pool8 + wv * kBrickVox;
-      uint4* src = (uint4*)base + lane_id();
+      uint4* src = brick_bytes(wv, ti, u, units, ubase, pool8);
       uint4 v[4];
 #pragma unroll
       for (int q = 0; q < 4; q++) v[q] = src[q * kWave];
-      ull parts = 0;  // 16 bits per load
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const uint32_t w4[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
-#pragma unroll
-        for (int d = 0; d < 4; d++) {
-          const uint32_t x = w4[d];  // four bytes, each 0 or 1
-          const uint32_t nib = (x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u);
-          parts |= (ull)nib << (16 * q + 4 * d);
-        }
-        if (v[q].x | v[q].y | v[q].z | v[q].w) src[q * kWave] = make_uint4(0u, 0u, 0u, 0u);
-      }
-      // word L = 16 q + r gathers the parts of load q from lanes 4 r .. 4 r + 3
-      const int L = lane_id(), q = L >> 4, r = L & 15;
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)parts, 4 * r + k, kWave);
-        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(parts >> 32), 4 * r + k, kWave);
-        const ull p = ((ull)hi << 32) | lo;
-        m |= ((p >> (16 * q)) & 0xFFFFull) << (16 * k);
-      }
+      m = pack_brick_bytes(src, v);
     }
-    pool[wv * kBrickWords + lane_id()] = m;
-    if (bcode) {  // the brick node's subtree size and child mask, for the build
-      const BrickShape B = brick_shape_of(m);
-      if (lane_id() == 0) bshape[wv] = brick_shape_word(B);
-    }
-    const uint32_t c = (uint32_t)__popcll(m);
-    const uint32_t tot = wave_incl_scan<uint32_t>(c);
-    // voxel count, plus a flag in bit kNonEmptyShift: one scan gives every brick its
-    // voxel offset and its rank among the non-empty bricks (the touch pass flags a
-    // superset of the bricks that end up holding voxels)
-    if (lane_id() == kWave - 1) bcount[wv] = tot + (tot ? (1ull << kNonEmptyShift) : 0ull);
+    brick_tail(wv, m, pool, bcount, bcode, bshape);
   }
 }
 
@@ -1464,7 +1511,12 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   const size_t szP = ((nu * sizeof(VoxParams)) + 255) & ~(size_t)255;
   const size_t szB = (((nu + 1) * 8) + 255) & ~(size_t)255;
   const size_t szT = (((nu + 1) * 4) + 255) & ~(size_t)255;
-  SVO_TRY(ensure(c.unitdev, szP + 2 * szB + szT));
+  {
+    const void* prev = c.unitdev.p;
+    const size_t prev_cap = c.unitdev.cap;
+    SVO_TRY(ensure(c.unitdev, szP + 2 * szB + szT));
+    if (c.unitdev.p != prev || c.unitdev.cap != prev_cap) c.unit_img.clear();
+  }
   VoxParams* d_units = (VoxParams*)c.unitdev.p;
   uint64_t* d_ubase = (uint64_t*)((char*)c.unitdev.p + szP);
   uint32_t* d_toff = (uint32_t*)((char*)c.unitdev.p + szP + szB);
@@ -1474,7 +1526,11 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     memcpy(img.data(), P.data(), nu * sizeof(VoxParams));
     memcpy(img.data() + szP, ubase.data(), (nu + 1) * 8);
     memcpy(img.data() + szP + szB, toff.data(), (nu + 1) * 4);
-    SVO_TRY(upload_async(c, d_units, img.data(), img.size()));
+    // a pass over the same units as the last one (the same mesh again) finds them there
+    if (img != c.unit_img) {
+      SVO_TRY(upload_async(c, d_units, img.data(), img.size()));
+      c.unit_img.swap(img);
+    }
   }
 
   // dense passes write their bricks straight into the brick stream (every flagged brick
@@ -1626,11 +1682,18 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
           ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
           CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox}, bc_bound,
           c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrNumOcc)));
-    hipLaunchKernelGGL(k_unit_voxels, dim3((nu + 256) / 256), dim3(256), 0, c.stream,
-                       c.occ_list.as<uint32_t>(), ctr, d_ubase, nu, c.bcount.as<uint64_t>(),
-                       d_uvoff);
-    SVO_TRY(hipGetLastError());
-    {
+    if (nu == 1 && geometry_only) {
+      // one unit and no payload records (the only device reader of the unit offsets):
+      // its voxels are the whole pass, [0, voxel count)
+      const ReadReq rq = {hc, c.counters.p, kNumCounters * sizeof(ull)};
+      SVO_TRY(read_back(c, &rq, 1));
+      uvoff[0] = 0;
+      uvoff[1] = hc[kCtrNumVox] & kVoxMask;
+    } else {
+      hipLaunchKernelGGL(k_unit_voxels, dim3((nu + 256) / 256), dim3(256), 0, c.stream,
+                         c.occ_list.as<uint32_t>(), ctr, d_ubase, nu, c.bcount.as<uint64_t>(),
+                         d_uvoff);
+      SVO_TRY(hipGetLastError());
       const ReadReq rq[2] = {{uvoff.data(), d_uvoff, (nu + 1) * 8},
                              {hc, c.counters.p, kNumCounters * sizeof(ull)}};
       SVO_TRY(read_back(c, rq, 2));
