@@ -479,28 +479,36 @@ struct MarkAction {
   __device__ __forceinline__ void flush() {}
 };
 
-// MarkAction for passes whose units are all dense and geometry-only: the byte store and
-// the brick flag with no per-voxel mode tests, through global (not generic) pointers.
+// MarkAction for passes whose units are all dense: the cell store and the brick flag
+// without the slot-pool path, through global (not generic) pointers. kWide: some units
+// of the pass carry payload (4-byte cells, the unit's wide flag decides per voxel);
+// otherwise every cell is a byte.
 typedef __attribute__((address_space(1))) uint8_t gbyte_t;
 typedef __attribute__((address_space(1))) uint32_t gword_t;
-struct MarkDenseAction {
+template <bool kWide>
+struct MarkDenseT {
   static constexpr bool kTouch = false;
   gword_t* table;
   gbyte_t* bytes;
   uint64_t base;
+  int wide;
   uint32_t cbrick = 0xFFFFFFFFu;
-  __device__ static MarkDenseAction make(const ActArgs& A, const VoxParams& P) {
-    MarkDenseAction m;
+  __device__ static MarkDenseT make(const ActArgs& A, const VoxParams& P) {
+    MarkDenseT m;
     m.table = (gword_t*)(A.table + P.table_base);
     m.bytes = (gbyte_t*)P.grid8;
     m.base = P.bstart;
+    m.wide = kWide ? P.wide : 0;
     return m;
   }
   __device__ __forceinline__ void touch(uint64_t) {}
-  __device__ __forceinline__ void add(uint64_t code, uint32_t) {
+  __device__ __forceinline__ void add(uint64_t code, uint32_t tri) {
     const uint64_t rel = code - base;
     const uint32_t b = (uint32_t)(rel >> kBrickBits);
-    bytes[rel] = 1;
+    if (kWide && wide)  // the lowest covering triangle as the largest ~tri (0 = unmarked)
+      atomicMax((uint32_t*)bytes + rel, ~tri);
+    else
+      bytes[rel] = 1;
     if (b != cbrick) {
       cbrick = b;
       table[b] = 1u;
@@ -508,6 +516,8 @@ struct MarkDenseAction {
   }
   __device__ __forceinline__ void flush() {}
 };
+using MarkDenseAction = MarkDenseT<false>;
+using MarkDenseWideAction = MarkDenseT<true>;
 
 // Second payload pass: lowest triangle index per voxel rank.
 struct WinnerAction {
@@ -1629,7 +1639,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     if (dense && geometry_only)
       SVO_TRY(tile_pass(MarkDenseAction{}, "vox_small_mark", queues, 1, 0));
     else if (dense)
-      SVO_TRY(tile_pass(MarkAction{}, "vox_small_mark", queues, 1, 0));
+      SVO_TRY(tile_pass(MarkDenseWideAction{}, "vox_small_mark", queues, 1, 0));
     else
       SVO_TRY(tile_pass(TouchAction{}, "vox_small_touch", queues, 1, 0));
     SVO_TRY((scan_single<uint64_t>(ColLoad{l_cols},
@@ -1640,7 +1650,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
       if (geometry_only)
         SVO_TRY(large_pass(MarkDenseAction{}, "vox_large_mark", 0));
       else
-        SVO_TRY(large_pass(MarkAction{}, "vox_large_mark", 0));
+        SVO_TRY(large_pass(MarkDenseWideAction{}, "vox_large_mark", 0));
       // number the flagged bricks in Morton order: table entry = rank + 1, occ = list
       SVO_TRY(occ_scan());
     } else {
