@@ -67,6 +67,15 @@ __device__ __forceinline__ float canon_nan(float f) {
 // Wave-level helpers (64 lanes)
 // ------------------------------------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
+// Order a wave's LDS stores before its lanes read one another's values (and those reads
+// before the wave's next stores): a wave's LDS operations complete in issue order, so a
+// wavefront-scope fence (which keeps the compiler from moving them) and a wave barrier
+// suffice.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <typename T>
 __device__ __forceinline__ T shfl_up_t(T v, int d);

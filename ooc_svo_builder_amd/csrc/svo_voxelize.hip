@@ -917,7 +917,10 @@ struct ColLoad {
 
 // The brick stream entry of every non-empty flagged brick and, with kCodes, for payload
 // units the sorted codes of its voxels, tagged with the unit's job in bits kJobShift+
-// (and the payload winners of dense payload units).
+// (and the payload winners of dense payload units). stage_cells: the launch gives each
+// wave kBrickVox words of dynamic LDS, into which a dense payload brick's 4-byte cells
+// are copied with coalesced loads (and cleared in the grid) before its voxels read their
+// winners from there, instead of one dependent global load per voxel.
 template <bool kCodes>
 __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__ occ,
                                                     const ull* __restrict__ ctr,
@@ -932,7 +935,9 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
                                                     uint64_t* __restrict__ bcode,
                                                     uint64_t* __restrict__ bvoff,
                                                     uint32_t* __restrict__ bshape,
-                                                    uint32_t* __restrict__ winner) {
+                                                    uint32_t* __restrict__ winner,
+                                                    int stage_cells) {
+  extern __shared__ uint32_t s_cells[];  // stage_cells: kBrickVox per wave
   const uint64_t n_occ = ctr[kCtrNumOcc];
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
     const uint64_t ti = occ[wv];
@@ -961,6 +966,28 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
     if (wprefix) wprefix[slot] = (uint32_t)pos;
     const uint64_t cb = (b0 + ((uint64_t)lane_id() << 6)) | tag;
     // dense payload unit: each voxel's winner from its cell, which is cleared
+    if (units[u].wide && stage_cells) {
+      uint32_t* sc = s_cells + (threadIdx.x / kWave) * kBrickVox;
+      uint4* src = (uint4*)((uint32_t*)units[u].grid8 + (ti - ubase[u]) * kBrickVox);
+      uint4 v[kBrickVox / 4 / kWave];
+#pragma unroll
+      for (int r = 0; r < kBrickVox / 4 / kWave; r++) v[r] = src[r * kWave + lane_id()];
+#pragma unroll
+      for (int r = 0; r < kBrickVox / 4 / kWave; r++) {
+        ((uint4*)sc)[r * kWave + lane_id()] = v[r];
+        if (v[r].x | v[r].y | v[r].z | v[r].w) src[r * kWave + lane_id()] = make_uint4(0u, 0u, 0u, 0u);
+      }
+      wave_lds_sync();
+      const uint32_t* mine = sc + lane_id() * 64;  // the cells of the lane's word
+      while (word) {
+        const int bit = __ffsll(word) - 1;
+        winner[pos] = ~mine[bit];
+        codes[pos++] = cb + (uint64_t)bit;
+        word &= word - 1;
+      }
+      wave_lds_sync();  // the next brick's cells overwrite these
+      continue;
+    }
     uint32_t* cells = units[u].wide ? (uint32_t*)units[u].grid8 + (ti - ubase[u]) * kBrickVox +
                                           lane_id() * 64
                                     : nullptr;
@@ -1807,14 +1834,18 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     // writes codes and winners (rewriting those entries with the same values)
     if (!dense || codes_now) {
       ktimer_begin(c, "brick_emit");
+      // dense payload units: their cells are staged through LDS, kBrickVox words per wave
+      const bool stage_cells = codes_now && dense && any_payload;
       hipLaunchKernelGGL(codes_now ? k_brick_emit<true> : k_brick_emit<false>,
-                       dim3(brick_grid(n_occ)), dim3(256), 0, c.stream,
+                       dim3(brick_grid(n_occ)), dim3(256),
+                       stage_cells ? (256 / kWave) * kBrickVox * 4 : 0, c.stream,
                        c.occ_list.as<uint32_t>(), ctr, d_units, d_ubase, nu, pool_words,
                        c.bcount.as<uint64_t>(), c.codes.as<uint64_t>() + s0,
                        winner_walk ? c.wprefix.as<uint32_t>() : (uint32_t*)nullptr, s0,
                        c.bwords.as<uint64_t>() + nb0 * kBrickWords, c.bcode.as<uint64_t>() + nb0,
                        c.bvoff.as<uint64_t>() + nb0, c.bshape.as<uint32_t>() + nb0,
-                       any_payload ? c.winner.as<uint32_t>() : (uint32_t*)nullptr);
+                       any_payload ? c.winner.as<uint32_t>() : (uint32_t*)nullptr,
+                       stage_cells ? 1 : 0);
       ktimer_end(c);
       SVO_TRY(hipGetLastError());
     }
