@@ -419,78 +419,25 @@ struct TouchAction {
 // Mark voxels with plain byte stores of 1: concurrent marks of one voxel all store the
 // same value, so no atomic read-modify-write is needed however many triangles cover it
 // (about three per voxel on a terrain).
-//  * dense units: a byte per voxel of the unit's Morton range (the reference's own layout,
-//    voxelizer.cpp:144, kept in HBM); the brick's table entry is flagged with a plain
-//    store when the thread enters a brick. Only flagged bricks are compacted and cleared
-//    afterwards, so the grid is never swept. Dense payload units keep 4 bytes per voxel
-//    and resolve the winning triangle in the same walk (atomicMax of ~triangle).
-//  * touch units: 4096 bytes per pool slot; the slot comes from the table (touch pass +
-//    table scan), cached per thread.
-struct MarkAction {
-  static constexpr bool kTouch = false;
-  uint32_t* table;
-  uint8_t* bytes;  // dense grid, or the slot pool
-  ull* ctr;
-  uint64_t pool_cap;
-  uint64_t base;  // code of the first voxel of the unit's first brick
-  int dense, wide;
-  uint32_t cbrick = 0xFFFFFFFFu, cslot = 0;
-  __device__ static MarkAction make(const ActArgs& A, const VoxParams& P) {
-    MarkAction m;
-    m.table = A.table + P.table_base;
-    m.bytes = P.dense ? P.grid8 : A.pool8;
-    m.ctr = A.ctr;
-    m.pool_cap = A.pool_cap;
-    m.base = P.bstart;
-    m.dense = P.dense;
-    m.wide = P.wide;
-    return m;
-  }
-  __device__ __forceinline__ void touch(uint64_t) {}
-  __device__ __forceinline__ void add(uint64_t code, uint32_t tri) {
-    const uint64_t rel = code - base;
-    const uint32_t b = (uint32_t)(rel >> kBrickBits);
-    if (dense) {
-      // payload: the lowest covering triangle (voxelizer.cpp:263, first in partition
-      // order wins) as the largest ~tri; 0 = unmarked
-      if (wide)
-        atomicMax((uint32_t*)bytes + rel, ~tri);
-      else
-        bytes[rel] = 1;
-      if (b != cbrick) {
-        cbrick = b;
-        table[b] = 1u;
-      }
-      return;
-    }
-    if (b != cbrick) {
-      cbrick = b;
-      cslot = table[b];
-    }
-    const uint32_t s = cslot;
-    if (s == 0 || s > pool_cap) {
-      // s == 0 cannot happen (the touch pass covers every candidate voxel); s > pool_cap
-      // means the pool is too small and the pass is redone with a larger one
-      atomicAdd(&ctr[s == 0 ? kCtrError : kCtrOverflow], 1ull);
-      return;
-    }
-    bytes[(uint64_t)(s - 1) * kBrickVox + (rel & (kBrickVox - 1))] = 1;
-  }
-  __device__ __forceinline__ void flush() {}
-};
-
-// MarkAction for passes whose units are all dense: the cell store and the brick flag
-// without the slot-pool path, through global (not generic) pointers. kWide: some units
-// of the pass carry payload (4-byte cells, the unit's wide flag decides per voxel);
-// otherwise every cell is a byte.
+//  * dense units (MarkDenseT): a byte per voxel of the unit's Morton range (the
+//    reference's own layout, voxelizer.cpp:144, kept in HBM); the brick's table entry is
+//    flagged with a plain store when the thread enters a brick. Only flagged bricks are
+//    compacted and cleared afterwards, so the grid is never swept. Dense payload units
+//    keep 4 bytes per voxel and resolve the winning triangle in the same walk (atomicMax
+//    of ~triangle: voxelizer.cpp:263, the first triangle in partition order wins).
+//  * touch units (MarkPoolAction): 4096 bytes per pool slot; the slot comes from the table
+//    (touch pass + table scan), cached per thread.
+// Both store through global (not generic) pointers.
 typedef __attribute__((address_space(1))) uint8_t gbyte_t;
 typedef __attribute__((address_space(1))) uint32_t gword_t;
+// kWide: some units of the pass carry payload (4-byte cells; the unit's wide flag decides
+// per voxel); otherwise every cell is a byte.
 template <bool kWide>
 struct MarkDenseT {
   static constexpr bool kTouch = false;
   gword_t* table;
   gbyte_t* bytes;
-  uint64_t base;
+  uint64_t base;  // code of the first voxel of the unit's first brick
   int wide;
   uint32_t cbrick = 0xFFFFFFFFu;
   __device__ static MarkDenseT make(const ActArgs& A, const VoxParams& P) {
@@ -518,6 +465,43 @@ struct MarkDenseT {
 };
 using MarkDenseAction = MarkDenseT<false>;
 using MarkDenseWideAction = MarkDenseT<true>;
+
+struct MarkPoolAction {
+  static constexpr bool kTouch = false;
+  const gword_t* table;
+  gbyte_t* bytes;  // the slot pool
+  ull* ctr;
+  uint64_t pool_cap;
+  uint64_t base;  // code of the first voxel of the unit's first brick
+  uint32_t cbrick = 0xFFFFFFFFu, cslot = 0;
+  __device__ static MarkPoolAction make(const ActArgs& A, const VoxParams& P) {
+    MarkPoolAction m;
+    m.table = (const gword_t*)(A.table + P.table_base);
+    m.bytes = (gbyte_t*)A.pool8;
+    m.ctr = A.ctr;
+    m.pool_cap = A.pool_cap;
+    m.base = P.bstart;
+    return m;
+  }
+  __device__ __forceinline__ void touch(uint64_t) {}
+  __device__ __forceinline__ void add(uint64_t code, uint32_t) {
+    const uint64_t rel = code - base;
+    const uint32_t b = (uint32_t)(rel >> kBrickBits);
+    if (b != cbrick) {
+      cbrick = b;
+      cslot = table[b];
+    }
+    const uint32_t s = cslot;
+    if (s == 0 || s > pool_cap) {
+      // s == 0 cannot happen (the touch pass covers every candidate voxel); s > pool_cap
+      // means the pool is too small and the pass is redone with a larger one
+      atomicAdd(&ctr[s == 0 ? kCtrError : kCtrOverflow], 1ull);
+      return;
+    }
+    bytes[(uint64_t)(s - 1) * kBrickVox + (rel & (kBrickVox - 1))] = 1;
+  }
+  __device__ __forceinline__ void flush() {}
+};
 
 // Second payload pass: lowest triangle index per voxel rank.
 struct WinnerAction {
@@ -1684,8 +1668,8 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
       SVO_TRY(large_pass(TouchAction{}, "vox_large_touch", 0));
       SVO_TRY(occ_scan());
       // mark pass: byte stores into the slots
-      SVO_TRY(tile_pass(MarkAction{}, "vox_small_mark", queues + kQueueWords, 0, 0));
-      SVO_TRY(large_pass(MarkAction{}, "vox_large_mark", 0));
+      SVO_TRY(tile_pass(MarkPoolAction{}, "vox_small_mark", queues + kQueueWords, 0, 0));
+      SVO_TRY(large_pass(MarkPoolAction{}, "vox_large_mark", 0));
     }
     // per-brick voxel counts (bytes packed to bits, bytes cleared) and their offsets
     ktimer_begin(c, "brick_count");
