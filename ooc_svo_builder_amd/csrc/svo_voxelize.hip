@@ -1745,7 +1745,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   }
 
   // ---- job layout: voxel and data-record ranges
-  bool any_payload = false;
+  const bool any_payload = !geometry_only;
   if (batch) {
     // the passes of a batch take its jobs' partitions in job order: a job's range starts
     // at its first voxel and grows with every pass that holds its partitions (jobs start
@@ -1770,7 +1770,6 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     J.n_data = J.payload ? 1 + J.nvox : 2;
     n_records += J.n_data;
   }
-  for (uint32_t u = 0; u < nu; u++) any_payload |= units[u].stride == 21;
   std::vector<JobDev> jd(c.jobs.size());
   for (size_t j = 0; j < c.jobs.size(); j++) {
     const Job& J = c.jobs[j];
