@@ -152,6 +152,7 @@ struct Context {
   uint64_t dense_limit = 0;     // largest dense grid in bytes (set from free HBM)
   DevBuf counters;
   DevBuf large_unit, large_tri, large_cols, large_off;
+  DevBuf large_walk;  // walk constants of large triangles (44 words each, capped)
   DevBuf occ_list, bcount, wprefix, winner;
   DevBuf scan_scratch;
   DevBuf scan_status;  // status words of the single-pass scans of a voxelize pass

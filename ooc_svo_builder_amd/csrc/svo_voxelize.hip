@@ -549,8 +549,9 @@ __device__ __forceinline__ uint32_t upper_index(const T* __restrict__ a, uint32_
   return lo;
 }
 
-// Small tier, persistent: blocks take tiles of kTileTris consecutive triangles of one unit from
-// a device work queue (tile_off[u] = first tile of unit u). For a tile, each thread builds
+// Small tier, persistent: blocks take tiles of tt <= kTileTris consecutive triangles of one
+// unit from a device work queue (tile_off[u] = first tile of unit u); tt is smaller for
+// passes with few triangles, so that their tiles still spread over the CUs. For a tile, each thread builds
 // its triangle's walk constants into LDS; a block scan lays out the (triangle, column)
 // items; then the threads share the items, one column each, so the work per lane is a few
 // voxel tests regardless of triangle size. Triangles with more than small_cols columns
@@ -565,6 +566,7 @@ constexpr int kTileTris = 192;  // triangles per tile: 35 KB of walk constants i
 constexpr uint32_t kQueues = 8;        // tile queues per persistent launch
 constexpr uint32_t kQueueStride = 16;  // words between their counters (own cache lines)
 constexpr uint32_t kQueueWords = kQueues * kQueueStride;
+constexpr uint64_t kLargeWalkMax = 1u << 18;  // large triangles with stored walk constants
 constexpr int kSkipDense = 1;   // units voxelized in a dense grid (touch pass)
 constexpr int kSkipBinary = 2;  // geometry-only units (payload winner pass)
 
@@ -578,7 +580,8 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
                                                     const uint32_t* __restrict__ tile_off,
                                                     ActArgs A, ull* queue,
                                                     uint32_t* large_unit, uint32_t* large_tri,
-                                                    uint32_t* large_cols, int collect,
+                                                    uint32_t* large_cols, uint32_t* large_walk,
+                                                    uint32_t walk_cap, uint32_t tt, int collect,
                                                     int skip) {
   __shared__ uint32_t sk[kWalkWords][kTileTris];
   __shared__ uint32_t soff[kTileTris + 1];
@@ -613,9 +616,9 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
     const uint32_t u = upper_index<uint32_t>(tile_off, n_units + 1, tile);
     const VoxParams P = units[u];
     const bool active = !skipped(P, skip);
-    const uint32_t i = (tile - tile_off[u]) * kTileTris + tid;
+    const uint32_t i = (tile - tile_off[u]) * tt + tid;
     uint32_t items = 0;
-    if (active && tid < (uint32_t)kTileTris && i < P.n) {
+    if (active && tid < tt && i < P.n) {
       const uint32_t t = P.list ? P.list[i] : i;
       Setup S;
       setup_tri(P.tris + (uint64_t)t * P.stride, P, S);
@@ -629,6 +632,13 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
           large_unit[idx] = u;
           large_tri[idx] = t;
           large_cols[idx] = (uint32_t)ncols;
+          if (idx < walk_cap) {  // its walk constants, for the column threads
+            uint32_t* dst = large_walk + idx * kWalkWords;
+#pragma unroll
+            for (int f = 0; f < kWalkF; f++) dst[f] = __float_as_uint(K.f[f]);
+#pragma unroll
+            for (int f = 0; f < kWalkI; f++) dst[kWalkF + f] = (uint32_t)K.i[f];
+          }
         }
       } else if (ncols <= (uint64_t)P.direct_cols) {
         // few columns: this thread walks them now, from registers
@@ -645,13 +655,13 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
     }
     uint32_t total;
     const uint32_t inc = block_incl_scan<uint32_t, kTile>(items, wsum, total);
-    if (tid < (uint32_t)kTileTris) soff[tid] = inc - items;
-    if (tid == kTileTris - 1) soff[kTileTris] = total;
+    if (tid < tt) soff[tid] = inc - items;
+    if (tid == tt - 1) soff[tt] = total;
     __syncthreads();
     Act act = Act::make(A, P);
     for (uint32_t it = tid; it < total; it += kTile) {
       // owner: last j with soff[j] <= it (triangles without items have empty ranges)
-      const uint32_t lo = upper_index<uint32_t>(soff, kTileTris, it);
+      const uint32_t lo = upper_index<uint32_t>(soff, tt, it);
       Walk K;
 #pragma unroll
       for (int f = 0; f < kWalkF; f++) K.f[f] = __uint_as_float(sk[f][lo]);
@@ -678,26 +688,51 @@ __global__ __launch_bounds__(256) void k_vox_large(const VoxParams* __restrict__
                                                    const uint32_t* __restrict__ large_unit,
                                                    const uint32_t* __restrict__ large_tri,
                                                    const uint64_t* __restrict__ large_off,
-                                                   int skip) {
+                                                   const uint32_t* __restrict__ large_walk,
+                                                   uint32_t walk_cap, int skip) {
   fill_spread_table();
   __syncthreads();
+  __shared__ uint32_t s_j0;
   const ull* ctr = A.ctr;
   const uint32_t n_large = (uint32_t)ctr[kCtrLargeCount];
   const uint64_t total = ctr[kCtrCols];
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
-    const uint32_t j = upper_index<uint64_t>(large_off, n_large, g);
+  // each block walks one contiguous range of the columns, 256 at a time: the owner of its
+  // first column comes from one binary search, and a thread's owner only moves forward
+  // (a triangle here has more than small_cols columns), one offset compare per step
+  const uint64_t per = (total + gridDim.x - 1) / gridDim.x;
+  const uint64_t beg = (uint64_t)blockIdx.x * per;
+  const uint64_t end = min(total, beg + per);
+  if (beg >= end) return;  // uniform per block
+  if (threadIdx.x == 0) s_j0 = upper_index<uint64_t>(large_off, n_large, beg);
+  __syncthreads();
+  uint32_t j = s_j0;
+  for (uint64_t g = beg + threadIdx.x; g < end; g += blockDim.x) {
+    while (j + 1 < n_large && large_off[j + 1] <= g) j++;
     const VoxParams& P = units[large_unit[j]];
     if (skipped(P, skip)) continue;
     const uint32_t t = large_tri[j];
     const uint64_t local = g - large_off[j];
-    Setup S;
-    setup_tri(P.tris + (uint64_t)t * P.stride, P, S);
     Walk K;
-    make_walk(S, P, t, K);
-    const uint64_t eu = (uint64_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
-    const int a = K.i[I_G0U] + (int)(local % eu);
-    const int b = K.i[I_G0V] + (int)(local / eu);
+    if (j < walk_cap) {  // stored by the small tier (the columns of a triangle share it)
+      const uint32_t* src = large_walk + (uint64_t)j * kWalkWords;
+#pragma unroll
+      for (int f = 0; f < kWalkF; f++) K.f[f] = __uint_as_float(src[f]);
+#pragma unroll
+      for (int f = 0; f < kWalkI; f++) K.i[f] = (int)src[kWalkF + f];
+    } else {
+      Setup S;
+      setup_tri(P.tris + (uint64_t)t * P.stride, P, S);
+      make_walk(S, P, t, K);
+    }
+    const uint32_t eu = (uint32_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
+    int a, b;
+    if (local <= 0xFFFFFFFFull) {  // 32-bit division (a 64-bit one is a long routine)
+      a = K.i[I_G0U] + (int)((uint32_t)local % eu);
+      b = K.i[I_G0V] + (int)((uint32_t)local / eu);
+    } else {
+      a = K.i[I_G0U] + (int)(local % eu);
+      b = K.i[I_G0V] + (int)(local / eu);
+    }
     Act act = Act::make(A, P);
     if constexpr (Act::kTouch)
       touch_column(K, P, a, b, act);
@@ -1469,6 +1504,15 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   std::vector<uint64_t> ubase(nu + 1);
   std::vector<uint32_t> toff(nu + 1);
   uint64_t ntab = 0, ntiles = 0, ntri = 0;
+  // triangles per tile: kTileTris once the pass has about four tiles of them per block of
+  // the persistent grid, else fewer (a multiple of 32, at least 32)
+  uint32_t tt = (uint32_t)kTileTris;
+  {
+    uint64_t all = 0;
+    for (uint32_t u = 0; u < nu; u++) all += units[u].n;
+    const uint64_t want = all / (4 * (uint64_t)std::max(1, c.tune.tile_blocks));
+    tt = (uint32_t)std::min<uint64_t>(kTileTris, std::max<uint64_t>(32, (want + 31) / 32 * 32));
+  }
   for (uint32_t u = 0; u < nu; u++) {
     // a unit smaller than a brick shares its brick with its neighbours: the brick stream
     // would hold that brick more than once
@@ -1484,7 +1528,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     ubase[u] = ntab;
     toff[u] = (uint32_t)ntiles;
     ntab += unit_bricks(units[u]);
-    ntiles += (units[u].n + kTileTris - 1) / kTileTris;
+    ntiles += (units[u].n + tt - 1) / tt;
     ntri += units[u].n;
   }
   ubase[nu] = ntab;
@@ -1570,6 +1614,10 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   SVO_TRY(ensure(c.large_tri, ntri * 4));
   SVO_TRY(ensure(c.large_cols, ntri * 4));
   SVO_TRY(ensure(c.large_off, ntri * 8 + 8));
+  // walk constants of the first kLargeWalkMax large triangles of a pass (the others are
+  // set up again by each of their column threads)
+  const uint32_t walk_cap = (uint32_t)std::min<uint64_t>(ntri, kLargeWalkMax);
+  SVO_TRY(ensure(c.large_walk, (size_t)walk_cap * kWalkWords * 4 + 4));
   SVO_TRY(ensure(c.occ_list, ntab * 4));
   SVO_TRY(ensure_scan_scratch(c, std::max<uint64_t>(ntri, ntab), 8));
   ull* ctr = c.counters.as<ull>();
@@ -1594,11 +1642,13 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   uint32_t* l_tri = c.large_tri.as<uint32_t>();
   uint32_t* l_cols = c.large_cols.as<uint32_t>();
   uint64_t* l_off = c.large_off.as<uint64_t>();
+  uint32_t* l_walk = c.large_walk.as<uint32_t>();
   auto tile_pass = [&](auto tag, const char* name, ull* queue, int collect, int skip) {
     using Act = decltype(tag);
     ktimer_begin(c, name);
     hipLaunchKernelGGL((k_vox_tile<Act>), dim3(tile_grid), dim3(kTile), 0, c.stream, d_units, nu,
-                       d_toff, args(), queue, l_unit, l_tri, l_cols, collect, skip);
+                       d_toff, args(), queue, l_unit, l_tri, l_cols, l_walk, walk_cap, tt,
+                       collect, skip);
     ktimer_end(c);
     return hipGetLastError();
   };
@@ -1606,7 +1656,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     using Act = decltype(tag);
     ktimer_begin(c, name);
     hipLaunchKernelGGL((k_vox_large<Act>), dim3(large_grid), dim3(256), 0, c.stream, d_units,
-                       args(), l_unit, l_tri, l_off, skip);
+                       args(), l_unit, l_tri, l_off, l_walk, walk_cap, skip);
     ktimer_end(c);
     return hipGetLastError();
   };
