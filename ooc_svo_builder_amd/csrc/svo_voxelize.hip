@@ -683,6 +683,14 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
 
 // Column-parallel tier: one thread per column of the deferred triangles; the counts live
 // on the device (ctr[kCtrLargeCount] triangles, ctr[kCtrCols] columns), the grid strides.
+// walk constants stored as 44 words (kWalkF floats, then kWalkI ints)
+__device__ __forceinline__ void load_walk(const uint32_t* __restrict__ src, Walk& K) {
+#pragma unroll
+  for (int f = 0; f < kWalkF; f++) K.f[f] = __uint_as_float(src[f]);
+#pragma unroll
+  for (int f = 0; f < kWalkI; f++) K.i[f] = (int)src[kWalkF + f];
+}
+
 template <typename Act>
 __global__ __launch_bounds__(256) void k_vox_large(const VoxParams* __restrict__ units, ActArgs A,
                                                    const uint32_t* __restrict__ large_unit,
@@ -708,18 +716,31 @@ __global__ __launch_bounds__(256) void k_vox_large(const VoxParams* __restrict__
   uint32_t j = s_j0;
   for (uint64_t g = beg + threadIdx.x; g < end; g += blockDim.x) {
     while (j + 1 < n_large && large_off[j + 1] <= g) j++;
-    const VoxParams& P = units[large_unit[j]];
-    if (skipped(P, skip)) continue;
-    const uint32_t t = large_tri[j];
-    const uint64_t local = g - large_off[j];
+    // the lanes of a wave nearly always share their triangle: then its unit parameters
+    // and walk constants come in with scalar loads (one per 16 words) instead of some 60
+    // vector loads per column, which bound this kernel
+    // (a wave that straddles two triangles sets its lanes' walks up again)
+    const uint32_t jf = (uint32_t)__builtin_amdgcn_readfirstlane((int)j);
+    VoxParams P;
     Walk K;
-    if (j < walk_cap) {  // stored by the small tier (the columns of a triangle share it)
-      const uint32_t* src = large_walk + (uint64_t)j * kWalkWords;
-#pragma unroll
-      for (int f = 0; f < kWalkF; f++) K.f[f] = __uint_as_float(src[f]);
-#pragma unroll
-      for (int f = 0; f < kWalkI; f++) K.i[f] = (int)src[kWalkF + f];
+    uint32_t t;
+    uint64_t local;
+    bool stored = false;
+    if (__all(j == jf)) {
+      P = units[large_unit[jf]];
+      t = large_tri[jf];
+      local = g - large_off[jf];
+      if (jf < walk_cap) {
+        load_walk(large_walk + (uint64_t)jf * kWalkWords, K);
+        stored = true;
+      }
     } else {
+      P = units[large_unit[j]];
+      t = large_tri[j];
+      local = g - large_off[j];
+    }
+    if (skipped(P, skip)) continue;
+    if (!stored) {
       Setup S;
       setup_tri(P.tris + (uint64_t)t * P.stride, P, S);
       make_walk(S, P, t, K);
