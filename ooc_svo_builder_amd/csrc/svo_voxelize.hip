@@ -1127,11 +1127,17 @@ __global__ __launch_bounds__(256) void k_payload_records(
     const uint64_t* __restrict__ uvoff, uint32_t n_units, const uint64_t* __restrict__ codes,
     uint64_t s0, const uint32_t* __restrict__ winner, uint64_t nvox, float* __restrict__ out,
     ull* ctr) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nvox) return;
+  __shared__ uint32_t s_u0;
+  const uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x;
+  const uint64_t i = b0 + threadIdx.x;
   // the unit holding voxel i of the pass (uvoff: first voxel of each unit); only payload
-  // units have their codes written
-  const uint32_t u = upper_index<uint64_t>(uvoff, n_units + 1, i);
+  // units have their codes written. The block's voxels are consecutive: one binary search
+  // for its first voxel, then a thread steps forward over the (rare) unit boundaries
+  if (threadIdx.x == 0) s_u0 = upper_index<uint64_t>(uvoff, n_units + 1, b0);
+  __syncthreads();
+  if (i >= nvox) return;
+  uint32_t u = s_u0;
+  while (u + 1 < n_units && uvoff[u + 1] <= i) u++;
   if (units[u].stride == 9) return;
   const uint64_t tagged = codes[s0 + i];
   const JobDev& J = jobs[units[u].job];
