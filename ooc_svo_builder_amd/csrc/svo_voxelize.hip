@@ -897,7 +897,9 @@ __device__ __forceinline__ void brick_count_body(const uint32_t* __restrict__ oc
                                                  uint32_t* __restrict__ bshape,
                                                  uint32_t* __restrict__ table) {
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
-    const uint64_t ti = occ[wv];
+    // the brick is the wave's: its table index in a scalar register, so that the unit
+    // search below runs on scalar loads
+    const uint64_t ti = (uint32_t)__builtin_amdgcn_readfirstlane((int)occ[wv]);
     const uint32_t u = ONE ? 0u : upper_index<uint64_t>(ubase, n_units + 1, ti);
     brick_head(wv, ti, u, units, ubase, bcode, table);
     ull m = 0;
@@ -999,7 +1001,7 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
   extern __shared__ uint32_t s_cells[];  // stage_cells: kBrickVox per wave
   const uint64_t n_occ = ctr[kCtrNumOcc];
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
-    const uint64_t ti = occ[wv];
+    const uint64_t ti = (uint32_t)__builtin_amdgcn_readfirstlane((int)occ[wv]);  // wave's brick
     const uint32_t u = upper_index<uint64_t>(ubase, n_units + 1, ti);
     const uint64_t slot = wv * kBrickWords + lane_id();
     ull word = pool[slot];
