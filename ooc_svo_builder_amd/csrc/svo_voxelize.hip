@@ -630,7 +630,9 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
       s_tile = tl;
     }
     __syncthreads();
-    const uint32_t tile = s_tile;
+    // the block's tile in a scalar register: the unit search and the unit's parameters
+    // then run on scalar loads
+    const uint32_t tile = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_tile);
     if (tile >= total_tiles) break;
     const uint32_t u = upper_index<uint32_t>(tile_off, n_units + 1, tile);
     const VoxParams P = units[u];
