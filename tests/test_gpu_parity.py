@@ -376,6 +376,33 @@ def test_voxelizer_paths_identical(small_cols, dense_mb, direct):
         _run_both(ctx, generators.terrain(n=49, seed=8, payload=True), True, 128, 2048)
 
 
+def test_payload_dense_coverage(gpu_ctx):
+    """A fine payload terrain on coarse grids (about 30 and 120 triangles per column): the
+    lowest covering triangle wins every voxel although neighbouring lanes of a wave mark
+    the same cells in the same step (voxelizer.cpp:263)."""
+    _say("payload dense coverage")
+    tris = generators.terrain(n=257, seed=5, payload=True)
+    for G in (64, 32):
+        _run_both(gpu_ctx, tris, True, G, 2048)
+
+
+def test_large_tier_beyond_walk_cache():
+    """More large triangles in one pass than the large tier keeps walk constants for
+    (2^18): those beyond are set up again by their column threads. Every triangle of a
+    318,402-triangle terrain at 2048 goes to the large tier (small_cols 4) and gives the
+    same bytes as the small tier's own walks (these knobs never change the output)."""
+    _say("large tier beyond the walk cache")
+    tris = generators.terrain(n=400, seed=9, payload=False)
+    with SvoContext(0) as ctx:
+        k = ctx.get_tuning()
+        ref = ctx.run(tris, False, 2048, 8192)
+        ref_nodes = ctx.nodes(ref.n_nodes).tobytes()
+        ctx.set_knobs(4, 0, k[2], k[3])
+        info = ctx.run(tris, False, 2048, 8192)
+        assert info.n_nodes == ref.n_nodes and info.nfilled == ref.nfilled
+        assert ctx.nodes(info.n_nodes).tobytes() == ref_nodes
+
+
 def test_mesh_load_file_stream(gpu_ctx, tmp_path):
     """TriReader replacement: a .tridata file streamed through the pinned chunks gives the
     same voxels as the in-memory load; a short file is an error naming what it holds."""
