@@ -113,6 +113,21 @@ __device__ __forceinline__ T wave_incl_scan(T v) {
   }
   return v;
 }
+// 32-bit form with data-parallel moves (DPP) instead of LDS permutes: shifts by 1, 2, 4
+// and 8 lanes within each row of 16 (lanes shifted in from outside the row read 0),
+// then lane 15 of rows 0 and 2 is added to rows 1 and 3, and lane 31 to rows 2 and 3
+// (rows the row mask leaves out add 0).
+template <>
+__device__ __forceinline__ uint32_t wave_incl_scan<uint32_t>(uint32_t v) {
+  int x = (int)v;
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return (uint32_t)x;
+}
 
 // inclusive scan over a block of BLOCK threads; 'total' receives the block sum.
 // smem must hold BLOCK/64 elements. All threads of the block must call it.
