@@ -128,6 +128,23 @@ __device__ __forceinline__ uint32_t wave_incl_scan<uint32_t>(uint32_t v) {
   x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
   return (uint32_t)x;
 }
+// 64-bit form: the same moves on both halves of the value, 64-bit adds
+template <int CTRL, int ROWS, bool BOUND>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, ROWS, 0xf, BOUND);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, ROWS, 0xf, BOUND);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+template <>
+__device__ __forceinline__ uint64_t wave_incl_scan<uint64_t>(uint64_t v) {
+  v += dpp_u64<0x111, 0xf, true>(v);
+  v += dpp_u64<0x112, 0xf, true>(v);
+  v += dpp_u64<0x114, 0xf, true>(v);
+  v += dpp_u64<0x118, 0xf, true>(v);
+  v += dpp_u64<0x142, 0xa, false>(v);
+  v += dpp_u64<0x143, 0xc, false>(v);
+  return v;
+}
 
 // inclusive scan over a block of BLOCK threads; 'total' receives the block sum.
 // smem must hold BLOCK/64 elements. All threads of the block must call it.
