@@ -176,11 +176,16 @@ __device__ __forceinline__ T block_incl_scan(T v, T* smem, T& total) {
 // s4 = 1 + sum s3 (non-null children only). All lanes of the wave must call it.
 // ------------------------------------------------------------------------------------
 // sum over the 8 lanes of a lane's group
+// (DPP moves: lane ^ 1 and lane ^ 2 by quad permutations, lane ^ 4 by a 4-lane shift
+// left or right within the 16-lane row)
 __device__ __forceinline__ uint32_t group8_sum(uint32_t v) {
-  v += (uint32_t)__shfl_xor((int)v, 1, kWave);
-  v += (uint32_t)__shfl_xor((int)v, 2, kWave);
-  v += (uint32_t)__shfl_xor((int)v, 4, kWave);
-  return v;
+  int x = (int)v;
+  x += __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xf, 0xf, true);  // quad_perm [1,0,3,2]
+  x += __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xf, 0xf, true);  // quad_perm [2,3,0,1]
+  const int up = __builtin_amdgcn_update_dpp(0, x, 0x104, 0xf, 0xf, true);    // row_shl:4
+  const int down = __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);  // row_shr:4
+  x += (lane_id() & 4) ? down : up;
+  return (uint32_t)x;
 }
 
 struct BrickShape {
@@ -212,9 +217,8 @@ __device__ __forceinline__ BrickShape brick_shape_of(uint64_t word) {
   for (int q = 0; q < 8; q++)
     if ((nz >> (8 * q)) & 0xFFull) B.mask4 |= 1u << q;
   // s4 = 1 + sum of the groups' s3: each group counted once (its first lane)
-  uint32_t v = ((lane & 7) == 0) ? B.s3 : 0u;
-  for (int d = 1; d < kWave; d <<= 1) v += (uint32_t)__shfl_xor((int)v, d, kWave);
-  B.s4 = 1u + v;
+  const uint32_t v = wave_incl_scan<uint32_t>(((lane & 7) == 0) ? B.s3 : 0u);
+  B.s4 = 1u + (uint32_t)__builtin_amdgcn_readlane((int)v, kWave - 1);
   return B;
 }
 // s4 (below 2^13) and the brick's child mask in one word
