@@ -607,14 +607,22 @@ __global__ __launch_bounds__(kWave * kEmitWaves) void k_emit_bricks(
   }
   const uint32_t v3 = (q == 0 && B.s3) ? B.s3 - 1u : 0u;
   const uint32_t inc3 = wave_incl_scan<uint32_t>(v3);
-  const uint32_t pre3 = (uint32_t)__shfl((int)(inc3 - v3), g * 8, kWave);
+  // only group heads add to inc3, and every lane of a group knows its group's s3: the
+  // group's exclusive prefix is inc3 less the head's own term
+  const uint32_t pre3 = inc3 - (B.s3 ? B.s3 - 1u : 0u);
   const uint32_t v2 = B.word ? B.s2 - 1u : 0u;
-  uint32_t inc2 = v2;
-#pragma unroll
-  for (int d = 1; d < 8; d <<= 1) {
-    const uint32_t o = (uint32_t)__shfl_up((int)inc2, d, kWave);
-    if (q >= d) inc2 += o;
+  // inclusive scan within each group of 8 lanes (DPP row shifts, lanes whose source lies
+  // in the previous group add nothing)
+  int x2 = (int)v2;
+  {
+    const int o1 = __builtin_amdgcn_update_dpp(0, x2, 0x111, 0xf, 0xf, true);  // row_shr:1
+    x2 += (q >= 1) ? o1 : 0;
+    const int o2 = __builtin_amdgcn_update_dpp(0, x2, 0x112, 0xf, 0xf, true);  // row_shr:2
+    x2 += (q >= 2) ? o2 : 0;
+    const int o4 = __builtin_amdgcn_update_dpp(0, x2, 0x114, 0xf, 0xf, true);  // row_shr:4
+    x2 += (q >= 4) ? o4 : 0;
   }
+  const uint32_t inc2 = (uint32_t)x2;
   const uint32_t wo = pre3 + (inc2 - v2);
   const uint32_t pw = (uint32_t)__popcll(B.word);
   const uint32_t incw = wave_incl_scan<uint32_t>(pw);
