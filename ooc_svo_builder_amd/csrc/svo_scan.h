@@ -268,7 +268,7 @@ __global__ __launch_bounds__(kSingleBlock) void k_scan_single(Load ld, Store st,
           const unsigned long long incl = __ballot((w >> 62) == 2ull);
           const int first = incl ? __ffsll((long long)incl) - 1 : kWave;
           const T val = (lane_id() <= first) ? (T)(w & kStVal) : (T)0;
-          prefix += shfl_t<T>(wave_incl_scan<T>(val), kWave - 1);
+          prefix += wave_last<T>(wave_incl_scan<T>(val));
           found = incl != 0ull;
         }
         j -= (int64_t)kWave * kLookWindows;
