@@ -776,7 +776,7 @@ __global__ __launch_bounds__(kUpTile) void k_up_count(Arena A, Tags T, const Job
     const uint64_t b = __ballot(t >= 5 + k);
     if (lane_id() == 0) wcnt[w][k] = (uint32_t)__popcll(b);
   }
-  const uint64_t sw = shfl_t<uint64_t>(wave_incl_scan<uint64_t>(s4), kWave - 1);
+  const uint64_t sw = wave_last<uint64_t>(wave_incl_scan<uint64_t>(s4));
   if (lane_id() == 0) ws4[w] = sw;
   __syncthreads();
   if (threadIdx.x < kUpLevels && 5 + (int)threadIdx.x <= H) {
@@ -810,13 +810,13 @@ __global__ __launch_bounds__(1024) void k_up_scan(Arena A, LevelMeta M, uint64_t
         const uint64_t v = t < ntiles ? tile_s4[t] : 0ull;
         const uint64_t inc = wave_incl_scan<uint64_t>(v);
         if (t < ntiles) tile_s4[t] = carry + inc - v;
-        carry += shfl_t<uint64_t>(inc, kWave - 1);
+        carry += wave_last<uint64_t>(inc);
       } else {
         uint32_t* cell = tile_cnt + (uint64_t)t * kUpLevels + k;
         const uint32_t v = t < ntiles ? *cell : 0u;
         const uint32_t inc = wave_incl_scan<uint32_t>(v);
         if (t < ntiles) *cell = (uint32_t)carry + inc - v;
-        carry += shfl_t<uint32_t>(inc, kWave - 1);
+        carry += wave_last<uint32_t>(inc);
       }
     }
     if (lane_id() == 0) s_tot[k] = carry;
