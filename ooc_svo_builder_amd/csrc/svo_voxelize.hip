@@ -192,25 +192,29 @@ struct Walk {
   }
 };
 
-// Copy one axis' plane roles: HP/AP/BP = first edge of the hoisted plane and of planes
-// A and B; HS = the hoisted plane's operands come as (pb, pa); AV0/BV0 = the column
-// coordinate is the plane's first operand.
-template <int HP, int AP, int BP, bool HS, bool AV0, bool BV0>
-__device__ __forceinline__ void fill_planes(const Setup& S, Walk& K) {
+// The plane roles for column axis w, as value selects: the hoisted plane's edges (H0, H1
+// in operand order (pa, pb), HD), and planes A and B with their column-coordinate factor
+// (AV, BV), the other factor (AC, BC) and the offset (AD, BD). Edges: xy = ne[0..2],
+// yz = ne[3..5], zx = ne[6..8].
+//   w = 2: hoisted xy(pa, pb); A = yz(pb, pc); B = zx(pc, pa)
+//   w = 0: hoisted yz(pa, pb); A = xy(pc, pa); B = zx(pb, pc)
+//   w = 1: hoisted zx(pb, pa); A = xy(pa, pc); B = yz(pc, pb)
+// (A branch per axis is merged by the compiler into an index on the edge arrays, which
+// puts them in scratch memory.)
+__device__ __forceinline__ void fill_planes(const Setup& S, Walk& K, int w) {
+  const bool w2 = (w == 2), w0 = (w == 0);
+  auto pick = [&](float a2, float a0, float a1) { return w2 ? a2 : (w0 ? a0 : a1); };
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    const float hx = S.ne[HP + k][0], hy = S.ne[HP + k][1];
-    K.f[F_H0 + k] = HS ? hy : hx;
-    K.f[F_H1 + k] = HS ? hx : hy;
-    K.f[F_HD + k] = S.de[HP + k];
-    const float ax = S.ne[AP + k][0], ay = S.ne[AP + k][1];
-    K.f[F_AV + k] = AV0 ? ax : ay;
-    K.f[F_AC + k] = AV0 ? ay : ax;
-    K.f[F_AD + k] = S.de[AP + k];
-    const float bx = S.ne[BP + k][0], by = S.ne[BP + k][1];
-    K.f[F_BV + k] = BV0 ? bx : by;
-    K.f[F_BC + k] = BV0 ? by : bx;
-    K.f[F_BD + k] = S.de[BP + k];
+    K.f[F_H0 + k] = pick(S.ne[k][0], S.ne[3 + k][0], S.ne[6 + k][1]);
+    K.f[F_H1 + k] = pick(S.ne[k][1], S.ne[3 + k][1], S.ne[6 + k][0]);
+    K.f[F_HD + k] = pick(S.de[k], S.de[3 + k], S.de[6 + k]);
+    K.f[F_AV + k] = pick(S.ne[3 + k][1], S.ne[k][0], S.ne[k][1]);
+    K.f[F_AC + k] = pick(S.ne[3 + k][0], S.ne[k][1], S.ne[k][0]);
+    K.f[F_AD + k] = pick(S.de[3 + k], S.de[k], S.de[k]);
+    K.f[F_BV + k] = pick(S.ne[6 + k][0], S.ne[6 + k][1], S.ne[3 + k][0]);
+    K.f[F_BC + k] = pick(S.ne[6 + k][1], S.ne[6 + k][0], S.ne[3 + k][1]);
+    K.f[F_BD + k] = pick(S.de[6 + k], S.de[6 + k], S.de[3 + k]);
   }
 }
 
@@ -220,13 +224,7 @@ __device__ __forceinline__ void make_walk(const Setup& S, const VoxParams& P, ui
   const int w = S.w;
   const int ua = (w == 0) ? 1 : 0;
   const int va = (w == 2) ? 1 : 2;
-  // edges of the three planes: xy = ne[0..2], yz = ne[3..5], zx = ne[6..8]
-  if (w == 2)         // hoisted xy(pa, pb); A = yz(pb, pc); B = zx(pc, pa)
-    fill_planes<0, 3, 6, false, false, true>(S, K);
-  else if (w == 0)    // hoisted yz(pa, pb); A = xy(pc, pa); B = zx(pb, pc)
-    fill_planes<3, 0, 6, false, true, false>(S, K);
-  else                // hoisted zx(pb, pa); A = xy(pa, pc); B = yz(pc, pb)
-    fill_planes<6, 0, 3, true, false, true>(S, K);
+  fill_planes(S, K, w);
   const float n0 = S.n[0], n1 = S.n[1], n2 = S.n[2];
   const float nw = (w == 0) ? n0 : (w == 1 ? n1 : n2);
   const float nua = (ua == 0) ? n0 : n1;
@@ -247,19 +245,16 @@ __device__ __forceinline__ void make_walk(const Setup& S, const VoxParams& P, ui
   } else {
     K.f[F_KA] = K.f[F_KB] = K.f[F_ALO] = K.f[F_AHI] = 0.0f;
   }
-  if (w == 0) {
-    K.i[I_G0U] = S.g0[1]; K.i[I_G1U] = S.g1[1];
-    K.i[I_G0V] = S.g0[2]; K.i[I_G1V] = S.g1[2];
-    K.i[I_G0W] = S.g0[0]; K.i[I_G1W] = S.g1[0];
-  } else if (w == 1) {
-    K.i[I_G0U] = S.g0[0]; K.i[I_G1U] = S.g1[0];
-    K.i[I_G0V] = S.g0[2]; K.i[I_G1V] = S.g1[2];
-    K.i[I_G0W] = S.g0[1]; K.i[I_G1W] = S.g1[1];
-  } else {
-    K.i[I_G0U] = S.g0[0]; K.i[I_G1U] = S.g1[0];
-    K.i[I_G0V] = S.g0[1]; K.i[I_G1V] = S.g1[1];
-    K.i[I_G0W] = S.g0[2]; K.i[I_G1W] = S.g1[2];
-  }
+  // bbox along (ua, va, w) by value selects (a branch per axis would let the compiler
+  // index the bbox arrays with w, which puts them in scratch memory)
+  const int x0 = S.g0[0], y0 = S.g0[1], z0 = S.g0[2];
+  const int x1 = S.g1[0], y1 = S.g1[1], z1 = S.g1[2];
+  K.i[I_G0U] = (w == 0) ? y0 : x0;
+  K.i[I_G1U] = (w == 0) ? y1 : x1;
+  K.i[I_G0V] = (w == 2) ? y0 : z0;
+  K.i[I_G1V] = (w == 2) ? y1 : z1;
+  K.i[I_G0W] = (w == 0) ? x0 : (w == 1 ? y0 : z0);
+  K.i[I_G1W] = (w == 0) ? x1 : (w == 1 ? y1 : z1);
   K.i[I_WS] = w | (S.slab ? 4 : 0);
   K.i[I_TRI] = (int)tri;
 }
