@@ -32,6 +32,7 @@ extern "C" {
 #define SVO_ERR_DEVICE 2      /* HIP runtime error */
 #define SVO_ERR_NOMEM 3       /* device allocation failed */
 #define SVO_ERR_UNSUPPORTED 4 /* option not implemented on the GPU path */
+#define SVO_ERR_IO 5          /* the output sink (or an output file) failed */
 
 /* colour modes of main.cpp:23,154-179 (-c model|fixed|linear|normal) */
 #define SVO_COLOR_MODEL 0
@@ -148,6 +149,43 @@ int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payloa
             int src_on_device, const float bbox[6], uint64_t gridsize,
             uint64_t memory_limit_mb, int color_mode, int generate_levels,
             svo_run_info* info);
+
+/* Streamed (out-of-core) build: OctreeBuilder (OctreeBuilder.h:33-36,
+ * OctreeBuilder.cpp:4-168) fed one partition at a time, as main.cpp:329-387 does, writing
+ * its records out as each partition completes instead of holding the whole octree.
+ *
+ * The sink receives the .octreenodes (kind SVO_SINK_NODES, 24-byte records) and
+ * .octreedata (kind SVO_SINK_DATA, 32-byte records) byte streams in order, in chunks of
+ * whole records in host memory valid for the call; it returns 0 to go on, anything else
+ * stops the build with SVO_ERR_IO. A NULL sink collects the records in HBM instead (then
+ * svo_copy_nodes / svo_copy_data / svo_nodes_device read them after svo_stream_end).
+ *
+ * svo_stream_begin: the OctreeBuilder constructor (the null record, and the white record
+ * of geometry-only builds, go out at once); gridsize and n_partitions as for svo_voxelize
+ * (the mesh loaded by svo_mesh_load* decides payload or geometry-only).
+ * svo_stream_partition: the voxels added since the previous call (svo_voxelize of this
+ * partition, or svo_add_voxels within its Morton range) form the subtree of partition
+ * 'partition': it is built, its records except its root go out, and the voxel stream is
+ * emptied. Partitions come in increasing order; a partition never passed is empty.
+ * svo_stream_end: finalizeTree (OctreeBuilder.cpp:34-55): the partition roots, the
+ * levels above them and the root go out; the .octree header counts and the hazard flag
+ * (as for svo_build). No part of the build holds more than one partition's voxels and
+ * subtree, so the 2^32-voxel limit of svo_build applies per partition. */
+#define SVO_SINK_NODES 0
+#define SVO_SINK_DATA 1
+typedef int (*svo_sink_fn)(void* user, int kind, const void* bytes, uint64_t size);
+int svo_stream_begin(svo_ctx* ctx, uint64_t gridsize, uint64_t n_partitions,
+                     int generate_levels, svo_sink_fn sink, void* user);
+int svo_stream_partition(svo_ctx* ctx, uint64_t partition);
+int svo_stream_end(svo_ctx* ctx, uint64_t* n_nodes_out, uint64_t* n_data_out, int* hazard_out);
+
+/* main.cpp:281-399 with the streamed build: like svo_run, the records going to 'sink'
+ * (see svo_stream_begin) partition by partition. svo_run itself takes this path, with the
+ * records collected in HBM, whenever the grid is split into more than one partition. */
+int svo_run_stream(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payload,
+                   int src_on_device, const float bbox[6], uint64_t gridsize,
+                   uint64_t memory_limit_mb, int color_mode, int generate_levels,
+                   svo_sink_fn sink, void* user, svo_run_info* info);
 
 /* Batched form (no reference counterpart: main.cpp builds one mesh per process).
  * Runs main.cpp:281-399 for n_jobs meshes at once: every job's partitions are voxelized

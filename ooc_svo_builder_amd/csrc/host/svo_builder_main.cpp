@@ -237,6 +237,15 @@ string val_to_string(uint64_t v) {
   return o.str();
 }
 
+// The two output files the streamed build writes into (.octreenodes, .octreedata).
+struct OutFiles {
+  FILE* f[2] = {nullptr, nullptr};
+};
+int file_sink(void* user, int kind, const void* bytes, uint64_t size) {
+  FILE* f = ((OutFiles*)user)->f[kind == SVO_SINK_DATA ? 1 : 0];
+  return fwrite(bytes, 1, (size_t)size, f) == (size_t)size ? 0 : 1;
+}
+
 int die(svo_ctx* ctx, const char* what) {
   std::cerr << "svo_builder: " << what << " failed: " << svo_last_error() << endl;
   if (ctx) svo_ctx_destroy(ctx);
@@ -313,10 +322,23 @@ int main(int argc, char* argv[]) {
     for (uint64_t i = 0; i < n_partitions; i++)
       cout << "  partition " << i << " - tri_count: " << counts[i] << endl;
 
-  // VOXELIZATION + SVO (main.cpp:311-387)
+  // VOXELIZATION + SVO (main.cpp:311-387): the octree is streamed into the output files
+  // partition by partition (OctreeBuilder.cpp:4-168 owns the two files the same way),
+  // the copy of one chunk of records overlapping the write of the previous one
   const float unitlength =
       (text_roundtrip(ti.bbox[3]) - text_roundtrip(ti.bbox[0])) / (float)gridsize;
   const uint64_t morton_part = (gridsize * gridsize * gridsize) / n_partitions;
+  OutFiles out;
+  out.f[0] = fopen((base + ".octreenodes").c_str(), "wb");
+  out.f[1] = fopen((base + ".octreedata").c_str(), "wb");
+  if (!out.f[0] || !out.f[1]) {
+    std::cerr << "svo_builder: cannot open output files for " << base << endl;
+    svo_ctx_destroy(ctx);
+    return 1;
+  }
+  if (svo_stream_begin(ctx, gridsize, n_partitions, generate_levels ? 1 : 0, file_sink, &out) !=
+      SVO_OK)
+    return die(ctx, "svo_stream_begin");
   uint64_t nfilled = 0;
   for (uint64_t i = 0; i < n_partitions; i++) {
     if (counts[i] == 0) continue;
@@ -331,44 +353,22 @@ int main(int argc, char* argv[]) {
     nfilled += nf;
     if (verbose) cout << "  found " << nf << " new voxels." << endl;
     cout << "Building SVO for partition " << i << " ..." << endl;
+    if (svo_stream_partition(ctx, i) != SVO_OK) return die(ctx, "svo_stream_partition");
   }
   uint64_t n_nodes = 0, n_data = 0;
   int hazard = 0;
-  const int rc = svo_build(ctx, gridsize, generate_levels ? 1 : 0, &n_nodes, &n_data, &hazard);
-  if (rc == SVO_ERR_UNSUPPORTED) {
-    std::cerr << "svo_builder: " << svo_last_error() << endl;
+  if (svo_stream_end(ctx, &n_nodes, &n_data, &hazard) != SVO_OK) return die(ctx, "svo_stream_end");
+  if (fclose(out.f[0]) != 0 || fclose(out.f[1]) != 0) {
+    std::cerr << "svo_builder: writing the output files for " << base << " failed" << endl;
     svo_ctx_destroy(ctx);
     return 1;
   }
-  if (rc != SVO_OK) return die(ctx, "svo_build");
   if (hazard)
     std::cerr << "warning: the last filled voxel is the second-to-last Morton code (or the grid "
                  "is an empty 1^3): the reference's output is undefined here; the completed tree "
                  "is written"
               << endl;
-
-  // write .octreenodes / .octreedata in chunks, then the header (octree_io.h:74-83)
-  {
-    const uint64_t chunk = 1ull << 22;
-    std::vector<char> buf(chunk * 32);
-    FILE* fn = fopen((base + ".octreenodes").c_str(), "wb");
-    FILE* fd = fopen((base + ".octreedata").c_str(), "wb");
-    if (!fn || !fd) {
-      std::cerr << "svo_builder: cannot open output files for " << base << endl;
-      return 1;
-    }
-    for (uint64_t s = 0; s < n_nodes; s += chunk) {
-      const uint64_t k = std::min(chunk, n_nodes - s);
-      if (svo_copy_nodes(ctx, buf.data(), s, k) != SVO_OK) return die(ctx, "svo_copy_nodes");
-      fwrite(buf.data(), 24, k, fn);
-    }
-    for (uint64_t s = 0; s < n_data; s += chunk) {
-      const uint64_t k = std::min(chunk, n_data - s);
-      if (svo_copy_data(ctx, buf.data(), s, k) != SVO_OK) return die(ctx, "svo_copy_data");
-      fwrite(buf.data(), 32, k, fd);
-    }
-    fclose(fn);
-    fclose(fd);
+  {  // octree_io.h:74-83
     std::ofstream h((base + ".octree").c_str());
     h << "#octreeheader 1" << endl;
     h << "gridlength " << gridsize << endl;

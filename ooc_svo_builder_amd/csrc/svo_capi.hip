@@ -130,6 +130,8 @@ void ktimer_flush(Context& c) {
 using namespace svo;
 
 static int fail(hipError_t e, const char* where) {
+  if (e == hipErrorUnknown && !g_err.empty() && g_err.rfind("the output sink", 0) == 0)
+    return SVO_ERR_IO;
   if (e == hipErrorOutOfMemory) {
     if (g_err.empty()) g_err = std::string(where) + ": out of device memory";
     return SVO_ERR_NOMEM;
@@ -164,6 +166,13 @@ static int fail(hipError_t e, const char* where) {
 
 static int is_pow2_u64(uint64_t x) { return x != 0 && !(x & (x - 1)); }
 
+// What the sticky consistency-failure counter counts (kCtrError): a look-back wait of a
+// single-pass scan that ran out of spins, a brick without a pool slot on the touched-brick
+// path, and a payload voxel whose covering triangle was not found.
+static const char* const kErrSources =
+    " consistency failures (scan look-back time-outs, brick pool slots not found or payload "
+    "voxels without a covering triangle)";
+
 // the sticky consistency-failure counter
 static hipError_t read_err(Context& c, unsigned long long* err) {
   *err = c.err_count;  // read with the build's results
@@ -197,6 +206,7 @@ static void reset_plain(Context& c) {
   c.n_bricks = 0;
   c.bricks_ok = true;
   c.codes_lazy = false;
+  c.st.active = false;  // a new mesh ends any streamed build in progress
 }
 
 // The 6-significant-digit text round trip a bbox value takes through the .trip header
@@ -279,6 +289,7 @@ void svo_ctx_destroy(svo_ctx* ctx) {
                     &c.lvl_pstart, &c.records2, &c.bwords, &c.bcode, &c.bvoff,
                     &c.scan_status, &c.lvl_sib, &c.up_tiles, &c.bshape, &c.large_walk};
   for (DevBuf* b : bufs) release(*b);
+  stream_release(c);
   if (c.stage) (void)hipHostFree(c.stage);
   c.stage = nullptr;
   if (c.hread) (void)hipHostFree(c.hread);
@@ -559,8 +570,7 @@ int svo_build(svo_ctx* ctx, uint64_t gridsize, int generate_levels, uint64_t* n_
   TRYC(read_err(c, &err),
        "svo_build");
   if (err) {
-    g_err = "internal: " + std::to_string(err) +
-            " voxels without a covering triangle in the payload pass";
+    g_err = "internal: " + std::to_string(err) + kErrSources;
     return SVO_ERR_DEVICE;
   }
   if (n_nodes_out) *n_nodes_out = c.n_nodes;
@@ -640,9 +650,70 @@ int svo_kernel_timing_reset(svo_ctx* ctx) {
   return SVO_OK;
 }
 
-int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payload,
-            int src_on_device, const float bbox[6], uint64_t gridsize, uint64_t memory_limit_mb,
-            int color_mode, int generate_levels, svo_run_info* info) {
+int svo_stream_begin(svo_ctx* ctx, uint64_t gridsize, uint64_t n_partitions,
+                     int generate_levels, svo_sink_fn sink, void* user) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (c.batch_state) {
+    g_err = "no mesh loaded (the context holds a batch)";
+    return SVO_ERR_INVALID;
+  }
+  if (!is_pow2_u64(gridsize) || gridsize > (1u << 15) || n_partitions == 0) {
+    g_err = "gridsize must be a power of 2 up to 32768 and n_partitions > 0";
+    return SVO_ERR_INVALID;
+  }
+  uint64_t p = n_partitions;
+  while (p % 8 == 0) p /= 8;
+  if (p != 1 || n_partitions > gridsize * gridsize * gridsize) {
+    g_err = "n_partitions must be a power of 8 not above gridsize^3";
+    return SVO_ERR_INVALID;
+  }
+  TRYC(stream_begin(c, gridsize, n_partitions, generate_levels, (SinkFn)sink, user),
+       "svo_stream_begin");
+  return SVO_OK;
+}
+
+int svo_stream_partition(svo_ctx* ctx, uint64_t partition) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (!c.st.active) {
+    g_err = "no streamed build in progress (svo_stream_begin)";
+    return SVO_ERR_INVALID;
+  }
+  if (partition < c.st.next_part || partition >= c.st.n_parts) {
+    g_err = "partitions must come in increasing order, below n_partitions";
+    return SVO_ERR_INVALID;
+  }
+  hipError_t e = stream_partition(c, partition);
+  if (e != hipSuccess) {
+    c.st.active = false;
+    return fail(e, "svo_stream_partition");
+  }
+  return SVO_OK;
+}
+
+int svo_stream_end(svo_ctx* ctx, uint64_t* n_nodes_out, uint64_t* n_data_out, int* hazard_out) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  if (!c.st.active) {
+    g_err = "no streamed build in progress (svo_stream_begin)";
+    return SVO_ERR_INVALID;
+  }
+  hipError_t e = stream_end(c, n_nodes_out, n_data_out, hazard_out);
+  if (e != hipSuccess) {
+    c.st.active = false;
+    return fail(e, "svo_stream_end");
+  }
+  return SVO_OK;
+}
+
+// main.cpp:281-399 in memory. One partition: voxelize and build in HBM (svo_build).
+// Several partitions, or a sink: the streamed build, one partition's subtree at a time
+// (records to the sink, or collected in HBM when sink == NULL).
+static int run_impl(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payload,
+                    int src_on_device, const float bbox[6], uint64_t gridsize,
+                    uint64_t memory_limit_mb, int color_mode, int generate_levels,
+                    bool streamed, svo_sink_fn sink, void* user, svo_run_info* info) {
   CHECK_CTX();
   Context& c = ctx->c;
   if (!bbox || !is_pow2_u64(gridsize) || memory_limit_mb <= 1) {
@@ -662,6 +733,7 @@ int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payloa
     TRYC(fill_u64(c.counters.as<uint64_t>(), 0, kNumCounters, c.stream), "svo_run");
   }
   const uint64_t np = svo_estimate_partitions(gridsize, memory_limit_mb);
+  streamed = streamed || np > 1;
   const float u_part = (bbox[3] - bbox[0]) / (float)gridsize;  // partitioner.cpp:45
   const float u_vox =
       (text_roundtrip(bbox[3]) - text_roundtrip(bbox[0])) / (float)gridsize;  // main.cpp:311
@@ -673,23 +745,55 @@ int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payloa
   if (r) return r;
   TRYC(hipEventRecord(c.ev1, c.stream), "svo_run");
   const uint64_t morton_part = (gridsize * gridsize * gridsize) / np;
-  for (uint64_t i = 0; i < np; i++) {
-    if (counts[i] == 0) continue;  // main.cpp:330
-    r = svo_voxelize(ctx, i, i * morton_part, (i + 1) * morton_part, u_vox, gridsize, color_mode,
-                     nullptr);
-    if (r) return r;
-  }
-  TRYC(hipEventRecord(c.ev2, c.stream), "svo_run");
   uint64_t nn = 0, nd = 0;
   int hz = 0;
-  c.build_done_ev = c.ev3;  // recorded by the build before its final read-back
-  r = svo_build(ctx, gridsize, generate_levels, &nn, &nd, &hz);
-  c.build_done_ev = nullptr;
-  if (r) return r;
-  TRYC(hipEventSynchronize(c.ev3), "svo_run");
-  TRYC(hipEventElapsedTime(&ms_p, c.ev0, c.ev1), "svo_run");
-  TRYC(hipEventElapsedTime(&ms_v, c.ev1, c.ev2), "svo_run");
-  TRYC(hipEventElapsedTime(&ms_b, c.ev2, c.ev3), "svo_run");
+  if (streamed) {
+    TRYC(hipEventSynchronize(c.ev1), "svo_run");
+    TRYC(hipEventElapsedTime(&ms_p, c.ev0, c.ev1), "svo_run");
+    r = svo_stream_begin(ctx, gridsize, np, generate_levels, sink, user);
+    if (r) return r;
+    for (uint64_t i = 0; i < np; i++) {
+      if (counts[i] == 0) continue;  // main.cpp:330
+      TRYC(hipEventRecord(c.ev0, c.stream), "svo_run");
+      r = svo_voxelize(ctx, i, i * morton_part, (i + 1) * morton_part, u_vox, gridsize,
+                       color_mode, nullptr);
+      if (r) return r;
+      TRYC(hipEventRecord(c.ev1, c.stream), "svo_run");
+      r = svo_stream_partition(ctx, i);
+      if (r) return r;
+      TRYC(hipEventRecord(c.ev2, c.stream), "svo_run");
+      TRYC(hipEventSynchronize(c.ev2), "svo_run");
+      float a = 0, b = 0;
+      TRYC(hipEventElapsedTime(&a, c.ev0, c.ev1), "svo_run");
+      TRYC(hipEventElapsedTime(&b, c.ev1, c.ev2), "svo_run");
+      ms_v += a;
+      ms_b += b;
+    }
+    TRYC(hipEventRecord(c.ev0, c.stream), "svo_run");
+    r = svo_stream_end(ctx, &nn, &nd, &hz);
+    if (r) return r;
+    TRYC(hipEventRecord(c.ev1, c.stream), "svo_run");
+    TRYC(hipEventSynchronize(c.ev1), "svo_run");
+    float b = 0;
+    TRYC(hipEventElapsedTime(&b, c.ev0, c.ev1), "svo_run");
+    ms_b += b;
+  } else {
+    for (uint64_t i = 0; i < np; i++) {
+      if (counts[i] == 0) continue;  // main.cpp:330
+      r = svo_voxelize(ctx, i, i * morton_part, (i + 1) * morton_part, u_vox, gridsize,
+                       color_mode, nullptr);
+      if (r) return r;
+    }
+    TRYC(hipEventRecord(c.ev2, c.stream), "svo_run");
+    c.build_done_ev = c.ev3;  // recorded by the build before its final read-back
+    r = svo_build(ctx, gridsize, generate_levels, &nn, &nd, &hz);
+    c.build_done_ev = nullptr;
+    if (r) return r;
+    TRYC(hipEventSynchronize(c.ev3), "svo_run");
+    TRYC(hipEventElapsedTime(&ms_p, c.ev0, c.ev1), "svo_run");
+    TRYC(hipEventElapsedTime(&ms_v, c.ev1, c.ev2), "svo_run");
+    TRYC(hipEventElapsedTime(&ms_b, c.ev2, c.ev3), "svo_run");
+  }
   if (info) {
     info->n_partitions = np;
     info->nfilled = c.nfilled;
@@ -701,6 +805,21 @@ int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payloa
     info->ms_build = ms_b;
   }
   return SVO_OK;
+}
+
+int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payload,
+            int src_on_device, const float bbox[6], uint64_t gridsize, uint64_t memory_limit_mb,
+            int color_mode, int generate_levels, svo_run_info* info) {
+  return run_impl(ctx, records, n_triangles, payload, src_on_device, bbox, gridsize,
+                  memory_limit_mb, color_mode, generate_levels, false, nullptr, nullptr, info);
+}
+
+int svo_run_stream(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payload,
+                   int src_on_device, const float bbox[6], uint64_t gridsize,
+                   uint64_t memory_limit_mb, int color_mode, int generate_levels,
+                   svo_sink_fn sink, void* user, svo_run_info* info) {
+  return run_impl(ctx, records, n_triangles, payload, src_on_device, bbox, gridsize,
+                  memory_limit_mb, color_mode, generate_levels, true, sink, user, info);
 }
 
 int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_result* results,
@@ -829,7 +948,7 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
   TRYC(read_err(c, &err),
        "svo_batch_run");
   if (err) {
-    g_err = "internal: " + std::to_string(err) + " voxelizer consistency failures";
+    g_err = "internal: " + std::to_string(err) + kErrSources;
     return SVO_ERR_DEVICE;
   }
   int any_hazard = 0;

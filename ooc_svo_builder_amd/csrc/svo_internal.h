@@ -120,6 +120,39 @@ struct VoxUnit {
   uint64_t gridsize;
 };
 
+// Output sink of the streamed build: consecutive chunks of the .octreenodes (kind 0) and
+// .octreedata (kind 1) byte streams, in host memory valid during the call (the C ABI's
+// svo_sink_fn). A non-zero return stops the build.
+typedef int (*SinkFn)(void* user, int kind, const void* bytes, uint64_t size);
+
+// A partition's subtree root kept for the levels above the partitions: its 24-byte record
+// (children_base and data index already global) and, with -levels, its averaged colour
+// and normal.
+struct PartRoot {
+  uint64_t part;
+  uint64_t rec[3];
+  float cache[6];
+};
+
+// Streamed (out-of-core) build, OctreeBuilder.cpp:4-168 driven per partition as
+// main.cpp:329-387 does: each partition's subtree is built alone and its records leave
+// (to the sink, or appended to out_nodes / out_data in HBM) before the next partition;
+// the partition roots, the levels above them and the root come last.
+struct StreamState {
+  bool active = false;
+  uint64_t gridsize = 0, n_parts = 0;
+  int levels = 0, payload = 0;
+  int depth = 0, part_depth = 0;  // log2 gridsize, log2 of a partition's edge
+  SinkFn sink = nullptr;
+  void* user = nullptr;
+  uint64_t next_part = 0;               // partitions below this one are done
+  uint64_t nodes_out = 0, data_out = 0; // records emitted so far
+  std::vector<PartRoot> roots;          // non-empty partitions, in order
+  int hazard = 0;
+  bool sink_failed = false;             // the sink returned non-zero
+  DevBuf out_nodes, out_data;           // the records, when there is no sink
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -188,6 +221,13 @@ struct Context {
   bool data_in_records2 = false;            // the built data records are in records2
   const DevBuf& data_out() const { return data_in_records2 ? records2 : records; }
 
+  StreamState st;
+  // streamed output: two pinned chunk buffers for the device-to-host copies that feed the
+  // sink, and their copy events
+  void* drain[2] = {nullptr, nullptr};
+  size_t drain_cap = 0;
+  hipEvent_t drain_ev[2] = {nullptr, nullptr};
+
   // streaming mesh ingest: two pinned chunk buffers and their copy events
   void* ingest[2] = {nullptr, nullptr};
   size_t ingest_cap = 0;
@@ -253,6 +293,17 @@ hipError_t ensure_codes(Context& c);
 // carries job tags (a batch). levels (-levels, OctreeBuilder.cpp:82-99) is supported for
 // a single job.
 hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels);
+
+// Streamed build (svo_stream.hip): begin (emits the leading data records), one call per
+// partition after its voxels were added (builds, emits and empties the voxel stream),
+// end (levels above the partitions, the root; the counts). Without a sink the records
+// collect in StreamState::out_nodes / out_data and end moves them to nodes / records.
+hipError_t stream_begin(Context& c, uint64_t gridsize, uint64_t n_parts, int levels,
+                        SinkFn sink, void* user);
+hipError_t stream_partition(Context& c, uint64_t part);
+hipError_t stream_end(Context& c, uint64_t* n_nodes, uint64_t* n_data, int* hazard);
+// Free the streamed build's output buffers and pinned chunks.
+void stream_release(Context& c);
 
 hipError_t fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s);

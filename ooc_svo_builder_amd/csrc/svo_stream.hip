@@ -1,0 +1,383 @@
+// Streamed (out-of-core) SVO build: the octree assembled partition by partition, the way
+// the reference's OctreeBuilder streams its records to disk while main.cpp feeds it one
+// Morton-range partition at a time (main.cpp:329-387, OctreeBuilder.cpp:4-168).
+//
+// Partition i is an aligned Morton cube, so its voxels form the subtree of one node Q_i
+// at depth log8(P). The streaming builder writes every node record of that subtree except
+// Q_i itself while partition i is fed, before anything of partition i+1; Q_i is written
+// when its group of 8 sibling partitions completes, with the levels above the
+// partitions, and the root comes last (OctreeBuilder.cpp:34-55, 58-79):
+//   nodes = [subtree(Q_0) - Q_0] [subtree(Q_1) - Q_1] ... [levels above partitions] root
+// Each partition is built alone by the level passes (svo_build.hip) over its own voxels
+// at the global gridsize. That tree is Q_i's subtree plus the chain of D - Dp single-child
+// ancestors up to the root, and in its post-order output Q_i sits at size(Q_i) - 1 with
+// the chain after it. Records [0, size(Q_i) - 1) are the partition's output once their
+// children_base moves by the node records emitted before, and their data index by the
+// data records emitted before (the 0 of internal nodes and the shared white record 1 of
+// geometry-only leaves stay).
+//
+// Data records follow the same order: payload voxel records in voxel order; with -levels
+// every record in post-order, so partition i contributes its subtree's records (Q_i's
+// average last) and the chain's records are dropped. The levels above the partitions are
+// at most a few thousand nodes (P = 8^k) and are grouped on the host exactly like the
+// reference's buffers group eight siblings (OctreeBuilder.cpp:58-128).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "svo_internal.h"
+
+namespace svo {
+
+#define SVO_TRY(x)                   \
+  do {                               \
+    hipError_t _e = (x);             \
+    if (_e != hipSuccess) return _e; \
+  } while (0)
+
+constexpr uint64_t kNoChildren = ~0ull;  // offsets[8] all -1: a leaf
+
+// dst[i] = src[i] with children_base moved by cb_add (nodes with children) and the data
+// index by data_add (indices from hdr up). src == dst rebases in place.
+__global__ __launch_bounds__(256) void k_rebase_nodes(const uint64_t* src, uint64_t* dst,
+                                                      uint64_t n, uint64_t cb_add,
+                                                      uint64_t data_add, uint64_t hdr) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    uint64_t d = src[3 * i];
+    uint64_t cb = src[3 * i + 1];
+    const uint64_t off = src[3 * i + 2];
+    if (off != kNoChildren) cb += cb_add;
+    if (d >= hdr) d += data_add;
+    dst[3 * i] = d;
+    dst[3 * i + 1] = cb;
+    dst[3 * i + 2] = off;
+  }
+}
+
+static void rebase_host(uint64_t* r, uint64_t cb_add, uint64_t data_add, uint64_t hdr) {
+  if (r[2] != kNoChildren) r[1] += cb_add;
+  if (r[0] >= hdr) r[0] += data_add;
+}
+
+static int ilog2(uint64_t v) {
+  int d = 0;
+  while ((1ull << d) < v) d++;
+  return d;
+}
+
+static hipError_t sink_error(Context& c) {
+  c.st.sink_failed = true;
+  set_error("the output sink stopped the build");
+  return hipErrorUnknown;
+}
+
+// Device bytes to the sink: chunks of whole records copied into two pinned buffers, the
+// copy of one chunk running while the sink consumes the other.
+static hipError_t drain_to_sink(Context& c, int kind, const void* dsrc, uint64_t bytes,
+                                uint64_t rec) {
+  const size_t cap = 64u << 20;
+  if (c.drain_cap < cap) {
+    for (int k = 0; k < 2; k++) {
+      if (c.drain[k]) (void)hipHostFree(c.drain[k]);
+      c.drain[k] = nullptr;
+    }
+    c.drain_cap = 0;
+    for (int k = 0; k < 2; k++) {
+      SVO_TRY(hipHostMalloc(&c.drain[k], cap, hipHostMallocDefault));
+      if (!c.drain_ev[k]) SVO_TRY(hipEventCreateWithFlags(&c.drain_ev[k], hipEventDisableTiming));
+    }
+    c.drain_cap = cap;
+  }
+  const uint64_t chunk = (c.drain_cap / rec) * rec;
+  uint64_t pend[2] = {0, 0};
+  const char* src = (const char*)dsrc;
+  uint64_t off = 0;
+  for (uint64_t k = 0; off < bytes; k++) {
+    const int b = (int)(k & 1);
+    if (pend[b]) {  // chunk k - 2: copied by now or soon; the sink takes it before reuse
+      SVO_TRY(hipEventSynchronize(c.drain_ev[b]));
+      if (c.st.sink(c.st.user, kind, c.drain[b], pend[b]) != 0) return sink_error(c);
+      pend[b] = 0;
+    }
+    const uint64_t n = std::min<uint64_t>(chunk, bytes - off);
+    SVO_TRY(hipMemcpyAsync(c.drain[b], src + off, n, hipMemcpyDeviceToHost, c.stream));
+    SVO_TRY(hipEventRecord(c.drain_ev[b], c.stream));
+    pend[b] = n;
+    off += n;
+  }
+  // the two chunks still held, older first
+  const uint64_t nk = (bytes + chunk - 1) / chunk;
+  for (uint64_t k = nk >= 2 ? nk - 2 : 0; k < nk; k++) {
+    const int b = (int)(k & 1);
+    if (!pend[b]) continue;
+    SVO_TRY(hipEventSynchronize(c.drain_ev[b]));
+    if (c.st.sink(c.st.user, kind, c.drain[b], pend[b]) != 0) return sink_error(c);
+    pend[b] = 0;
+  }
+  return hipSuccess;
+}
+
+// Host records (the leading data records, the levels above the partitions) to the
+// output: straight to the sink, or appended to the device arrays.
+static hipError_t emit_host(Context& c, int kind, const void* src, uint64_t bytes) {
+  if (!bytes) return hipSuccess;
+  StreamState& s = c.st;
+  if (s.sink) {
+    if (s.sink(s.user, kind, src, bytes) != 0) return sink_error(c);
+    return hipSuccess;
+  }
+  DevBuf& b = kind == 0 ? s.out_nodes : s.out_data;
+  const uint64_t used = kind == 0 ? s.nodes_out * 24 : s.data_out * 32;
+  SVO_TRY(ensure_keep(b, used + bytes, used, c.stream));
+  return upload_async(c, (char*)b.p + used, src, bytes);
+}
+
+static void reset_voxel_stream(Context& c) {
+  c.n_vox = 0;
+  c.n_bricks = 0;
+  c.bricks_ok = true;
+  c.codes_lazy = false;
+}
+
+hipError_t stream_begin(Context& c, uint64_t gridsize, uint64_t n_parts, int levels,
+                        SinkFn sink, void* user) {
+  StreamState& s = c.st;
+  s.active = true;
+  s.gridsize = gridsize;
+  s.n_parts = n_parts;
+  s.levels = levels ? 1 : 0;
+  s.payload = c.payload;
+  s.depth = ilog2(gridsize);
+  s.part_depth = s.depth - ilog2(n_parts) / 3;
+  s.sink = sink;
+  s.user = user;
+  s.next_part = 0;
+  s.nodes_out = s.data_out = 0;
+  s.roots.clear();
+  s.hazard = 0;
+  s.sink_failed = false;
+  reset_voxel_stream(c);
+  c.n_nodes = c.n_data = 0;
+  // OctreeBuilder.cpp:25-29: the null record, and for geometry-only builds the white
+  // record every leaf refers to
+  float hdr[16];
+  memset(hdr, 0, sizeof(hdr));
+  hdr[10] = hdr[11] = hdr[12] = 1.0f;
+  const uint64_t nh = s.payload ? 1 : 2;
+  SVO_TRY(emit_host(c, 1, hdr, nh * 32));
+  s.data_out = nh;
+  return hipSuccess;
+}
+
+hipError_t stream_partition(Context& c, uint64_t part) {
+  StreamState& s = c.st;
+  s.next_part = part + 1;
+  if (c.n_vox == 0) {  // no voxels: a null subtree, nothing written
+    reset_voxel_stream(c);
+    return hipSuccess;
+  }
+  Job& J = c.jobs[0];
+  J.gridsize = s.gridsize;
+  J.voff = 0;
+  J.nvox = c.n_vox;
+  SVO_TRY(build_jobs(c, 1, false, s.levels));
+  if (c.err_count) {
+    set_error("internal: " + std::to_string(c.err_count) +
+              " consistency failures in partition " + std::to_string(part) +
+              " (scan look-back time-outs, brick pool slots not found or payload voxels "
+              "without a covering triangle)");
+    return hipErrorUnknown;
+  }
+  s.hazard |= J.hazard;
+  const uint64_t chain = (uint64_t)(s.depth - s.part_depth);
+  if (c.n_nodes < chain + 1) {
+    set_error("internal: a partition's tree is smaller than its ancestor chain");
+    return hipErrorUnknown;
+  }
+  const uint64_t q = c.n_nodes - 1 - chain;  // Q's record; [0, q) is the partition's output
+  const uint64_t hdr = s.payload ? 1 : 2;
+  const uint64_t cb_add = s.nodes_out, data_add = s.data_out - hdr;
+  PartRoot pr;
+  memset(&pr, 0, sizeof(pr));
+  pr.part = part;
+  {
+    const ReadReq rq = {pr.rec, c.nodes.as<uint64_t>() + 3 * q, 24};
+    SVO_TRY(read_back(c, &rq, 1));
+  }
+  // -levels: Q's average (an internal node's record; a payload leaf's own voxel record;
+  // geometry-only leaves carry zero caches)
+  if (s.levels && (pr.rec[2] != kNoChildren || s.payload)) {
+    float rec[8];
+    const ReadReq rq = {rec, (const char*)c.data_out().p + pr.rec[0] * 32, 32};
+    SVO_TRY(read_back(c, &rq, 1));
+    memcpy(pr.cache, rec + 2, sizeof(pr.cache));
+  }
+  rebase_host(pr.rec, cb_add, data_add, hdr);
+  if (q) {
+    const unsigned grid = (unsigned)std::min<uint64_t>((q + 255) / 256, 16384);
+    if (s.sink) {
+      hipLaunchKernelGGL(k_rebase_nodes, dim3(grid), dim3(256), 0, c.stream,
+                         (const uint64_t*)c.nodes.as<uint64_t>(), c.nodes.as<uint64_t>(), q,
+                         cb_add, data_add, hdr);
+      SVO_TRY(hipGetLastError());
+      SVO_TRY(drain_to_sink(c, 0, c.nodes.p, q * 24, 24));
+    } else {
+      SVO_TRY(ensure_keep(s.out_nodes, (s.nodes_out + q) * 24, s.nodes_out * 24, c.stream));
+      hipLaunchKernelGGL(k_rebase_nodes, dim3(grid), dim3(256), 0, c.stream,
+                         (const uint64_t*)c.nodes.as<uint64_t>(),
+                         s.out_nodes.as<uint64_t>() + 3 * s.nodes_out, q, cb_add, data_add, hdr);
+      SVO_TRY(hipGetLastError());
+    }
+  }
+  uint64_t d0 = 0, dn = 0;
+  if (s.levels) {
+    d0 = hdr;
+    dn = c.n_data - hdr - chain;
+  } else if (s.payload) {
+    d0 = 1;
+    dn = c.n_vox;
+  }
+  if (dn) {
+    const char* src = (const char*)c.data_out().p + d0 * 32;
+    if (s.sink) {
+      SVO_TRY(drain_to_sink(c, 1, src, dn * 32, 32));
+    } else {
+      SVO_TRY(ensure_keep(s.out_data, (s.data_out + dn) * 32, s.data_out * 32, c.stream));
+      SVO_TRY(hipMemcpyAsync((char*)s.out_data.p + s.data_out * 32, src, dn * 32,
+                             hipMemcpyDeviceToDevice, c.stream));
+    }
+  }
+  s.nodes_out += q;
+  s.data_out += dn;
+  s.roots.push_back(pr);
+  reset_voxel_stream(c);
+  return hipSuccess;
+}
+
+namespace {
+struct TopNode {
+  bool null = true;
+  uint64_t rec[3] = {0, 0, kNoChildren};
+  float cache[6] = {0, 0, 0, 0, 0, 0};
+};
+inline float canon_nan_host(float f) {
+  if (f != f) {
+    const uint32_t u = 0xFFC00000u;
+    memcpy(&f, &u, 4);
+  }
+  return f;
+}
+}  // namespace
+
+hipError_t stream_end(Context& c, uint64_t* n_nodes, uint64_t* n_data, int* hazard) {
+  StreamState& s = c.st;
+  // the levels above the partitions: groups of 8 siblings from the partition roots up
+  // (OctreeBuilder.cpp:58-128); a group with a non-null member writes its non-null
+  // members consecutively, the parent points at the first one
+  std::vector<TopNode> lvl(s.n_parts);
+  for (const PartRoot& r : s.roots) {
+    TopNode& t = lvl[r.part];
+    t.null = false;
+    memcpy(t.rec, r.rec, sizeof(t.rec));
+    memcpy(t.cache, r.cache, sizeof(t.cache));
+  }
+  std::vector<uint64_t> nodes_h;  // 3 words per record
+  std::vector<float> data_h;      // 8 words per record
+  uint64_t npos = s.nodes_out, dpos = s.data_out;
+  while (lvl.size() > 1) {
+    std::vector<TopNode> up(lvl.size() / 8);
+    for (size_t g = 0; g < up.size(); g++) {
+      const TopNode* ch = &lvl[8 * g];
+      bool any = false;
+      for (int k = 0; k < 8; k++) any = any || !ch[k].null;
+      if (!any) continue;
+      TopNode& p = up[g];
+      p.null = false;
+      uint64_t cb = 0, offs = 0;
+      bool first = true;
+      for (int k = 0; k < 8; k++) {
+        uint64_t byte = 0xFF;
+        if (!ch[k].null) {
+          const uint64_t pos = npos++;
+          nodes_h.insert(nodes_h.end(), ch[k].rec, ch[k].rec + 3);
+          if (first) cb = pos;
+          first = false;
+          byte = pos - cb;
+        }
+        offs |= byte << (8 * k);
+      }
+      p.rec[0] = 0;
+      p.rec[1] = cb;
+      p.rec[2] = offs;
+      if (s.levels) {  // OctreeBuilder.cpp:82-99, the float order of average_children
+        float a[6] = {0, 0, 0, 0, 0, 0};
+        float notnull = 0.0f;
+        for (int k = 0; k < 8; k++) {
+          if (!ch[k].null) notnull += 1.0f;
+          for (int q = 0; q < 6; q++) a[q] = a[q] + ch[k].cache[q];
+        }
+        float avg[6];
+        avg[0] = a[0] / notnull;
+        avg[1] = a[1] / notnull;
+        avg[2] = a[2] / notnull;
+        const float t0 = a[3] / notnull, t1 = a[4] / notnull, t2 = a[5] / notnull;
+        const float inv = 1.0f / sqrtf((t0 * t0 + t1 * t1) + t2 * t2);
+        avg[3] = t0 * inv;
+        avg[4] = t1 * inv;
+        avg[5] = t2 * inv;
+        float rec[8];
+        memset(rec, 0, 8);  // morton 0
+        for (int q = 0; q < 6; q++) rec[2 + q] = canon_nan_host(avg[q]);
+        data_h.insert(data_h.end(), rec, rec + 8);
+        p.rec[0] = dpos++;
+        memcpy(p.cache, avg, sizeof(avg));
+      }
+    }
+    lvl.swap(up);
+  }
+  nodes_h.insert(nodes_h.end(), lvl[0].rec, lvl[0].rec + 3);  // the root, last (:42)
+  SVO_TRY(emit_host(c, 0, nodes_h.data(), nodes_h.size() * 8));
+  SVO_TRY(emit_host(c, 1, data_h.data(), data_h.size() * 4));
+  const uint64_t nn = s.nodes_out + nodes_h.size() / 3;
+  const uint64_t nd = s.data_out + data_h.size() / 8;
+  // an empty 1^3 grid: the reference reads a root buffer that was never filled
+  if (s.gridsize == 1 && s.roots.empty()) s.hazard = 1;
+  s.active = false;
+  if (s.sink) {
+    SVO_TRY(hipStreamSynchronize(c.stream));
+    c.n_nodes = c.n_data = 0;  // nothing of the octree is held on the device
+  } else {
+    std::swap(c.nodes, s.out_nodes);
+    std::swap(c.records, s.out_data);
+    c.data_in_records2 = false;
+    c.n_nodes = nn;
+    c.n_data = nd;
+  }
+  c.hazard = s.hazard;
+  Job& J = c.jobs[0];
+  J.n_nodes = nn;
+  J.n_data = nd;
+  J.nbase = J.dbase = 0;
+  J.hazard = s.hazard;
+  if (n_nodes) *n_nodes = nn;
+  if (n_data) *n_data = nd;
+  if (hazard) *hazard = s.hazard;
+  return hipSuccess;
+}
+
+void stream_release(Context& c) {
+  release(c.st.out_nodes);
+  release(c.st.out_data);
+  for (int k = 0; k < 2; k++) {
+    if (c.drain[k]) (void)hipHostFree(c.drain[k]);
+    if (c.drain_ev[k]) (void)hipEventDestroy(c.drain_ev[k]);
+    c.drain[k] = nullptr;
+    c.drain_ev[k] = nullptr;
+  }
+  c.drain_cap = 0;
+}
+
+}  // namespace svo

@@ -174,36 +174,56 @@ def terrain(n: int = 1025, seed: int = 2, payload: bool = True) -> np.ndarray:
 # ----------------------------------------------------------------------------
 
 
+def _soup_chunk(args):
+    """Chunk k of a sphere soup: its own generator, seeded by (seed, k)."""
+    seed, k, m, centers, radii, cdf, median_edge, sigma = args
+    rng = np.random.default_rng([seed, k])
+    n_spheres = len(radii)
+    sid = np.minimum(np.searchsorted(cdf, rng.uniform(size=m)), n_spheres - 1)
+    d = rng.normal(size=(m, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    ctr = centers[sid] + d * radii[sid, None]
+    # tangent basis
+    helper = np.where(np.abs(d[:, :1]) < 0.9, np.array([[1.0, 0, 0]]), np.array([[0, 1.0, 0]]))
+    t1 = np.cross(d, helper)
+    t1 /= np.linalg.norm(t1, axis=1, keepdims=True)
+    t2 = np.cross(d, t1)
+    edge = median_edge * np.exp(sigma * rng.normal(size=m))
+    theta = rng.uniform(0.0, 2.0 * np.pi, size=m)
+    rad = edge / np.sqrt(3.0)
+    out = np.empty((m, 9), dtype=np.float32)
+    for k3 in range(3):
+        ang = theta + 2.0 * np.pi * k3 / 3.0
+        v = ctr + rad[:, None] * (np.cos(ang)[:, None] * t1 + np.sin(ang)[:, None] * t2)
+        out[:, 3 * k3:3 * k3 + 3] = v.astype(np.float32)
+    return out
+
+
 def sphere_soup(n_tris: int, n_spheres: int = 200, seed: int = 3,
                 median_edge: float = 4.0 / 8192.0, sigma: float = 1.0,
-                chunk: int = 1 << 22) -> np.ndarray:
+                chunk: int = 1 << 20, workers: int = 1) -> np.ndarray:
     """Equilateral triangles centred on the surfaces of seeded spheres (centres
     U[0.1,0.9]^3, radii log-uniform in [0.01, 0.2]); sphere picked by area, edge
-    length log-normal, random in-plane rotation. Returns (N, 9) float32."""
+    length log-normal (median median_edge, sigma), random in-plane rotation. Returns
+    (N, 9) float32. Chunks of 'chunk' triangles draw from their own seeded generators,
+    so 'workers' processes give the same soup as one."""
     rng = np.random.default_rng(seed)
     centers = rng.uniform(0.1, 0.9, size=(n_spheres, 3))
     radii = np.exp(rng.uniform(np.log(0.01), np.log(0.2), size=n_spheres))
     w = radii ** 2
     cdf = np.cumsum(w / w.sum())
+    jobs = [(seed, k, min(chunk, n_tris - s), centers, radii, cdf, median_edge, sigma)
+            for k, s in enumerate(range(0, n_tris, chunk))]
     out = np.empty((n_tris, 9), dtype=np.float32)
-    for s in range(0, n_tris, chunk):
-        m = min(chunk, n_tris - s)
-        sid = np.minimum(np.searchsorted(cdf, rng.uniform(size=m)), n_spheres - 1)
-        d = rng.normal(size=(m, 3))
-        d /= np.linalg.norm(d, axis=1, keepdims=True)
-        ctr = centers[sid] + d * radii[sid, None]
-        # tangent basis
-        helper = np.where(np.abs(d[:, :1]) < 0.9, np.array([[1.0, 0, 0]]), np.array([[0, 1.0, 0]]))
-        t1 = np.cross(d, helper)
-        t1 /= np.linalg.norm(t1, axis=1, keepdims=True)
-        t2 = np.cross(d, t1)
-        edge = median_edge * np.exp(sigma * rng.normal(size=m))
-        theta = rng.uniform(0.0, 2.0 * np.pi, size=m)
-        rad = edge / np.sqrt(3.0)
-        for k in range(3):
-            ang = theta + 2.0 * np.pi * k / 3.0
-            v = ctr + rad[:, None] * (np.cos(ang)[:, None] * t1 + np.sin(ang)[:, None] * t2)
-            out[s:s + m, 3 * k:3 * k + 3] = v.astype(np.float32)
+    if workers > 1 and len(jobs) > 1:
+        from concurrent.futures import ProcessPoolExecutor
+        with ProcessPoolExecutor(min(workers, len(jobs))) as ex:
+            parts = ex.map(_soup_chunk, jobs)
+            for k, part in enumerate(parts):
+                out[k * chunk:k * chunk + len(part)] = part
+    else:
+        for k, j in enumerate(jobs):
+            out[k * chunk:k * chunk + j[2]] = _soup_chunk(j)
     return out
 
 

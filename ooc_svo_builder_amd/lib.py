@@ -26,6 +26,12 @@ SVO_ERR_INVALID = 1
 SVO_ERR_DEVICE = 2
 SVO_ERR_NOMEM = 3
 SVO_ERR_UNSUPPORTED = 4
+SVO_ERR_IO = 5
+
+SINK_NODES = 0
+SINK_DATA = 1
+# svo_sink_fn: int (*)(void* user, int kind, const void* bytes, uint64_t size)
+SINK_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_uint64)
 
 COLOR_MODES = {"model": 0, "fixed": 1, "linear": 2, "normal": 3}
 
@@ -89,6 +95,11 @@ SIGNATURES = {
     "svo_kernel_timing_reset": (_i32, [_vp]),
     "svo_run": (_i32, [_vp, _vp, _u64, _i32, _i32, _vp, _u64, _u64, _i32, _i32,
                        C.POINTER(RunInfo)]),
+    "svo_stream_begin": (_i32, [_vp, _u64, _u64, _i32, _vp, _vp]),
+    "svo_stream_partition": (_i32, [_vp, _u64]),
+    "svo_stream_end": (_i32, [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_i32)]),
+    "svo_run_stream": (_i32, [_vp, _vp, _u64, _i32, _i32, _vp, _u64, _u64, _i32, _i32, _vp,
+                              _vp, C.POINTER(RunInfo)]),
     "svo_batch_run": (_i32, [_vp, C.POINTER(Job), C.c_uint32, C.POINTER(JobResult),
                              C.POINTER(RunInfo)]),
     "svo_ctx_autotune": (_i32, [_vp, C.POINTER(Job), C.c_uint32, C.POINTER(C.c_int * 4)]),
@@ -349,6 +360,54 @@ class SvoContext:
         self.payload = bool(payload)
         self.last_info = info
         return info
+
+    def run_stream(self, records, payload: bool, gridsize: int, sink, memory_limit_mb: int = 2048,
+                   bbox=(0, 0, 0, 1, 1, 1), color_mode: int = 0, levels: bool = False) -> RunInfo:
+        """main.cpp's flow with the streamed build: sink(kind, memoryview) receives the
+        .octreenodes (kind 0) and .octreedata (kind 1) bytes in order and returns nothing
+        (an exception stops the build). records=None reruns on the mesh in HBM."""
+        L = load_library()
+        b = np.asarray(bbox, np.float32)
+        info = RunInfo()
+        errors = []
+
+        def _cb(user, kind, ptr, size):
+            try:
+                sink(kind, (C.c_char * size).from_address(ptr))
+                return 0
+            except Exception as e:  # reported after the call
+                errors.append(e)
+                return 1
+        cb = SINK_FN(_cb)
+        if records is None:
+            args = (None, 0, int(payload), 0)
+        elif hasattr(records, "data_ptr"):
+            args = (C.c_void_p(records.data_ptr()), int(records.shape[0]), int(payload),
+                    1 if records.is_cuda else 0)
+        else:
+            r = np.ascontiguousarray(records, np.float32)
+            args = (_ptr(r), r.shape[0], int(payload), 0)
+        rc = L.svo_run_stream(self._h, *args, _ptr(b), gridsize, memory_limit_mb, color_mode,
+                              int(levels), C.cast(cb, C.c_void_p), None, C.byref(info))
+        if errors:
+            raise errors[0]
+        _check(rc)
+        self.payload = bool(payload)
+        self.last_info = info
+        return info
+
+    def stream_begin(self, gridsize: int, n_partitions: int, levels: bool = False):
+        """Streamed build with its records collected in HBM (no sink)."""
+        _check(load_library().svo_stream_begin(self._h, gridsize, n_partitions, int(levels),
+                                               None, None))
+
+    def stream_partition(self, part: int):
+        _check(load_library().svo_stream_partition(self._h, part))
+
+    def stream_end(self):
+        nn, nd, hz = C.c_uint64(0), C.c_uint64(0), C.c_int(0)
+        _check(load_library().svo_stream_end(self._h, C.byref(nn), C.byref(nd), C.byref(hz)))
+        return int(nn.value), int(nd.value), int(hz.value)
 
     def rerun(self, payload: bool, gridsize: int, memory_limit_mb: int = 2048,
               bbox=(0, 0, 0, 1, 1, 1), color_mode: int = 0, levels: bool = False) -> RunInfo:

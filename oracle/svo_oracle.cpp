@@ -40,6 +40,7 @@
 #include <iostream>
 #include <sstream>
 #include <string>
+#include <memory>
 #include <vector>
 
 namespace orc {
@@ -947,7 +948,8 @@ void* oracle_pipeline(const float* tris, size_t n, int payload, const float bbox
   std::vector<std::vector<uint64_t>> lists;
   if (np > 1) lists = partition_lists(tris, n, pl, u_part, gridsize, np);
   const uint64_t morton_part = (gridsize * gridsize * gridsize) / np;
-  std::vector<uint8_t> voxels((size_t)morton_part);
+  // main.cpp:314 allocates the grid uninitialised; voxelize clears it per partition
+  std::unique_ptr<uint8_t[]> voxels(new uint8_t[(size_t)morton_part]);
   BuildResult* r = new BuildResult();
   Builder b(&r->sink, gridsize, pl, levels != 0);
   size_t nfilled = 0;
@@ -964,7 +966,7 @@ void* oracle_pipeline(const float* tris, size_t n, int payload, const float bbox
     }
     if (cnt == 0) continue;  // main.cpp:330
     run_partition(b, src, cnt, pl, i * morton_part, (i + 1) * morton_part, u_vox,
-                  voxels.data(), sparseness_limit, color_mode, gridsize, nfilled);
+                  voxels.get(), sparseness_limit, color_mode, gridsize, nfilled);
   }
   b.finalize();
   r->hazard = b.hazard();
@@ -1130,7 +1132,7 @@ int oracle_cli_main(int argc, char** argv, int payload) {
   if (parse_trip_header(tp.base + ".trip", tr) != 1) return 0;
   const float unitlength = (tr.bbox[3] - tr.bbox[0]) / (float)tr.gridsize;
   const uint64_t morton_part = (tr.gridsize * tr.gridsize * tr.gridsize) / tr.n_partitions;
-  std::vector<uint8_t> voxels((size_t)morton_part);
+  std::unique_ptr<uint8_t[]> voxels(new uint8_t[(size_t)morton_part]);  // main.cpp:314
 
   BuildResult r;
   r.sink.fn = std::fopen((tr.base + ".octreenodes").c_str(), "wb");
@@ -1151,7 +1153,7 @@ int oracle_cli_main(int argc, char** argv, int payload) {
     }
     cout << "Building SVO for partition " << i << " ..." << endl;
     run_partition(b, part.data(), tr.counts[i], pl, i * morton_part, (i + 1) * morton_part,
-                  unitlength, voxels.data(), sparseness, color, tr.gridsize, nfilled);
+                  unitlength, voxels.get(), sparseness, color, tr.gridsize, nfilled);
   }
   b.finalize();
   if (b.hazard())

@@ -1,0 +1,134 @@
+"""GPU parity at the boundary behaviours of main.cpp and the partitioner.
+
+* The .trip text round trip (main.cpp:306-311, trip_tools.h:112): the voxelizer's
+  unitlength comes from the bbox printed with 6 significant digits and parsed back, the
+  partitioner's from the .tri values (partitioner.cpp:45). A bbox of 0.77777779 makes
+  the two differ.
+* svo_partition / svo_partition_indices against the oracle's partition
+  (partitioner.cpp:101-149, intersection.h:49-53): per-partition counts (the .trip
+  header) and in-order triangle lists (the .tripdata order), with triangles on the
+  partition planes, NaN coordinates and triangles outside the grid.
+"""
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from ooc_svo_builder_amd import CLI_BINARY, CLI_PAYLOAD, generators
+from ooc_svo_builder_amd.formats import files_identical
+
+pytestmark = pytest.mark.gpu
+
+BMAX = np.float32(0.77777779)
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _text6(v):
+    return np.float32(float("%g" % float(v)))
+
+
+def test_roundtrip_changes_unitlength():
+    """The case is real: the two unitlengths differ in float32 (checked on the CPU too)."""
+    G = np.float32(128)
+    assert BMAX / G != _text6(BMAX) / G
+
+
+@pytest.mark.parametrize("payload,G,L", [(False, 128, 2048), (True, 128, 2048),
+                                         (False, 256, 2), (True, 256, 2)])
+def test_run_trip_roundtrip_bbox(gpu_ctx, payload, G, L):
+    _say("trip round trip payload=%d G=%d L=%d" % (payload, G, L))
+    tris = generators.terrain(n=65, seed=21, payload=payload).copy()
+    tris[:, :9] *= BMAX
+    bbox = (0.0, 0.0, 0.0, float(BMAX), float(BMAX), float(BMAX))
+    info = gpu_ctx.run(tris, payload, G, L, bbox=bbox)
+    nodes_w, data_w, hz_w, np_w, nf_w = oracle.pipeline(tris, payload, G, L, bbox=bbox)
+    assert (info.n_partitions, info.nfilled, info.hazard) == (np_w, nf_w, hz_w)
+    assert gpu_ctx.nodes(info.n_nodes).tobytes() == nodes_w.tobytes()
+    assert gpu_ctx.data(info.n_data).tobytes() == data_w.tobytes()
+
+
+def test_cli_trip_roundtrip_bbox(tmp_path):
+    """The CLIs read a .tri header whose bbox has more than 6 significant digits."""
+    _say("cli trip round trip")
+    cases = [("b", False, ["-s", "128", "-l", "2048"]), ("p", True, ["-s", "256", "-l", "2"])]
+    for name, payload, args in cases:
+        tris = generators.terrain(n=49, seed=22, payload=payload).copy()
+        tris[:, :9] *= BMAX
+        for who in ("gpu", "ref"):
+            d = tmp_path / who
+            d.mkdir(exist_ok=True)
+            base = str(d / name)
+            with open(base + ".tri", "w") as f:
+                f.write("#tri 1\nntriangles %d\ngeo_only %d\n" % (len(tris), 0 if payload else 1))
+                f.write("bbox  0 0 0 0.77777779 0.77777779 0.77777779\nEND\n")
+            np.ascontiguousarray(tris, "<f4").tofile(base + ".tridata")
+            exe = (CLI_PAYLOAD if payload else CLI_BINARY) if who == "gpu" else (
+                oracle.CLI_PAYLOAD if payload else oracle.CLI_BINARY)
+            r = subprocess.run([exe, "-f", base + ".tri"] + args, capture_output=True,
+                               text=True, timeout=120)
+            assert r.returncode == 0, r.stdout + r.stderr
+        P = oracle.estimate_partitions(int(args[1]), int(args[3]))
+        out = "%s%s_%d" % (name, args[1], P)
+        assert files_identical(str(tmp_path / "gpu" / out), str(tmp_path / "ref" / out)), name
+
+
+def _partition_soup(G, P):
+    """Triangles on the partition planes, NaN coordinates, outside the grid, plus a
+    random soup; in [0,1]^3 world units."""
+    rng = np.random.default_rng(G + P)
+    k = round(P ** (1 / 3))
+    planes = np.arange(k + 1) / k  # the partition boundaries along each axis
+    parts = [generators.icosphere(subdivisions=3, radius=0.45).reshape(-1, 3, 3)]
+    c = rng.uniform(0, 1, (600, 1, 3)) + rng.normal(0, 0.01, (600, 3, 3))
+    parts.append(c)
+    # a vertex (or all three) exactly on a boundary plane
+    b = rng.uniform(0, 1, (300, 3, 3))
+    ax = rng.integers(0, 3, 300)
+    pl = planes[rng.integers(0, k + 1, 300)]
+    for i in range(300):
+        if i % 2:
+            b[i, :, ax[i]] = pl[i]
+        else:
+            b[i, 0, ax[i]] = pl[i]
+    parts.append(b)
+    # bboxes touching a boundary from one side only
+    e = rng.uniform(0, 1, (100, 3, 3)) * 0.02
+    e[:, :, 0] += 0.5 - 0.02
+    e[:, 0, 0] = 0.5
+    parts.append(e)
+    # outside the grid, straddling the grid faces
+    parts.append(rng.uniform(-0.5, 1.5, (100, 3, 3)))
+    soup = np.concatenate(parts).reshape(-1, 9).astype(np.float32)
+    # NaN in one coordinate, in all coordinates, and +-inf
+    nan = soup[:40].copy()
+    nan[:10, 0] = np.nan
+    nan[10:20, 4] = np.nan
+    nan[20:30, :] = np.nan
+    nan[30:35, 2] = np.inf
+    nan[35:40, 5] = -np.inf
+    return np.concatenate([soup, nan])
+
+
+@pytest.mark.parametrize("G,P,payload", [(256, 8, False), (512, 64, False), (64, 8, True),
+                                         (128, 512, False)])
+def test_partition_lists_match_oracle(gpu_ctx, G, P, payload):
+    _say("partition lists G=%d P=%d payload=%d" % (G, P, payload))
+    tris = _partition_soup(G, P)
+    if payload:
+        pad = np.zeros((len(tris), 12), np.float32)
+        tris = np.concatenate([tris, pad], axis=1)
+    gpu_ctx.load_mesh(tris, payload)
+    counts = gpu_ctx.partition(np.float32(1.0) / np.float32(G), G, P)
+    want_counts, want_lists = oracle.partition(tris, payload, (0, 0, 0, 1, 1, 1), G, P)
+    assert np.array_equal(counts, want_counts.astype(np.uint64))
+    seen = np.zeros(len(tris), np.int64)
+    for p in range(P):
+        got = gpu_ctx.partition_indices(p, int(counts[p]))
+        assert np.array_equal(got.astype(np.uint64), want_lists[p]), p
+        seen[got] += 1
+    # boundary triangles land in several partitions, triangles outside the grid in none
+    assert seen.max() >= 2 and (seen == 0).any()
