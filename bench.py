@@ -10,15 +10,20 @@ partition). Prints one JSON line.
 The CPU baseline is the single-threaded CPU restatement of the reference algorithm
 (oracle/), timed on the same host on the same workload.
 
-Two more sections report the batched and served forms on a 512-mesh mix (icosphere /
-terrain 50/50, 1k-2M triangles log-uniform, gridsize 64-1024, 30% payload; meshes
-resident in HBM):
-  * batched: the whole mix in one svo_batch_run against a plain per-mesh loop of
-    svo_run, every job's node and data bytes compared between the two;
-  * served: open-loop Poisson arrivals (50-6000 jobs/s, plus 2% large jobs: a 2049^2
-    terrain, 8,388,608 triangles, payload, gridsize 2048) into svo_server with a 200 GB
-    admission budget; p50/p99/p999 latency and jobs/s, next to the plain loop's latency
-    under the same arrivals (a FIFO queue served at the measured per-job plain times).
+More sections:
+  * cli: the drop-in CLI end to end (.tri/.tridata file in -> .octree* files out) on the
+    headline mesh, next to the oracle CLI on the same file (one core), files compared;
+  * out_of_core: the 5x10^7-triangle sphere soup at -s 8192 -l 65536 (8 partitions of
+    4096^3), streamed build with the records collected in HBM;
+  * batched and served, on a 512-mesh mix (icosphere / terrain 50/50, 1k-2M triangles
+    log-uniform, gridsize 64-1024, 30% payload; meshes resident in HBM):
+      - batched: the whole mix in one svo_batch_run against a plain per-mesh loop of
+        svo_run, every job's node and data bytes compared between the two;
+      - served: open-loop Poisson arrivals (50-6000 jobs/s, plus 2% large jobs: a 2049^2
+        terrain, 8,388,608 triangles, payload, gridsize 2048) into svo_server with a
+        200 GB admission budget; p50/p99/p999 latency and jobs/s, next to a plain loop
+        (svo_run, one job after another in arrival order) measured under the same
+        arrivals, and a sample of the served results byte-compared with the plain path.
 """
 import argparse
 import json
@@ -41,38 +46,76 @@ WORKLOADS = {
                      "icosphere subdiv 5 (20,480 tris), binary, -s 256 -l 2048"),
     "terrain1024p": ("terrain", True, 1024, 2048,
                      "fBm terrain 1025x1025 vertices (2,097,152 tris), payload, -s 1024 -l 2048"),
+    "soup8192": ("soup", False, 8192, 65536,
+                 "sphere soup 5e7 tris (200 spheres, edges log-normal median 4 voxels), "
+                 "binary, -s 8192 -l 65536 (8 partitions)"),
 }
 
+# MI355X: 256 CUs at 2.4 GHz; a wave64 VALU instruction issues at one per CU per cycle
+# (4 SIMDs, 4 cycles each), so the non-packed VALU issue peak is 256 * 2.4e9 per second
+VALU_ISSUE_PEAK = 256 * 2.4e9
 
-TRAFFIC_FILES = {"terrain2048": "hbm_traffic_terrain2048_binary.md"}
+COUNTER_FILES = {"terrain2048": "hbm_traffic_terrain2048_binary.md"}
 
 
-def measured_traffic(workload, timer):
-    """(HBM bytes per launch, SIMD VALU issue share) of the kernel behind 'timer' from the
-    committed rocprofv3 counter summary of this workload (None where there is none)."""
-    name = TRAFFIC_FILES.get(workload)
+def _git_blob_sha(path):
+    import hashlib
+    data = open(path, "rb").read()
+    return hashlib.sha1(b"blob %d\0" % len(data) + data).hexdigest()
+
+
+def measured_counters(workload, timer):
+    """From the committed rocprofv3 counter summary of this workload, for the kernel behind
+    'timer': HBM bytes per launch, VALU instructions per launch, the SIMD VALU issue
+    share, the summary's file name and whether the kernel sources it names are unchanged
+    since (their git blob hashes)."""
+    out = {"traffic": None, "valu_insts": None, "valu_share": None, "from": None,
+           "current": None}
+    name = COUNTER_FILES.get(workload)
     path = os.path.join(ROOT, "profiles", name) if name else None
     if not path or not os.path.exists(path):
-        return None, None
-    traffic = share = None
+        return out
+    out["from"] = "profiles/" + name
     kernel = None
+    current = True
     for line in open(path):
+        if line.startswith("sources:"):
+            for item in line.split(":", 1)[1].split(","):
+                src, _, sha = item.strip().partition(" ")
+                fp = os.path.join(ROOT, src)
+                current = current and os.path.exists(fp) and _git_blob_sha(fp) == sha.strip()
+            out["current"] = current
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         try:
             if len(cells) == 8 and cells[1] == timer:
                 kernel = cells[0]
-                traffic = float(cells[7]) * 1e6
+                out["traffic"] = float(cells[7]) * 1e6
             elif len(cells) == 9 and kernel and cells[0] == kernel and cells[8] != "nan":
-                share = float(cells[8])
+                out["valu_insts"] = float(cells[2])
+                out["valu_share"] = float(cells[8])
         except ValueError:
             pass
-    return traffic, share
+    return out
+
+
+def host_cpu():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count(), len(os.sched_getaffinity(0))
 
 
 def make_mesh(kind, payload):
     from ooc_svo_builder_amd import generators
     if kind == "terrain":
         return generators.terrain(n=1025, seed=2, payload=payload)
+    if kind == "soup":
+        return generators.sphere_soup(50_000_000, workers=max(1, min(16, os.cpu_count() or 1)))
     return generators.icosphere()
 
 
@@ -85,7 +128,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched section")
     ap.add_argument("--no-served", action="store_true", help="skip the served section")
-    ap.add_argument("--only", choices=["headline", "batched", "served"],
+    ap.add_argument("--no-cli", action="store_true", help="skip the CLI end-to-end section")
+    ap.add_argument("--no-ooc", action="store_true", help="skip the out-of-core section")
+    ap.add_argument("--only", choices=["headline", "batched", "served", "cli", "ooc"],
                     help="run one section only (profiling)")
     ap.add_argument("--knobs", default=None,
                     help="voxelizer knobs small_cols,direct_cols,tile_blocks,large_blocks "
@@ -99,13 +144,15 @@ def main():
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy()
-    if args.only in ("batched", "served"):
+    if args.only in ("batched", "served", "cli", "ooc"):
         args.no_cpu = True
         args.no_batched = args.only != "batched"
         args.no_served = args.only != "served"
+        args.no_cli = args.only != "cli"
+        args.no_ooc = args.only != "ooc"
         args.steps, args.warmup = 1, 0
     elif args.only == "headline":
-        args.no_batched = args.no_served = True
+        args.no_batched = args.no_served = args.no_cli = args.no_ooc = True
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(the multi-GPU sharding is a design, not a measured path)" % args.gpus,
@@ -199,7 +246,12 @@ def main():
         algo_bytes_per_launch = build_bytes / per_step_launches
     avg_launch_ms = dom_ms / launches
     achieved = algo_bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
-    traffic, valu_share = measured_traffic(args.workload, dom)
+    cnt = measured_counters(args.workload, dom)
+    valu_issue = (cnt["valu_insts"] / (avg_launch_ms * 1e-3 * VALU_ISSUE_PEAK)
+                  if cnt["valu_insts"] else None)
+    # the binding roof: VALU issue when the kernel's SIMDs spend most cycles issuing
+    # vector instructions (SQ counters), else HBM
+    bound = "valu" if (cnt["valu_share"] or 0) > 0.8 else "hbm"
 
     value = N * args.steps / (ms_total * 1e-3)
     stage_ms = {k: v / args.steps for k, v in st.items()}
@@ -230,28 +282,58 @@ def main():
         "kernels_ms_per_step": {k: v[1] / kt_steps for k, v in ktimes.items()},
         "knobs": dict(zip(("small_cols", "direct_cols", "tile_blocks", "large_blocks"),
                           ctx.get_tuning())),
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": bound, "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "simd_valu_share": valu_share,
+                     "traffic": cnt["traffic"], "hbm_frac": achieved / HBM_PEAK_GBS,
+                     "valu_issue_frac": valu_issue, "simd_valu_share": cnt["valu_share"],
+                     "valu_insts_per_launch": cnt["valu_insts"],
+                     "counters_from": cnt["from"], "counters_current": cnt["current"],
                      "avg_launch_ms": avg_launch_ms,
                      "algo_bytes_per_launch": algo_bytes_per_launch},
+        "launches_per_step": int(info.launches), "host_syncs_per_step": int(info.host_syncs),
         "wall_s_timed": wall,
     }
 
     if not args.no_cpu:
         import oracle
-        t = time.perf_counter()
-        nodes_w, data_w, hz_w, np_w, nf_w = oracle.pipeline(tris, payload, G, L)
-        cpu_s = time.perf_counter() - t
+        model, ncpu, naff = host_cpu()
+        # one core, like the reference's single-threaded build: this thread pinned to
+        # one CPU of the ones it may use
+        keep = os.sched_getaffinity(0)
+        core = min(keep)
+        os.sched_setaffinity(0, {core})
+        try:
+            t = time.perf_counter()
+            nodes_w, data_w, hz_w, np_w, nf_w = oracle.pipeline(tris, payload, G, L)
+            cpu_s = time.perf_counter() - t
+        finally:
+            os.sched_setaffinity(0, keep)
         # same workload, so the CPU result doubles as a parity check of this run
-        same = (nf_w == V and len(nodes_w) == n_nodes and
-                np.array_equal(ctx.nodes(n_nodes), nodes_w))
+        same = (nf_w == V and len(nodes_w) == n_nodes and len(data_w) == n_data and
+                np.array_equal(ctx.nodes(n_nodes), nodes_w) and
+                np.array_equal(ctx.data(n_data), data_w))
         out["cpu_baseline"] = {"value": N / cpu_s, "unit": "triangles/s", "cores": 1,
                                "kind": "port",
-                               "sample": "the full workload once (%d triangles), %.1f s" % (N, cpu_s),
+                               "sample": "the full workload once (%d triangles), %.1f s, "
+                                         "pinned to CPU %d" % (N, cpu_s, core),
+                               "host_cpu": model, "host_nproc": ncpu,
+                               "host_cpus_allowed": naff,
                                "gpu_speedup": value / (N / cpu_s),
                                "bit_exact_vs_gpu": bool(same)}
     ctx.close()
+    del d_tris
+    torch.cuda.empty_cache()
+
+    if not args.no_cli and args.workload == "terrain2048":
+        try:
+            out["cli"] = cli_end_to_end(tris, G, L)
+        except Exception as e:  # the headline line is printed regardless
+            out["cli"] = {"error": repr(e)}
+    if not args.no_ooc and args.workload != "soup8192":
+        try:
+            out["out_of_core"] = out_of_core(max(2, min(args.steps, 3)))
+        except Exception as e:
+            out["out_of_core"] = {"error": repr(e)}
 
     mix = None
     if not (args.no_batched and args.no_served):
@@ -286,6 +368,101 @@ def calibrate_copy():
     torch.cuda.synchronize()
     print(json.dumps({"calibration": "copy", "bytes_per_launch": n, "launches": 5}), flush=True)
     return 0
+
+
+def cli_end_to_end(tris, G, L):
+    """The drop-in CLI on the headline mesh: .tri/.tridata in, .octree* out (wall clock,
+    process start and device set-up included), twice; the oracle CLI on the same file,
+    pinned to one core; the output files compared."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    import oracle
+    from ooc_svo_builder_amd import CLI_BINARY, write_tri
+    from ooc_svo_builder_amd.formats import files_identical
+    d = tempfile.mkdtemp(prefix="svo_cli_")
+    try:
+        res = {"mesh_bytes": int(tris.nbytes)}
+        for who in ("gpu", "ref"):
+            os.makedirs(os.path.join(d, who))
+            tri = write_tri(os.path.join(d, who, "terrain"), tris, False)
+            cmd = [CLI_BINARY, "-f", tri, "-s", str(G), "-l", str(L)]
+            runs = 2
+            if who == "ref":
+                cmd = ["taskset", "-c", str(min(os.sched_getaffinity(0))),
+                       oracle.CLI_BINARY] + cmd[1:]
+                runs = 1
+            times = []
+            for _ in range(runs):
+                t = time.perf_counter()
+                r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+                times.append(time.perf_counter() - t)
+                if r.returncode != 0:
+                    raise RuntimeError("%s CLI failed: %s" % (who, r.stderr[-500:]))
+            res[who + "_s"] = times
+        from ooc_svo_builder_amd import estimate_partitions
+        base = "terrain%d_%d" % (G, estimate_partitions(G, L))
+        res["identical"] = files_identical(os.path.join(d, "gpu", base),
+                                           os.path.join(d, "ref", base))
+        res["nodes_bytes"] = os.path.getsize(os.path.join(d, "gpu", base + ".octreenodes"))
+        res["speedup_vs_oracle_cli"] = res["ref_s"][0] / min(res["gpu_s"])
+        _log("cli: gpu %s s, oracle %.1f s, identical %s" % (
+            ["%.2f" % x for x in res["gpu_s"]], res["ref_s"][0], res["identical"]))
+        return res
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def out_of_core(steps):
+    """The 5x10^7-triangle sphere soup at -s 8192 -l 65536: 8 partitions of 4096^3, each
+    voxelized in a 64 GiB dense grid and built alone, the records collected in HBM."""
+    import torch
+
+    from ooc_svo_builder_amd import SvoContext
+    t = time.perf_counter()
+    tris = make_mesh("soup", False)
+    gen_s = time.perf_counter() - t
+    N = tris.shape[0]
+    d = torch.from_numpy(tris).cuda()
+    del tris
+    torch.cuda.synchronize()
+    ctx = SvoContext(0)
+    try:
+        ctx.load_mesh(d, False)
+        info = ctx.rerun(False, 8192, 65536)  # warm-up: allocations, the grid's first clear
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = {"partition": 0.0, "voxelize": 0.0, "build": 0.0}
+        for _ in range(steps):
+            info = ctx.rerun(False, 8192, 65536)
+            st["partition"] += info.ms_partition
+            st["voxelize"] += info.ms_voxelize
+            st["build"] += info.ms_build
+        torch.cuda.synchronize()
+        s_step = (time.perf_counter() - t) / steps
+        st = {k: v / steps for k, v in st.items()}
+        V, nn = int(info.nfilled), int(info.n_nodes)
+        R = 36
+        vox_bytes = N * R + 8 * V
+        build_bytes = 8 * V + 24 * nn + 32 * int(info.n_data)
+        part_bytes = 2 * N * R + 4 * N  # read twice, lists written
+        res = {"workload": WORKLOADS["soup8192"][4], "triangles": N, "steps": steps,
+               "generate_s": gen_s, "s_per_step": s_step, "triangles_per_s": N / s_step,
+               "partitions": int(info.n_partitions), "voxels": V, "nodes": nn,
+               "voxels_per_s": V / s_step, "nodes_per_s": nn / s_step, "stage_ms": st,
+               "stage_hbm_frac": {
+                   "partition": part_bytes / (st["partition"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                   "voxelize": vox_bytes / (st["voxelize"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                   "build": build_bytes / (st["build"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+               "launches_per_step": int(info.launches),
+               "host_syncs_per_step": int(info.host_syncs)}
+        _log("out of core: %.3f s per step, %d voxels, %d nodes" % (s_step, V, nn))
+        return res
+    finally:
+        ctx.close()
+        del d
+        torch.cuda.empty_cache()
 
 
 def _log(msg):
@@ -330,18 +507,23 @@ class MixBench:
         _log("plain loop: %d jobs in %.3f s" % (len(self.jobs), self.plain_s))
         self.want = []
         for j in self.jobs:
-            info = self._plain(j)
-            nodes = torch.empty(info.n_nodes * 24, dtype=torch.uint8, device=dev)
-            data = torch.empty(info.n_data * 32, dtype=torch.uint8, device=dev)
-            self.ctx.copy_nodes_to(nodes.data_ptr(), 0, info.n_nodes)
-            self.ctx.copy_data_to(data.data_ptr(), 0, info.n_data)
-            self.want.append((info.n_nodes, info.n_data, info.nfilled, nodes, data))
+            self.want.append(self._keep(self._plain(j)))
+
+    def _keep(self, info):
+        """The plain path's records of the job just run, kept on the device."""
+        torch = self.torch
+        nodes = torch.empty(info.n_nodes * 24, dtype=torch.uint8, device="cuda:0")
+        data = torch.empty(info.n_data * 32, dtype=torch.uint8, device="cuda:0")
+        self.ctx.copy_nodes_to(nodes.data_ptr(), 0, info.n_nodes)
+        self.ctx.copy_data_to(data.data_ptr(), 0, info.n_data)
+        return (info.n_nodes, info.n_data, info.nfilled, nodes, data)
 
     def _plain(self, j):
         return self.ctx.run(j["records"], j["payload"], j["gridsize"])
 
-    def drop_outputs(self):
-        self.want = []
+    def drop_outputs(self, keep_every=16):
+        """Free the plain outputs but every keep_every-th job's (the served sample)."""
+        self.want = [w if i % keep_every == 0 else None for i, w in enumerate(self.want)]
         self.torch.cuda.empty_cache()
 
     def batched(self):
@@ -381,6 +563,8 @@ class MixBench:
                              "build": info.ms_build},
                 "plain_loop_seconds": self.plain_s, "plain_jobs_per_s": n / self.plain_s,
                 "speedup_vs_plain": self.plain_s / best,
+                "launches_per_batch": int(info.launches),
+                "host_syncs_per_batch": int(info.host_syncs),
                 "identical_jobs": same, "identical": same == n,
                 "autotuned": {"small_cols": knobs[0], "direct_cols": knobs[1],
                               "tile_blocks": knobs[2], "large_blocks": knobs[3]},
@@ -388,13 +572,14 @@ class MixBench:
 
     def served(self, seconds):
         np, torch = self.np, self.torch
+        import threading
+
         from ooc_svo_builder_amd import SvoServer, generators
         large = dict(records=torch.from_numpy(generators.terrain(n=2049, seed=50, payload=True))
                      .to("cuda:0"), payload=True, gridsize=2048)
         torch.cuda.synchronize()
-        # plain service time of the large job (the plain loop's per-job times otherwise)
         self._plain(large)
-        torch.cuda.synchronize()
+        large_want = self._keep(self._plain(large))
         t = time.perf_counter()
         self._plain(large)
         large_ms = (time.perf_counter() - t) * 1e3
@@ -408,23 +593,26 @@ class MixBench:
         for tk in [srv.submit(j) for j in self.jobs[:16] + [large]]:
             srv.wait(tk)
             srv.release(tk)
+
+        def wait_until(t0, at):  # sleep through long gaps, spin through short ones
+            while True:
+                left = at - (time.perf_counter() - t0)
+                if left <= 0:
+                    return
+                if left > 0.002:
+                    time.sleep(left - 0.001)
+
         for lam in rates:
             n = max(1, int(lam * seconds))
-            gaps = rng.exponential(1.0 / lam, size=n)
-            arrive = np.cumsum(gaps)
+            arrive = np.cumsum(rng.exponential(1.0 / lam, size=n))
             pick = [(-1 if rng.uniform() < 0.02 else int(rng.integers(0, len(self.jobs))))
                     for _ in range(n)]
-            import threading
-            lat = []
-            done_t = []
-            tickets = []
+            lat, done_t, tickets, failed = [], [], [], []
+            checked = [0, 0]  # sampled results compared, identical
             cv = threading.Condition()
 
-            failed = []
-
             def collector():
-                k = 0
-                while k < n:
+                for k in range(n):
                     with cv:
                         while len(tickets) <= k:
                             cv.wait()
@@ -432,22 +620,26 @@ class MixBench:
                     try:
                         r, ms = srv.wait(tk)
                         lat.append(ms)
+                        w = large_want if pick[k] < 0 else self.want[pick[k]]
+                        if w is not None and (pick[k] >= 0 or checked[0] % 4 == 0):
+                            nn, nd, _, wn, wd = w
+                            gn = torch.empty_like(wn)
+                            gd = torch.empty_like(wd)
+                            srv.copy_nodes_to(tk, gn.data_ptr(), nn)
+                            srv.copy_data_to(tk, gd.data_ptr(), nd)
+                            checked[0] += 1
+                            checked[1] += int((r.n_nodes, r.n_data) == (nn, nd) and
+                                              torch.equal(gn, wn) and torch.equal(gd, wd))
                     except Exception as e:  # counted and reported, the sweep goes on
                         failed.append(repr(e))
                     done_t.append(time.perf_counter())
                     srv.release(tk)
-                    k += 1
             st0 = srv.stats()
             th = threading.Thread(target=collector)
             th.start()
             t0 = time.perf_counter()
             for i in range(n):
-                while True:  # sleep through long gaps, spin through short ones
-                    left = arrive[i] - (time.perf_counter() - t0)
-                    if left <= 0:
-                        break
-                    if left > 0.002:
-                        time.sleep(left - 0.001)
+                wait_until(t0, arrive[i])
                 tk = srv.submit(large if pick[i] < 0 else self.jobs[pick[i]])
                 with cv:
                     tickets.append(tk)
@@ -455,14 +647,15 @@ class MixBench:
             th.join()
             span = max(done_t) - t0
             st1 = srv.stats()
-            # the plain loop under the same arrivals: FIFO, measured per-job service times
-            svc = np.array([large_ms if p < 0 else self.plain_ms[p] for p in pick]) / 1e3
-            fin = np.empty(n)
-            tcur = 0.0
+            # the plain loop under the same arrivals: svo_run one job after another in
+            # arrival order (a job waits while the loop is busy), measured
+            plat = []
+            t0 = time.perf_counter()
             for i in range(n):
-                tcur = max(tcur, arrive[i]) + svc[i]
-                fin[i] = tcur
-            plat = (fin - arrive) * 1e3
+                wait_until(t0, arrive[i])
+                self._plain(large if pick[i] < 0 else self.jobs[pick[i]])
+                plat.append((time.perf_counter() - t0 - arrive[i]) * 1e3)
+            plain_span = time.perf_counter() - t0
             row = {"rate_jobs_per_s": lam, "jobs": n, "large_jobs": int(sum(p < 0 for p in pick)),
                    "failed_jobs": len(failed), "first_failure": failed[0] if failed else None,
                    "jobs_per_s": n / span, "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
@@ -472,17 +665,24 @@ class MixBench:
                    "mean_batch_ms": (st1.batch_ms - st0.batch_ms) /
                    max(1, st1.batches - st0.batches),
                    "peak_admitted_gb": st1.peak_admitted_bytes / 1e9,
-                   "plain_jobs_per_s": n / fin[-1], "plain_p50_ms": _pct(plat, 50),
+                   "sampled_results": checked[0], "sampled_identical": checked[1],
+                   "plain_jobs_per_s": n / plain_span, "plain_p50_ms": _pct(plat, 50),
                    "plain_p99_ms": _pct(plat, 99), "plain_p999_ms": _pct(plat, 99.9)}
             rows.append(row)
-            _log("served %d/s: %.1f jobs/s p50 %.2f p99 %.2f ms (plain p99 %.1f ms)" % (
-                lam, row["jobs_per_s"], row["p50_ms"], row["p99_ms"], row["plain_p99_ms"]))
+            _log("served %d/s: %.1f jobs/s p50 %.2f p99 %.2f ms; plain %.1f jobs/s p99 %.1f ms;"
+                 " %d/%d sampled identical" % (
+                     lam, row["jobs_per_s"], row["p50_ms"], row["p99_ms"],
+                     row["plain_jobs_per_s"], row["plain_p99_ms"], checked[1], checked[0]))
         srv.close()
         cap_plain = len(self.jobs) / self.plain_s
+        peak = max(r["jobs_per_s"] for r in rows)
         return {"budget_gb": 200, "slots": 2, "window_s": seconds, "sweep": rows,
                 "plain_capacity_jobs_per_s": cap_plain,
+                "peak_served_jobs_per_s": peak, "peak_vs_plain_capacity": peak / cap_plain,
                 "large_job_plain_ms": large_ms,
-                "plain_latency_model": "FIFO queue served at the measured per-job svo_run times"}
+                "plain_latency_model": "measured: svo_run one job after another in arrival order",
+                "sampled_check": "every 16th mix job and every 4th large job byte-compared "
+                                 "with the plain path"}
 
 
 if __name__ == "__main__":

@@ -144,6 +144,8 @@ typedef struct svo_run_info {
   uint64_t n_data;
   int hazard;
   float ms_partition, ms_voxelize, ms_build; /* device time per stage */
+  uint32_t launches;   /* kernel launches the call issued */
+  uint32_t host_syncs; /* host waits on the device the call made */
 } svo_run_info;
 int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payload,
             int src_on_device, const float bbox[6], uint64_t gridsize,
@@ -196,8 +198,9 @@ int svo_run_stream(svo_ctx* ctx, const float* records, uint64_t n_triangles, int
  * [node_offset, node_offset + n_nodes) and [data_offset, data_offset + n_data) of the
  * batch's arrays (svo_copy_nodes / svo_copy_data / svo_nodes_device / svo_data_device).
  * Limits: n_jobs <= 65536, gridsize <= 32768, fewer than 2^32 triangle references and
- * voxels in the whole batch. generate_levels must be 0 (-levels is supported by the
- * single-mesh calls). 'records' of a job is a host
+ * voxels in the whole batch. Jobs with generate_levels (-levels, whose internal nodes
+ * carry averaged data records) run one at a time through the plain path after the
+ * others, their records appended to the batch's arrays. 'records' of a job is a host
  * pointer, or a device pointer when src_on_device != 0 (used in place, no copy). */
 typedef struct svo_job {
   const float* records;
@@ -242,8 +245,10 @@ int svo_ctx_set_knobs(svo_ctx* ctx, const int knobs[4]);
  * HBM footprint of each job's working set and of its held results) and max_batch_jobs,
  * and runs them as one batch (svo_batch_run). Jobs that arrive while batches are on the
  * GPU are admitted by the next free slot, so batches of different slots overlap on the
- * device. A job's node and data records stay in HBM, owned by its ticket, until
- * svo_server_release. Every result is byte-identical to svo_run on that job alone.
+ * device. A job that finds no other job queued, and every -levels job, runs alone
+ * through the plain path (svo_run). A job's node and data records stay in HBM, owned by
+ * its ticket, until svo_server_release. Every result is byte-identical to svo_run on that
+ * job alone.
  * All svo_server_* calls are thread-safe. A host 'records' pointer must stay valid until
  * the job is done (the records are read when its batch starts). */
 typedef struct svo_server svo_server;

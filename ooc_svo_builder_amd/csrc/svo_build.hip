@@ -1096,7 +1096,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     unsigned long long* brick_hazard = root_isize + n_jobs;
     // n[0] = V, off[1] = 0, flags = 0, root sizes = 0, hazard flags = 0, scan status = 0
     // (fast: n[4] = NB, off[4] = 0, off[5] = NB)
-    hipLaunchKernelGGL(k_build_init, dim3(1), dim3(256), 0, c.stream, meta,
+    SVO_LAUNCH(k_build_init, dim3(1), dim3(256), 0, c.stream, meta,
                        meta_words + st_words, V, NB, fast ? 1 : 0);
     SVO_TRY(hipGetLastError());
 
@@ -1112,18 +1112,18 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
       uint64_t* a_end = (uint64_t*)(ub + b_cnt + b_s4);
       uint64_t* job_end = (uint64_t*)(ub + b_cnt + b_s4 + b_end);
       ktimer_begin(c, "upper_levels");
-      hipLaunchKernelGGL(k_up_count, dim3(ntiles), dim3(kUpTile), 0, c.stream, A, T,
+      SVO_LAUNCH(k_up_count, dim3(ntiles), dim3(kUpTile), 0, c.stream, A, T,
                          (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),
                          c.bcode.as<uint64_t>(), c.bshape.as<uint32_t>(), NB, H, tile_cnt,
                          tile_s4, brick_hazard);
-      hipLaunchKernelGGL(k_up_scan, dim3(1), dim3(1024), 0, c.stream, A, M, NB, H, ntiles,
+      SVO_LAUNCH(k_up_scan, dim3(1), dim3(1024), 0, c.stream, A, M, NB, H, ntiles,
                          tile_cnt, tile_s4, a_end);
-      hipLaunchKernelGGL(k_up_apply, dim3(ntiles), dim3(kUpTile), 0, c.stream, A, M, T,
+      SVO_LAUNCH(k_up_apply, dim3(ntiles), dim3(kUpTile), 0, c.stream, A, M, T,
                          (const JobDev*)d_jobs, NB, H, (const uint32_t*)tile_cnt,
                          (const uint64_t*)tile_s4, (const uint64_t*)a_end, job_end);
       uint64_t nup = 0;
       for (int h = 4; h <= H; h++) nup += bound[h];
-      hipLaunchKernelGGL(k_up_finish, dim3((unsigned)((nup + 255) / 256)), dim3(256), 0,
+      SVO_LAUNCH(k_up_finish, dim3((unsigned)((nup + 255) / 256)), dim3(256), 0,
                          c.stream, A, M, T, (const JobDev*)d_jobs, H,
                          (const uint64_t*)job_end, root_size);
       ktimer_end(c);
@@ -1141,7 +1141,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
                                      M.n + h)));
       status += scan_single_words(bound[h]);
     }
-    hipLaunchKernelGGL(k_levels_tail, dim3(1), dim3(kTailBlock), 0, c.stream, A, M, T, d_jobs,
+    SVO_LAUNCH(k_levels_tail, dim3(1), dim3(kTailBlock), 0, c.stream, A, M, T, d_jobs,
                        n_jobs, c.codes.as<uint64_t>(), c.lvl_leafp.as<uint32_t>(), root_size,
                        root_isize, c.records.as<float>() + 8, h_tail, H,
                        fast ? brick_hazard : nullptr, node_cap,
@@ -1153,7 +1153,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
       uint64_t nup = 0;
       for (int h = hs; h <= H; h++) nup += bound[h];
       ktimer_begin(c, "emit_children");
-      hipLaunchKernelGGL(k_emit_upper, dim3((unsigned)std::max<uint64_t>((nup + 255) / 256, 1)),
+      SVO_LAUNCH(k_emit_upper, dim3((unsigned)std::max<uint64_t>((nup + 255) / 256, 1)),
                          dim3(256), 0, c.stream, A, M, T, (const JobDev*)d_jobs, hs, H, node_cap,
                          c.nodes.as<uint64_t>());
       ktimer_end(c);
@@ -1161,7 +1161,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     }
     for (int h = H; levels && h >= hs; h--) {
       ktimer_begin(c, "emit_children");
-      hipLaunchKernelGGL(k_emit_level, dim3((unsigned)std::max<uint64_t>((bound[h] + 255) / 256, 1)),
+      SVO_LAUNCH(k_emit_level, dim3((unsigned)std::max<uint64_t>((bound[h] + 255) / 256, 1)),
                          dim3(256), 0, c.stream, A, M, T, (const JobDev*)d_jobs, h, node_cap,
                          c.nodes.as<uint64_t>(), out_rec);
       ktimer_end(c);
@@ -1169,7 +1169,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     }
     if (fast) {
       ktimer_begin(c, "emit_bricks");
-      hipLaunchKernelGGL(k_emit_bricks, dim3((unsigned)((NB + kEmitWaves - 1) / kEmitWaves)),
+      SVO_LAUNCH(k_emit_bricks, dim3((unsigned)((NB + kEmitWaves - 1) / kEmitWaves)),
                          dim3(kWave * kEmitWaves), 0,
                          c.stream, A, M, T, (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),
                          c.bcode.as<uint64_t>(), c.bvoff.as<uint64_t>(), NB, node_cap,
@@ -1178,7 +1178,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
       SVO_TRY(hipGetLastError());
     } else if (V) {
       ktimer_begin(c, "emit_leaves");
-      hipLaunchKernelGGL(k_emit_leaves, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, c.stream,
+      SVO_LAUNCH(k_emit_leaves, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, c.stream,
                          A, M, T, (const JobDev*)d_jobs, node_cap, c.codes.as<uint64_t>(),
                          c.lvl_leafp.as<uint32_t>(), c.nodes.as<uint64_t>(),
                          c.records.as<float>(), out_rec);

@@ -66,7 +66,7 @@ hipError_t ensure_keep(DevBuf& b, size_t bytes, size_t used, hipStream_t s) {
       e = hipMemcpyAsync(np, b.p, std::min(used, b.cap), hipMemcpyDeviceToDevice, s);
       if (e != hipSuccess) return e;
     }
-    e = hipStreamSynchronize(s);
+    e = stream_sync(s);
     if (e != hipSuccess) return e;
     (void)hipFree(b.p);
   }
@@ -114,7 +114,7 @@ void ktimer_end(Context& c) {
 void ktimer_flush(Context& c) {
   for (PendingTimer& t : c.kpending) {
     float ms = 0;
-    if (hipEventSynchronize(t.b) == hipSuccess && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+    if (event_sync(t.b) == hipSuccess && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
       KernelStat& k = c.kstats[t.name];
       k.launches++;
       k.ms += ms;
@@ -185,7 +185,7 @@ static hipError_t copy_sync(Context& c, void* dst, const void* src, size_t bytes
   if (!bytes) return hipSuccess;
   hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, c.stream);
   if (e != hipSuccess) return e;
-  return hipStreamSynchronize(c.stream);
+  return stream_sync(c.stream);
 }
 static int ctz64(uint64_t x) { return x ? __builtin_ctzll(x) : 64; }
 
@@ -279,7 +279,7 @@ void svo_ctx_destroy(svo_ctx* ctx) {
   if (!ctx) return;
   Context& c = ctx->c;
   (void)hipSetDevice(c.device);
-  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  if (c.stream) (void)stream_sync(c.stream);
   DevBuf* bufs[] = {&c.tris, &c.jobdev, &c.batch_tris, &c.hitbox, &c.part_lists, &c.unitdev,
                     &c.table, &c.pool, &c.pool8, &c.grid8, &c.counters, &c.large_unit,
                     &c.large_tri, &c.large_cols, &c.large_off, &c.occ_list, &c.bcount,
@@ -290,6 +290,8 @@ void svo_ctx_destroy(svo_ctx* ctx) {
                     &c.scan_status, &c.lvl_sib, &c.up_tiles, &c.bshape, &c.large_walk};
   for (DevBuf* b : bufs) release(*b);
   stream_release(c);
+  if (c.aux) svo_ctx_destroy((svo_ctx*)c.aux);
+  c.aux = nullptr;
   if (c.stage) (void)hipHostFree(c.stage);
   c.stage = nullptr;
   if (c.hread) (void)hipHostFree(c.hread);
@@ -309,7 +311,7 @@ void svo_ctx_destroy(svo_ctx* ctx) {
 int svo_ctx_set_stream(svo_ctx* ctx, void* hip_stream) {
   CHECK_CTX();
   Context& c = ctx->c;
-  TRYC(hipStreamSynchronize(c.stream), "svo_ctx_set_stream");
+  TRYC(stream_sync(c.stream), "svo_ctx_set_stream");
   if (hip_stream) {
     if (c.own_stream) (void)hipStreamDestroy(c.stream);
     c.own_stream = false;
@@ -356,7 +358,7 @@ int svo_mesh_load(svo_ctx* ctx, const float* records, uint64_t n_triangles, int 
                         src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c.stream),
          "svo_mesh_load");
   TRYC(fill_u64(c.counters.as<uint64_t>(), 0, kNumCounters, c.stream), "svo_mesh_load");
-  TRYC(hipStreamSynchronize(c.stream), "svo_mesh_load");
+  TRYC(stream_sync(c.stream), "svo_mesh_load");
   c.n_tris = n_triangles;
   c.payload = payload ? 1 : 0;
   reset_plain(c);
@@ -402,7 +404,7 @@ int svo_mesh_load_file(svo_ctx* ctx, const char* tridata_path, uint64_t n_triang
   uint64_t off = 0;
   for (uint64_t k = 0; off < bytes; k++) {
     const int b = (int)(k & 1);
-    if (k >= 2 && (e = hipEventSynchronize(c.ingest_ev[b])) != hipSuccess) break;
+    if (k >= 2 && (e = event_sync(c.ingest_ev[b])) != hipSuccess) break;
     const size_t want = (size_t)std::min<uint64_t>(chunk, bytes - off);
     const size_t got = fread(c.ingest[b], 1, want, f);
     if (got &&
@@ -416,7 +418,7 @@ int svo_mesh_load_file(svo_ctx* ctx, const char* tridata_path, uint64_t n_triang
   fclose(f);
   if (e != hipSuccess) return fail(e, "svo_mesh_load_file");
   TRYC(fill_u64(c.counters.as<uint64_t>(), 0, kNumCounters, c.stream), "svo_mesh_load_file");
-  TRYC(hipStreamSynchronize(c.stream), "svo_mesh_load_file");
+  TRYC(stream_sync(c.stream), "svo_mesh_load_file");
   if (got_out) *got_out = off / R;
   if (off < bytes) {
     g_err = std::string(tridata_path) + " holds " + std::to_string(off / R) +
@@ -545,7 +547,7 @@ int svo_add_voxels(svo_ctx* ctx, const uint64_t* codes, const void* records32, u
                         hipMemcpyHostToDevice, c.stream),
          "svo_add_voxels");
   }
-  TRYC(hipStreamSynchronize(c.stream), "svo_add_voxels");
+  TRYC(stream_sync(c.stream), "svo_add_voxels");
   c.n_vox += n;
   return SVO_OK;
 }
@@ -626,7 +628,7 @@ int svo_copy_codes(svo_ctx* ctx, uint64_t* host_dst, uint64_t first, uint64_t co
 
 int svo_ctx_set_kernel_timing(svo_ctx* ctx, int on) {
   CHECK_CTX();
-  TRYC(hipStreamSynchronize(ctx->c.stream), "svo_ctx_set_kernel_timing");
+  TRYC(stream_sync(ctx->c.stream), "svo_ctx_set_kernel_timing");
   ktimer_flush(ctx->c);
   ctx->c.ktiming = on != 0;
   return SVO_OK;
@@ -634,7 +636,7 @@ int svo_ctx_set_kernel_timing(svo_ctx* ctx, int on) {
 
 int svo_kernel_timing(svo_ctx* ctx, const char* name, uint64_t* launches, double* total_ms) {
   CHECK_CTX();
-  TRYC(hipStreamSynchronize(ctx->c.stream), "svo_kernel_timing");
+  TRYC(stream_sync(ctx->c.stream), "svo_kernel_timing");
   ktimer_flush(ctx->c);
   auto it = ctx->c.kstats.find(name ? name : "");
   if (launches) *launches = it == ctx->c.kstats.end() ? 0 : it->second.launches;
@@ -644,7 +646,7 @@ int svo_kernel_timing(svo_ctx* ctx, const char* name, uint64_t* launches, double
 
 int svo_kernel_timing_reset(svo_ctx* ctx) {
   CHECK_CTX();
-  TRYC(hipStreamSynchronize(ctx->c.stream), "svo_kernel_timing_reset");
+  TRYC(stream_sync(ctx->c.stream), "svo_kernel_timing_reset");
   ktimer_flush(ctx->c);
   ctx->c.kstats.clear();
   return SVO_OK;
@@ -720,6 +722,7 @@ static int run_impl(svo_ctx* ctx, const float* records, uint64_t n_triangles, in
     g_err = "invalid run arguments";
     return SVO_ERR_INVALID;
   }
+  const uint64_t l0 = t_launches, s0 = t_syncs;
   int r = SVO_OK;
   if (records) {
     r = svo_mesh_load(ctx, records, n_triangles, payload, src_on_device);
@@ -748,7 +751,7 @@ static int run_impl(svo_ctx* ctx, const float* records, uint64_t n_triangles, in
   uint64_t nn = 0, nd = 0;
   int hz = 0;
   if (streamed) {
-    TRYC(hipEventSynchronize(c.ev1), "svo_run");
+    TRYC(event_sync(c.ev1), "svo_run");
     TRYC(hipEventElapsedTime(&ms_p, c.ev0, c.ev1), "svo_run");
     r = svo_stream_begin(ctx, gridsize, np, generate_levels, sink, user);
     if (r) return r;
@@ -762,7 +765,7 @@ static int run_impl(svo_ctx* ctx, const float* records, uint64_t n_triangles, in
       r = svo_stream_partition(ctx, i);
       if (r) return r;
       TRYC(hipEventRecord(c.ev2, c.stream), "svo_run");
-      TRYC(hipEventSynchronize(c.ev2), "svo_run");
+      TRYC(event_sync(c.ev2), "svo_run");
       float a = 0, b = 0;
       TRYC(hipEventElapsedTime(&a, c.ev0, c.ev1), "svo_run");
       TRYC(hipEventElapsedTime(&b, c.ev1, c.ev2), "svo_run");
@@ -773,7 +776,7 @@ static int run_impl(svo_ctx* ctx, const float* records, uint64_t n_triangles, in
     r = svo_stream_end(ctx, &nn, &nd, &hz);
     if (r) return r;
     TRYC(hipEventRecord(c.ev1, c.stream), "svo_run");
-    TRYC(hipEventSynchronize(c.ev1), "svo_run");
+    TRYC(event_sync(c.ev1), "svo_run");
     float b = 0;
     TRYC(hipEventElapsedTime(&b, c.ev0, c.ev1), "svo_run");
     ms_b += b;
@@ -789,7 +792,7 @@ static int run_impl(svo_ctx* ctx, const float* records, uint64_t n_triangles, in
     r = svo_build(ctx, gridsize, generate_levels, &nn, &nd, &hz);
     c.build_done_ev = nullptr;
     if (r) return r;
-    TRYC(hipEventSynchronize(c.ev3), "svo_run");
+    TRYC(event_sync(c.ev3), "svo_run");
     TRYC(hipEventElapsedTime(&ms_p, c.ev0, c.ev1), "svo_run");
     TRYC(hipEventElapsedTime(&ms_v, c.ev1, c.ev2), "svo_run");
     TRYC(hipEventElapsedTime(&ms_b, c.ev2, c.ev3), "svo_run");
@@ -803,6 +806,8 @@ static int run_impl(svo_ctx* ctx, const float* records, uint64_t n_triangles, in
     info->ms_partition = ms_p;
     info->ms_voxelize = ms_v;
     info->ms_build = ms_b;
+    info->launches = (uint32_t)(t_launches - l0);
+    info->host_syncs = (uint32_t)(t_syncs - s0);
   }
   return SVO_OK;
 }
@@ -822,6 +827,88 @@ int svo_run_stream(svo_ctx* ctx, const float* records, uint64_t n_triangles, int
                   memory_limit_mb, color_mode, generate_levels, true, sink, user, info);
 }
 
+// -levels jobs of a batch (OctreeBuilder.cpp:82-99 writes a data record per internal node,
+// which the shared level passes do not): each runs alone through the plain path on the
+// context's helper context, and its records are appended to the batch's arrays after the
+// shared jobs'.
+static int batch_with_levels(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs,
+                             svo_job_result* results, svo_run_info* info) {
+  Context& c = ctx->c;
+  const uint64_t l0 = t_launches, s0 = t_syncs;
+  std::vector<svo_job> shared;
+  std::vector<uint32_t> shared_idx, lv_idx;
+  for (uint32_t j = 0; j < n_jobs; j++) {
+    if (jobs[j].generate_levels) {
+      lv_idx.push_back(j);
+    } else {
+      shared.push_back(jobs[j]);
+      shared_idx.push_back(j);
+    }
+  }
+  svo_run_info tot{};
+  if (!shared.empty()) {
+    std::vector<svo_job_result> res(shared.size());
+    const int r = svo_batch_run(ctx, shared.data(), (uint32_t)shared.size(), res.data(), &tot);
+    if (r) return r;
+    for (size_t i = 0; i < shared.size(); i++) results[shared_idx[i]] = res[i];
+  } else {
+    reset_plain(c);
+    c.batch_state = true;
+  }
+  if (!c.aux) {
+    const int r = svo_ctx_create(c.device, (svo_ctx**)&c.aux);
+    if (r) return r;
+  }
+  svo_ctx* aux = (svo_ctx*)c.aux;
+  aux->c.tune = c.tune;
+  aux->c.dense_limit = c.dense_limit;
+  for (uint32_t j : lv_idx) {
+    const svo_job& q = jobs[j];
+    svo_run_info in{};
+    int r = svo_run(aux, q.records, q.n_triangles, q.payload, q.src_on_device, q.bbox, q.gridsize,
+                    q.memory_limit_mb, q.color_mode, 1, &in);
+    if (r) {
+      g_err = "batch job " + std::to_string(j) + ": " + g_err;
+      return r;
+    }
+    CHECK_CTX();  // back on the batch context's device, error string cleared
+    TRYC(ensure_keep(c.nodes, (c.n_nodes + in.n_nodes) * 24, c.n_nodes * 24, c.stream),
+         "svo_batch_run");
+    TRYC(ensure_keep(c.records, (c.n_data + in.n_data) * 32, c.n_data * 32, c.stream),
+         "svo_batch_run");
+    TRYC(hipMemcpyAsync((char*)c.nodes.p + c.n_nodes * 24, svo_nodes_device(aux),
+                        in.n_nodes * 24, hipMemcpyDeviceToDevice, c.stream),
+         "svo_batch_run");
+    TRYC(hipMemcpyAsync((char*)c.records.p + c.n_data * 32, svo_data_device(aux),
+                        in.n_data * 32, hipMemcpyDeviceToDevice, c.stream),
+         "svo_batch_run");
+    TRYC(stream_sync(c.stream), "svo_batch_run");
+    svo_job_result& o = results[j];
+    o.n_partitions = in.n_partitions;
+    o.nfilled = in.nfilled;
+    o.n_nodes = in.n_nodes;
+    o.n_data = in.n_data;
+    o.node_offset = c.n_nodes;
+    o.data_offset = c.n_data;
+    o.hazard = in.hazard;
+    c.n_nodes += in.n_nodes;
+    c.n_data += in.n_data;
+    tot.n_partitions += in.n_partitions;
+    tot.nfilled += in.nfilled;
+    tot.hazard |= in.hazard;
+    tot.ms_partition += in.ms_partition;
+    tot.ms_voxelize += in.ms_voxelize;
+    tot.ms_build += in.ms_build;
+  }
+  c.data_in_records2 = false;
+  tot.n_nodes = c.n_nodes;
+  tot.n_data = c.n_data;
+  tot.launches = (uint32_t)(t_launches - l0);
+  tot.host_syncs = (uint32_t)(t_syncs - s0);
+  if (info) *info = tot;
+  return SVO_OK;
+}
+
 int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_result* results,
                   svo_run_info* info) {
   CHECK_CTX();
@@ -830,6 +917,9 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
     g_err = "invalid batch arguments";
     return SVO_ERR_INVALID;
   }
+  for (uint32_t j = 0; j < n_jobs; j++)
+    if (jobs[j].generate_levels) return batch_with_levels(ctx, jobs, n_jobs, results, info);
+  const uint64_t l0 = t_launches, s0 = t_syncs;
   uint64_t host_bytes = 0;
   for (uint32_t j = 0; j < n_jobs; j++) {
     const svo_job& q = jobs[j];
@@ -837,10 +927,6 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
         q.color_mode < 0 || q.color_mode > 3 || (q.n_triangles && !q.records)) {
       g_err = "invalid arguments for batch job " + std::to_string(j);
       return SVO_ERR_INVALID;
-    }
-    if (q.generate_levels) {
-      g_err = "-levels is not implemented on the GPU path";
-      return SVO_ERR_UNSUPPORTED;
     }
     if (!q.src_on_device) host_bytes += (q.n_triangles * (q.payload ? 84 : 36) + 255) & ~255ull;
   }
@@ -940,7 +1026,7 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
   hipError_t e = build_jobs(c, n_jobs, true, 0);
   c.build_done_ev = nullptr;
   if (e != hipSuccess) return fail(e, "svo_batch_run");
-  TRYC(hipEventSynchronize(c.ev3), "svo_batch_run");
+  TRYC(event_sync(c.ev3), "svo_batch_run");
   TRYC(hipEventElapsedTime(&ms_p, c.ev0, c.ev1), "svo_batch_run");
   TRYC(hipEventElapsedTime(&ms_v, c.ev1, c.ev2), "svo_batch_run");
   TRYC(hipEventElapsedTime(&ms_b, c.ev2, c.ev3), "svo_batch_run");
@@ -973,6 +1059,8 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
     info->ms_partition = ms_p;
     info->ms_voxelize = ms_v;
     info->ms_build = ms_b;
+    info->launches = (uint32_t)(t_launches - l0);
+    info->host_syncs = (uint32_t)(t_syncs - s0);
   }
   return SVO_OK;
 }

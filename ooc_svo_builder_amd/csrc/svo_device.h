@@ -58,6 +58,52 @@ __device__ __forceinline__ float max_std(float a, float b) { return (a < b) ? b 
 __device__ __forceinline__ int clamp_i(int v, int lo, int hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
+
+// ------------------------------------------------------------------------------------
+// Partition hit boxes (partitioner.cpp:43-77,101-149, intersection.h:49-53): the range of
+// partition cubes along each axis that a triangle's bbox touches, from the cubes' world
+// bounds cube_lo[c], cube_hi[c] (c < K per axis), packed as 8-bit (lo, hi) pairs per
+// axis; ~0 when the bbox misses every cube. NaN never rejects.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t part_hitbox_of(const float* __restrict__ p,
+                                                   const float* __restrict__ cube_lo,
+                                                   const float* __restrict__ cube_hi, int K) {
+  uint64_t packed = 0;
+  bool none = false;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float mn = min_std(p[a], min_std(p[3 + a], p[6 + a]));
+    const float mx = max_std(p[a], max_std(p[3 + a], p[6 + a]));
+    // rejected on this axis: mx < lo_c or mn > hi_c (NaN never rejects)
+    int below = 0, above = 0;
+    for (int c = 0; c < K; c++) {
+      below += (cube_hi[c] < mn) ? 1 : 0;
+      above += (mx < cube_lo[c]) ? 1 : 0;
+    }
+    const int clo = below, chi = K - 1 - above;
+    if (clo > chi) none = true;
+    packed |= ((uint64_t)(clo & 0xFF) | ((uint64_t)(chi & 0xFF) << 8)) << (16 * a);
+  }
+  return none ? ~0ull : packed;
+}
+
+// 1 when the triangle with hit box hb[i] touches partition cube (cx, cy, cz)
+struct PartLoad {
+  const uint64_t* hb;
+  uint32_t cx, cy, cz;
+  __device__ uint32_t operator()(uint64_t i) const {
+    const uint64_t h = hb[i];
+    if (h == ~0ull) return 0u;
+    const uint32_t c[3] = {cx, cy, cz};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const uint32_t lo = (uint32_t)(h >> (16 * a)) & 0xFF, hi = (uint32_t)(h >> (16 * a + 8)) & 0xFF;
+      if (c[a] < lo || c[a] > hi) return 0u;
+    }
+    return 1u;
+  }
+};
+
 // NaN written to .octreedata is stored as the x86 default NaN 0xFFC00000
 __device__ __forceinline__ float canon_nan(float f) {
   return (f != f) ? __uint_as_float(0xFFC00000u) : f;

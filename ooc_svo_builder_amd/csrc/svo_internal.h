@@ -10,6 +10,26 @@
 
 namespace svo {
 
+// Kernel launches and host synchronizations issued by this thread (a context is used by
+// one thread at a time, so the difference across a call is that call's count).
+inline thread_local uint64_t t_launches = 0;
+inline thread_local uint64_t t_syncs = 0;
+inline void count_launch() { t_launches++; }
+inline hipError_t stream_sync(hipStream_t s) {
+  t_syncs++;
+  return hipStreamSynchronize(s);
+}
+inline hipError_t event_sync(hipEvent_t e) {
+  t_syncs++;
+  return hipEventSynchronize(e);
+}
+// every kernel launch of the library goes through this
+#define SVO_LAUNCH(...)       \
+  do {                        \
+    ::svo::count_launch();    \
+    hipLaunchKernelGGL(__VA_ARGS__); \
+  } while (0)
+
 // Sparse occupancy grid: the partition's Morton range is cut into aligned bricks of
 // 2^kBrickBits voxels (16^3). A dense table maps brick -> pool slot (0 = empty). A slot
 // holds the brick's voxels as bytes (marked by plain stores) and, after compaction, as
@@ -125,19 +145,20 @@ struct VoxUnit {
 // svo_sink_fn). A non-zero return stops the build.
 typedef int (*SinkFn)(void* user, int kind, const void* bytes, uint64_t size);
 
-// A partition's subtree root kept for the levels above the partitions: its 24-byte record
+// A node of the levels at and above the partition roots: its 24-byte record
 // (children_base and data index already global) and, with -levels, its averaged colour
-// and normal.
-struct PartRoot {
-  uint64_t part;
-  uint64_t rec[3];
-  float cache[6];
+// and normal; null = no voxels below.
+struct TopNode {
+  bool null = true;
+  uint64_t rec[3] = {0, 0, ~0ull};
+  float cache[6] = {0, 0, 0, 0, 0, 0};
 };
 
 // Streamed (out-of-core) build, OctreeBuilder.cpp:4-168 driven per partition as
 // main.cpp:329-387 does: each partition's subtree is built alone and its records leave
 // (to the sink, or appended to out_nodes / out_data in HBM) before the next partition;
-// the partition roots, the levels above them and the root come last.
+// the partition roots and the levels above them are grouped by eight as their groups
+// complete, and the root comes last.
 struct StreamState {
   bool active = false;
   uint64_t gridsize = 0, n_parts = 0;
@@ -147,7 +168,8 @@ struct StreamState {
   void* user = nullptr;
   uint64_t next_part = 0;               // partitions below this one are done
   uint64_t nodes_out = 0, data_out = 0; // records emitted so far
-  std::vector<PartRoot> roots;          // non-empty partitions, in order
+  std::vector<std::vector<TopNode>> tbuf;  // per level 0 (root) .. log8(P) (partition roots)
+  bool any_voxels = false;
   int hazard = 0;
   bool sink_failed = false;             // the sink returned non-zero
   DevBuf out_nodes, out_data;           // the records, when there is no sink
@@ -222,6 +244,8 @@ struct Context {
   const DevBuf& data_out() const { return data_in_records2 ? records2 : records; }
 
   StreamState st;
+  // helper context (an svo_ctx) of a batch's -levels jobs, created on first use
+  void* aux = nullptr;
   // streamed output: two pinned chunk buffers for the device-to-host copies that feed the
   // sink, and their copy events
   void* drain[2] = {nullptr, nullptr};

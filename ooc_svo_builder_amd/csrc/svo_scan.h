@@ -9,6 +9,7 @@
 #include <stdint.h>
 
 #include "svo_device.h"
+#include "svo_internal.h"
 
 namespace svo {
 
@@ -162,17 +163,17 @@ hipError_t scan_exclusive_items(Load ld, Store st, uint64_t n, T* scratch, hipSt
   const uint64_t tile = (uint64_t)kScanBlock * ITEMS;
   const uint64_t nb = (n + tile - 1) / tile;
   if (nb == 1) {
-    hipLaunchKernelGGL((k_scan_apply<T, Load, Store, ITEMS>), dim3(1), dim3(kScanBlock), 0, s, ld,
+    SVO_LAUNCH((k_scan_apply<T, Load, Store, ITEMS>), dim3(1), dim3(kScanBlock), 0, s, ld,
                        st, n, n_dev, (const T*)nullptr);
     return hipGetLastError();
   }
   T* sums = scratch;
-  hipLaunchKernelGGL((k_scan_reduce<T, Load, ITEMS>), dim3((unsigned)nb), dim3(kScanBlock), 0, s,
+  SVO_LAUNCH((k_scan_reduce<T, Load, ITEMS>), dim3((unsigned)nb), dim3(kScanBlock), 0, s,
                      ld, n, n_dev, sums);
   hipError_t e = scan_exclusive<T>(ArrayLoad<T>{sums}, ArrayStoreExcl<T>{sums}, nb,
                                    scratch + nb, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_scan_apply<T, Load, Store, ITEMS>), dim3((unsigned)nb), dim3(kScanBlock),
+  SVO_LAUNCH((k_scan_apply<T, Load, Store, ITEMS>), dim3((unsigned)nb), dim3(kScanBlock),
                      0, s, ld, st, n, n_dev, (const T*)sums);
   return hipGetLastError();
 }
@@ -304,10 +305,10 @@ hipError_t scan_single(Load ld, Store st, uint64_t n, unsigned long long* status
   const uint64_t tile = (uint64_t)kSingleBlock * items;
   const unsigned nb = (unsigned)((n + tile - 1) / tile);
   if (items == kScanItems)
-    hipLaunchKernelGGL((k_scan_single<T, Load, Store, kScanItems>), dim3(nb), dim3(kSingleBlock),
+    SVO_LAUNCH((k_scan_single<T, Load, Store, kScanItems>), dim3(nb), dim3(kSingleBlock),
                        0, s, ld, st, n, n_dev, status, err);
   else
-    hipLaunchKernelGGL((k_scan_single<T, Load, Store, 1>), dim3(nb), dim3(kSingleBlock), 0, s,
+    SVO_LAUNCH((k_scan_single<T, Load, Store, 1>), dim3(nb), dim3(kSingleBlock), 0, s,
                        ld, st, n, n_dev, status, err);
   return hipGetLastError();
 }

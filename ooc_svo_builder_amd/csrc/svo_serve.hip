@@ -153,32 +153,56 @@ static int collect(svo_server* s, svo_ctx* ctx, Ticket* t, const svo_job_result&
   return SVO_OK;
 }
 
-// Run a batch on a slot; on a batch failure each job is run alone so that one bad job
-// fails alone, with its own message.
+// One job alone on a slot, through the plain path (svo_run): a job that finds the queue
+// otherwise empty does not pay for the batch's job tags and layout, and -levels jobs
+// (OctreeBuilder.cpp:82-99), which the shared batch passes do not build, run this way.
+static int run_alone(svo_server* s, svo_ctx* c, Ticket* t) {
+  const svo_job& q = t->job;
+  svo_run_info info{};
+  const int rc = svo_run(c, q.records, q.n_triangles, q.payload, q.src_on_device, q.bbox,
+                         q.gridsize, q.memory_limit_mb, q.color_mode, q.generate_levels, &info);
+  if (rc != SVO_OK) {
+    t->error = svo_last_error();
+    return rc;
+  }
+  svo_job_result r{};
+  r.n_partitions = info.n_partitions;
+  r.nfilled = info.nfilled;
+  r.n_nodes = info.n_nodes;
+  r.n_data = info.n_data;
+  r.hazard = info.hazard;
+  return collect(s, c, t, r);
+}
+
+// Run a batch on a slot: a lone job and every -levels job through the plain path, the
+// others as one batch; on a batch failure each job is run alone so that one bad job fails
+// alone, with its own message.
 int svo_server::run_jobs(uint32_t slot, std::vector<Ticket*>& batch) {
   svo_ctx* c = ctx[slot];
-  std::vector<svo_job> jobs;
-  for (Ticket* t : batch) jobs.push_back(t->job);
-  std::vector<svo_job_result> res(batch.size());
-  int rc = svo_batch_run(c, jobs.data(), (uint32_t)jobs.size(), res.data(), nullptr);
-  if (rc == SVO_OK) {
-    for (size_t i = 0; i < batch.size(); i++) {
-      batch[i]->status = collect(this, c, batch[i], res[i]);
-    }
-  } else if (batch.size() == 1) {
-    batch[0]->status = rc;
-    batch[0]->error = svo_last_error();
-  } else {
-    for (size_t i = 0; i < batch.size(); i++) {
-      svo_job_result r{};
-      const int rc1 = svo_batch_run(c, &jobs[i], 1, &r, nullptr);
-      if (rc1 == SVO_OK) {
-        batch[i]->status = collect(this, c, batch[i], r);
-      } else {
-        batch[i]->status = rc1;
-        batch[i]->error = svo_last_error();
-      }
+  std::vector<Ticket*> shared;
+  for (Ticket* t : batch) {
+    if (batch.size() == 1 || t->job.generate_levels) {
+      t->status = run_alone(this, c, t);
       (void)hipStreamSynchronize((hipStream_t)svo_ctx_stream(c));
+    } else {
+      shared.push_back(t);
+    }
+  }
+  if (!shared.empty()) {
+    std::vector<svo_job> jobs;
+    for (Ticket* t : shared) jobs.push_back(t->job);
+    std::vector<svo_job_result> res(shared.size());
+    const int rc = svo_batch_run(c, jobs.data(), (uint32_t)jobs.size(), res.data(), nullptr);
+    if (rc == SVO_OK) {
+      for (size_t i = 0; i < shared.size(); i++) shared[i]->status = collect(this, c, shared[i], res[i]);
+    } else if (shared.size() == 1) {
+      shared[0]->status = rc;
+      shared[0]->error = svo_last_error();
+    } else {
+      for (Ticket* t : shared) {
+        t->status = run_alone(this, c, t);
+        (void)hipStreamSynchronize((hipStream_t)svo_ctx_stream(c));
+      }
     }
   }
   return hipStreamSynchronize((hipStream_t)svo_ctx_stream(c)) == hipSuccess ? SVO_OK

@@ -5,9 +5,10 @@
 // Partition i is an aligned Morton cube, so its voxels form the subtree of one node Q_i
 // at depth log8(P). The streaming builder writes every node record of that subtree except
 // Q_i itself while partition i is fed, before anything of partition i+1; Q_i is written
-// when its group of 8 sibling partitions completes, with the levels above the
-// partitions, and the root comes last (OctreeBuilder.cpp:34-55, 58-79):
-//   nodes = [subtree(Q_0) - Q_0] [subtree(Q_1) - Q_1] ... [levels above partitions] root
+// when its group of 8 sibling partitions completes (after partition 8j+7), the group of
+// their parents when 64 partitions complete, and so on; the root comes last
+// (OctreeBuilder.cpp:34-55, 58-79, 112-128). With P = 8:
+//   nodes = [subtree(Q_0) - Q_0] ... [subtree(Q_7) - Q_7] [non-null Q_0..Q_7] root
 // Each partition is built alone by the level passes (svo_build.hip) over its own voxels
 // at the global gridsize. That tree is Q_i's subtree plus the chain of D - Dp single-child
 // ancestors up to the root, and in its post-order output Q_i sits at size(Q_i) - 1 with
@@ -18,9 +19,10 @@
 //
 // Data records follow the same order: payload voxel records in voxel order; with -levels
 // every record in post-order, so partition i contributes its subtree's records (Q_i's
-// average last) and the chain's records are dropped. The levels above the partitions are
-// at most a few thousand nodes (P = 8^k) and are grouped on the host exactly like the
-// reference's buffers group eight siblings (OctreeBuilder.cpp:58-128).
+// average last) and the chain's records are dropped. The partition roots and the levels
+// above them (P = 8^k nodes at most per level) are grouped on the host, as the
+// reference's buffers group eight siblings (OctreeBuilder.cpp:58-128), each group's
+// records going out when it completes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -98,7 +100,7 @@ static hipError_t drain_to_sink(Context& c, int kind, const void* dsrc, uint64_t
   for (uint64_t k = 0; off < bytes; k++) {
     const int b = (int)(k & 1);
     if (pend[b]) {  // chunk k - 2: copied by now or soon; the sink takes it before reuse
-      SVO_TRY(hipEventSynchronize(c.drain_ev[b]));
+      SVO_TRY(event_sync(c.drain_ev[b]));
       if (c.st.sink(c.st.user, kind, c.drain[b], pend[b]) != 0) return sink_error(c);
       pend[b] = 0;
     }
@@ -113,7 +115,7 @@ static hipError_t drain_to_sink(Context& c, int kind, const void* dsrc, uint64_t
   for (uint64_t k = nk >= 2 ? nk - 2 : 0; k < nk; k++) {
     const int b = (int)(k & 1);
     if (!pend[b]) continue;
-    SVO_TRY(hipEventSynchronize(c.drain_ev[b]));
+    SVO_TRY(event_sync(c.drain_ev[b]));
     if (c.st.sink(c.st.user, kind, c.drain[b], pend[b]) != 0) return sink_error(c);
     pend[b] = 0;
   }
@@ -156,11 +158,19 @@ hipError_t stream_begin(Context& c, uint64_t gridsize, uint64_t n_parts, int lev
   s.user = user;
   s.next_part = 0;
   s.nodes_out = s.data_out = 0;
-  s.roots.clear();
+  s.tbuf.assign((size_t)(ilog2(n_parts) / 3) + 1, std::vector<TopNode>());
+  s.any_voxels = false;
   s.hazard = 0;
   s.sink_failed = false;
   reset_voxel_stream(c);
   c.n_nodes = c.n_data = 0;
+  if (!sink) {
+    // the larger of the two node (data) arrays, usually the previous run's output,
+    // collects this run's records and the other takes the partitions' builds: no growth
+    // copies when a mesh is built again
+    if (c.nodes.cap > s.out_nodes.cap) std::swap(c.nodes, s.out_nodes);
+    if (c.records.cap > s.out_data.cap) std::swap(c.records, s.out_data);
+  }
   // OctreeBuilder.cpp:25-29: the null record, and for geometry-only builds the white
   // record every leaf refers to
   float hdr[16];
@@ -172,13 +182,100 @@ hipError_t stream_begin(Context& c, uint64_t gridsize, uint64_t n_parts, int lev
   return hipSuccess;
 }
 
+static inline float canon_nan_host(float f) {
+  if (f != f) {
+    const uint32_t u = 0xFFC00000u;
+    memcpy(&f, &u, 4);
+  }
+  return f;
+}
+
+// A completed group of 8 siblings (OctreeBuilder.cpp:58-102, 112-128): a null parent when
+// all are null; otherwise the non-null ones are written consecutively, the parent points
+// at the first, and with -levels its average is written as a data record.
+static hipError_t top_group(Context& c, const std::vector<TopNode>& ch, TopNode& p) {
+  StreamState& s = c.st;
+  bool any = false;
+  for (const TopNode& t : ch) any = any || !t.null;
+  p = TopNode();
+  if (!any) return hipSuccess;
+  p.null = false;
+  uint64_t recs[8 * 3];
+  int nrec = 0;
+  uint64_t cb = 0, offs = 0;
+  for (int k = 0; k < 8; k++) {
+    uint64_t byte = 0xFF;
+    if (!ch[k].null) {
+      const uint64_t pos = s.nodes_out + (uint64_t)nrec;
+      memcpy(recs + 3 * nrec, ch[k].rec, 24);
+      if (nrec == 0) cb = pos;
+      byte = pos - cb;
+      nrec++;
+    }
+    offs |= byte << (8 * k);
+  }
+  SVO_TRY(emit_host(c, 0, recs, (uint64_t)nrec * 24));
+  s.nodes_out += (uint64_t)nrec;
+  p.rec[0] = 0;
+  p.rec[1] = cb;
+  p.rec[2] = offs;
+  if (s.levels) {  // OctreeBuilder.cpp:82-99, the float order of average_children
+    float a[6] = {0, 0, 0, 0, 0, 0};
+    float notnull = 0.0f;
+    for (int k = 0; k < 8; k++) {
+      if (!ch[k].null) notnull += 1.0f;
+      for (int q = 0; q < 6; q++) a[q] = a[q] + ch[k].cache[q];
+    }
+    float avg[6];
+    avg[0] = a[0] / notnull;
+    avg[1] = a[1] / notnull;
+    avg[2] = a[2] / notnull;
+    const float t0 = a[3] / notnull, t1 = a[4] / notnull, t2 = a[5] / notnull;
+    const float inv = 1.0f / sqrtf((t0 * t0 + t1 * t1) + t2 * t2);
+    avg[3] = t0 * inv;
+    avg[4] = t1 * inv;
+    avg[5] = t2 * inv;
+    float rec[8];
+    memset(rec, 0, 8);  // morton 0
+    for (int q = 0; q < 6; q++) rec[2 + q] = canon_nan_host(avg[q]);
+    SVO_TRY(emit_host(c, 1, rec, 32));
+    p.rec[0] = s.data_out++;
+    memcpy(p.cache, avg, sizeof(avg));
+  }
+  return hipSuccess;
+}
+
+// The next partition root (or null) at the partition level; full groups of 8 go up.
+static hipError_t top_push(Context& c, const TopNode& t) {
+  StreamState& s = c.st;
+  int L = (int)s.tbuf.size() - 1;
+  s.tbuf[L].push_back(t);
+  while (L > 0 && s.tbuf[L].size() == 8) {
+    TopNode p;
+    SVO_TRY(top_group(c, s.tbuf[L], p));
+    s.tbuf[L].clear();
+    L--;
+    s.tbuf[L].push_back(p);
+  }
+  return hipSuccess;
+}
+
+// Partitions [next_part, end) hold no voxels: null roots.
+static hipError_t top_skip_to(Context& c, uint64_t end) {
+  StreamState& s = c.st;
+  for (; s.next_part < end; s.next_part++) SVO_TRY(top_push(c, TopNode()));
+  return hipSuccess;
+}
+
 hipError_t stream_partition(Context& c, uint64_t part) {
   StreamState& s = c.st;
+  SVO_TRY(top_skip_to(c, part));
   s.next_part = part + 1;
   if (c.n_vox == 0) {  // no voxels: a null subtree, nothing written
     reset_voxel_stream(c);
-    return hipSuccess;
+    return top_push(c, TopNode());
   }
+  s.any_voxels = true;
   Job& J = c.jobs[0];
   J.gridsize = s.gridsize;
   J.voff = 0;
@@ -200,9 +297,8 @@ hipError_t stream_partition(Context& c, uint64_t part) {
   const uint64_t q = c.n_nodes - 1 - chain;  // Q's record; [0, q) is the partition's output
   const uint64_t hdr = s.payload ? 1 : 2;
   const uint64_t cb_add = s.nodes_out, data_add = s.data_out - hdr;
-  PartRoot pr;
-  memset(&pr, 0, sizeof(pr));
-  pr.part = part;
+  TopNode pr;
+  pr.null = false;
   {
     const ReadReq rq = {pr.rec, c.nodes.as<uint64_t>() + 3 * q, 24};
     SVO_TRY(read_back(c, &rq, 1));
@@ -219,14 +315,14 @@ hipError_t stream_partition(Context& c, uint64_t part) {
   if (q) {
     const unsigned grid = (unsigned)std::min<uint64_t>((q + 255) / 256, 16384);
     if (s.sink) {
-      hipLaunchKernelGGL(k_rebase_nodes, dim3(grid), dim3(256), 0, c.stream,
+      SVO_LAUNCH(k_rebase_nodes, dim3(grid), dim3(256), 0, c.stream,
                          (const uint64_t*)c.nodes.as<uint64_t>(), c.nodes.as<uint64_t>(), q,
                          cb_add, data_add, hdr);
       SVO_TRY(hipGetLastError());
       SVO_TRY(drain_to_sink(c, 0, c.nodes.p, q * 24, 24));
     } else {
       SVO_TRY(ensure_keep(s.out_nodes, (s.nodes_out + q) * 24, s.nodes_out * 24, c.stream));
-      hipLaunchKernelGGL(k_rebase_nodes, dim3(grid), dim3(256), 0, c.stream,
+      SVO_LAUNCH(k_rebase_nodes, dim3(grid), dim3(256), 0, c.stream,
                          (const uint64_t*)c.nodes.as<uint64_t>(),
                          s.out_nodes.as<uint64_t>() + 3 * s.nodes_out, q, cb_add, data_add, hdr);
       SVO_TRY(hipGetLastError());
@@ -252,102 +348,23 @@ hipError_t stream_partition(Context& c, uint64_t part) {
   }
   s.nodes_out += q;
   s.data_out += dn;
-  s.roots.push_back(pr);
   reset_voxel_stream(c);
-  return hipSuccess;
+  return top_push(c, pr);
 }
-
-namespace {
-struct TopNode {
-  bool null = true;
-  uint64_t rec[3] = {0, 0, kNoChildren};
-  float cache[6] = {0, 0, 0, 0, 0, 0};
-};
-inline float canon_nan_host(float f) {
-  if (f != f) {
-    const uint32_t u = 0xFFC00000u;
-    memcpy(&f, &u, 4);
-  }
-  return f;
-}
-}  // namespace
 
 hipError_t stream_end(Context& c, uint64_t* n_nodes, uint64_t* n_data, int* hazard) {
   StreamState& s = c.st;
-  // the levels above the partitions: groups of 8 siblings from the partition roots up
-  // (OctreeBuilder.cpp:58-128); a group with a non-null member writes its non-null
-  // members consecutively, the parent points at the first one
-  std::vector<TopNode> lvl(s.n_parts);
-  for (const PartRoot& r : s.roots) {
-    TopNode& t = lvl[r.part];
-    t.null = false;
-    memcpy(t.rec, r.rec, sizeof(t.rec));
-    memcpy(t.cache, r.cache, sizeof(t.cache));
-  }
-  std::vector<uint64_t> nodes_h;  // 3 words per record
-  std::vector<float> data_h;      // 8 words per record
-  uint64_t npos = s.nodes_out, dpos = s.data_out;
-  while (lvl.size() > 1) {
-    std::vector<TopNode> up(lvl.size() / 8);
-    for (size_t g = 0; g < up.size(); g++) {
-      const TopNode* ch = &lvl[8 * g];
-      bool any = false;
-      for (int k = 0; k < 8; k++) any = any || !ch[k].null;
-      if (!any) continue;
-      TopNode& p = up[g];
-      p.null = false;
-      uint64_t cb = 0, offs = 0;
-      bool first = true;
-      for (int k = 0; k < 8; k++) {
-        uint64_t byte = 0xFF;
-        if (!ch[k].null) {
-          const uint64_t pos = npos++;
-          nodes_h.insert(nodes_h.end(), ch[k].rec, ch[k].rec + 3);
-          if (first) cb = pos;
-          first = false;
-          byte = pos - cb;
-        }
-        offs |= byte << (8 * k);
-      }
-      p.rec[0] = 0;
-      p.rec[1] = cb;
-      p.rec[2] = offs;
-      if (s.levels) {  // OctreeBuilder.cpp:82-99, the float order of average_children
-        float a[6] = {0, 0, 0, 0, 0, 0};
-        float notnull = 0.0f;
-        for (int k = 0; k < 8; k++) {
-          if (!ch[k].null) notnull += 1.0f;
-          for (int q = 0; q < 6; q++) a[q] = a[q] + ch[k].cache[q];
-        }
-        float avg[6];
-        avg[0] = a[0] / notnull;
-        avg[1] = a[1] / notnull;
-        avg[2] = a[2] / notnull;
-        const float t0 = a[3] / notnull, t1 = a[4] / notnull, t2 = a[5] / notnull;
-        const float inv = 1.0f / sqrtf((t0 * t0 + t1 * t1) + t2 * t2);
-        avg[3] = t0 * inv;
-        avg[4] = t1 * inv;
-        avg[5] = t2 * inv;
-        float rec[8];
-        memset(rec, 0, 8);  // morton 0
-        for (int q = 0; q < 6; q++) rec[2 + q] = canon_nan_host(avg[q]);
-        data_h.insert(data_h.end(), rec, rec + 8);
-        p.rec[0] = dpos++;
-        memcpy(p.cache, avg, sizeof(avg));
-      }
-    }
-    lvl.swap(up);
-  }
-  nodes_h.insert(nodes_h.end(), lvl[0].rec, lvl[0].rec + 3);  // the root, last (:42)
-  SVO_TRY(emit_host(c, 0, nodes_h.data(), nodes_h.size() * 8));
-  SVO_TRY(emit_host(c, 1, data_h.data(), data_h.size() * 4));
-  const uint64_t nn = s.nodes_out + nodes_h.size() / 3;
-  const uint64_t nd = s.data_out + data_h.size() / 8;
+  SVO_TRY(top_skip_to(c, s.n_parts));  // the remaining partitions are empty
+  // the root, last (OctreeBuilder.cpp:42); an empty tree's root is a null record
+  const TopNode& root = s.tbuf[0].at(0);
+  SVO_TRY(emit_host(c, 0, root.rec, 24));
+  s.nodes_out++;
+  const uint64_t nn = s.nodes_out, nd = s.data_out;
   // an empty 1^3 grid: the reference reads a root buffer that was never filled
-  if (s.gridsize == 1 && s.roots.empty()) s.hazard = 1;
+  if (s.gridsize == 1 && !s.any_voxels) s.hazard = 1;
   s.active = false;
   if (s.sink) {
-    SVO_TRY(hipStreamSynchronize(c.stream));
+    SVO_TRY(stream_sync(c.stream));
     c.n_nodes = c.n_data = 0;  // nothing of the octree is held on the device
   } else {
     std::swap(c.nodes, s.out_nodes);

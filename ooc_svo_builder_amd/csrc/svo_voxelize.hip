@@ -165,14 +165,17 @@ __device__ __forceinline__ void setup_tri(const float* __restrict__ p, const Vox
 //     Ka, Kb per column (Kb = +0 when w = z; a zero's sign never changes the decision);
 //   * the projection test in the plane across w (3 edges) depends on (pa, pb) only and
 //     is evaluated once per column: (h0*pa + h1*pb) + hd;
-//   * each edge of the two other planes is (cv*pc + cc*pconst) + de per voxel.
+//   * each edge of the two other planes is (cv*pc + cc*pconst) + de per voxel, where
+//     pconst is pb for one of them (plane P) and pa for the other (plane Q).
 // Every product and sum is the reference's own (voxelizer.cpp:257-287); only operands of
-// single additions are swapped, which is exact in IEEE arithmetic.
+// single additions are swapped, which is exact in IEEE arithmetic. The products of pb
+// (h1*pb, cc*pb of plane P, the plane-test term of b, the slab term of b) are the same
+// along a row of columns and are computed once per row.
 enum WalkF {
   F_NW = 0, F_D1, F_D2, F_NUA, F_NVA,
   F_H0 = 5, F_H1 = 8, F_HD = 11,
-  F_AV = 14, F_AC = 17, F_AD = 20,
-  F_BV = 23, F_BC = 26, F_BD = 29,
+  F_PV = 14, F_PC = 17, F_PD = 20,
+  F_QV = 23, F_QC = 26, F_QD = 29,
   F_KA = 32, F_KB, F_ALO, F_AHI,
   kWalkF = 36
 };
@@ -193,12 +196,12 @@ struct Walk {
 };
 
 // The plane roles for column axis w, as value selects: the hoisted plane's edges (H0, H1
-// in operand order (pa, pb), HD), and planes A and B with their column-coordinate factor
-// (AV, BV), the other factor (AC, BC) and the offset (AD, BD). Edges: xy = ne[0..2],
-// yz = ne[3..5], zx = ne[6..8].
-//   w = 2: hoisted xy(pa, pb); A = yz(pb, pc); B = zx(pc, pa)
-//   w = 0: hoisted yz(pa, pb); A = xy(pc, pa); B = zx(pb, pc)
-//   w = 1: hoisted zx(pb, pa); A = xy(pa, pc); B = yz(pc, pb)
+// in operand order (pa, pb), HD), plane P (its other coordinate is pb) and plane Q (pa)
+// with their column-coordinate factor (PV, QV), the other factor (PC, QC) and the offset
+// (PD, QD). Edges: xy = ne[0..2], yz = ne[3..5], zx = ne[6..8].
+//   w = 2: hoisted xy(pa, pb); P = yz(pb, pc); Q = zx(pc, pa)
+//   w = 0: hoisted yz(pa, pb); P = zx(pb, pc); Q = xy(pc, pa)
+//   w = 1: hoisted zx(pb, pa); P = yz(pc, pb); Q = xy(pa, pc)
 // (A branch per axis is merged by the compiler into an index on the edge arrays, which
 // puts them in scratch memory.)
 __device__ __forceinline__ void fill_planes(const Setup& S, Walk& K, int w) {
@@ -209,12 +212,12 @@ __device__ __forceinline__ void fill_planes(const Setup& S, Walk& K, int w) {
     K.f[F_H0 + k] = pick(S.ne[k][0], S.ne[3 + k][0], S.ne[6 + k][1]);
     K.f[F_H1 + k] = pick(S.ne[k][1], S.ne[3 + k][1], S.ne[6 + k][0]);
     K.f[F_HD + k] = pick(S.de[k], S.de[3 + k], S.de[6 + k]);
-    K.f[F_AV + k] = pick(S.ne[3 + k][1], S.ne[k][0], S.ne[k][1]);
-    K.f[F_AC + k] = pick(S.ne[3 + k][0], S.ne[k][1], S.ne[k][0]);
-    K.f[F_AD + k] = pick(S.de[3 + k], S.de[k], S.de[k]);
-    K.f[F_BV + k] = pick(S.ne[6 + k][0], S.ne[6 + k][1], S.ne[3 + k][0]);
-    K.f[F_BC + k] = pick(S.ne[6 + k][1], S.ne[6 + k][0], S.ne[3 + k][1]);
-    K.f[F_BD + k] = pick(S.de[6 + k], S.de[6 + k], S.de[3 + k]);
+    K.f[F_PV + k] = pick(S.ne[3 + k][1], S.ne[6 + k][1], S.ne[3 + k][0]);
+    K.f[F_PC + k] = pick(S.ne[3 + k][0], S.ne[6 + k][0], S.ne[3 + k][1]);
+    K.f[F_PD + k] = pick(S.de[3 + k], S.de[6 + k], S.de[3 + k]);
+    K.f[F_QV + k] = pick(S.ne[6 + k][0], S.ne[k][0], S.ne[k][1]);
+    K.f[F_QC + k] = pick(S.ne[6 + k][1], S.ne[k][1], S.ne[k][0]);
+    K.f[F_QD + k] = pick(S.de[6 + k], S.de[k], S.de[k]);
   }
 }
 
@@ -259,20 +262,44 @@ __device__ __forceinline__ void make_walk(const Setup& S, const VoxParams& P, ui
   K.i[I_TRI] = (int)tri;
 }
 
+// The terms of a row of columns (fixed b) that do not depend on a: pb, the hoisted
+// plane's h1*pb, plane P's cc*pb, the plane test's term of pb and the slab's term of b.
+struct ColRow {
+  float pb;
+  float hb[3];
+  float kp[3];
+  float tvb;
+  float sb;
+};
+__device__ __forceinline__ ColRow col_row(const Walk& K, const VoxParams& P, int b) {
+  ColRow r;
+  r.pb = (float)b * P.unit;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    r.hb[k] = K.f[F_H1 + k] * r.pb;
+    r.kp[k] = K.f[F_PC + k] * r.pb;
+  }
+  r.tvb = K.f[F_NVA] * r.pb;
+  r.sb = K.f[F_KB] * (float)b;
+  return r;
+}
+
 // The voxels of column (a, b) that can pass: false when the column's projection test
 // fails, else [lo, hi] along w (the bbox, narrowed to the plane slab). Both passes of
 // the voxelizer call this with the same inputs, so they see the same ranges.
-__device__ __forceinline__ bool column_range(const Walk& K, float pa, float pb, int a, int b,
+__device__ __forceinline__ bool column_range(const Walk& K, float pa, const ColRow& R, int a,
                                              int& lo, int& hi) {
 #pragma unroll
   for (int k = 0; k < 3; k++)
-    if ((K.f[F_H0 + k] * pa + K.f[F_H1 + k] * pb) + K.f[F_HD + k] < 0.0f) return false;
+    if ((K.f[F_H0 + k] * pa + R.hb[k]) + K.f[F_HD + k] < 0.0f) return false;
   lo = K.i[I_G0W];
   hi = K.i[I_G1W];
   if (K.slab()) {
-    const float base = K.f[F_KA] * (float)a + K.f[F_KB] * (float)b;
-    const float l = fminf(fmaxf(base + K.f[F_ALO], (float)lo - 1.0f), (float)hi + 1.0f);
-    const float h = fmaxf(fminf(base + K.f[F_AHI], (float)hi + 1.0f), (float)lo - 1.0f);
+    // the slab ends, clamped to [lo - 1, hi + 1] (finite here: the slab is only used
+    // for finite normals and vertices) before the conversion to int
+    const float base = K.f[F_KA] * (float)a + R.sb;
+    const float l = __builtin_amdgcn_fmed3f(base + K.f[F_ALO], (float)lo - 1.0f, (float)hi + 1.0f);
+    const float h = __builtin_amdgcn_fmed3f(base + K.f[F_AHI], (float)lo - 1.0f, (float)hi + 1.0f);
     lo = max(lo, (int)ceilf(l));
     hi = min(hi, (int)floorf(h));
   }
@@ -282,11 +309,10 @@ __device__ __forceinline__ bool column_range(const Walk& K, float pa, float pb, 
 // Pre-pass: flag every brick that a column's candidate voxels fall in (a superset of the
 // bricks that will hold marked voxels).
 template <typename Act>
-__device__ __forceinline__ void touch_column(const Walk& K, const VoxParams& P, int a, int b,
-                                             Act& act) {
-  const float u = P.unit;
+__device__ __forceinline__ void touch_column(const Walk& K, const VoxParams& P, int a,
+                                             const ColRow& R, int b, Act& act) {
   int lo, hi;
-  if (!column_range(K, (float)a * u, (float)b * u, a, b, lo, hi)) return;
+  if (!column_range(K, (float)a * P.unit, R, a, lo, hi)) return;
   const int w = K.w();
   const int ua = (w == 0) ? 1 : 0;
   const int va = (w == 2) ? 1 : 2;
@@ -318,25 +344,19 @@ __device__ __forceinline__ uint64_t morton_step(uint64_t m, int k) {
 // Test the voxels of column (a, b) and hand every accepted one to act.add(code, tri);
 // cbase holds a and b spread onto their Morton axes.
 template <typename Act>
-__device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, int a, int b,
-                                            uint64_t cbase, Act& act) {
+__device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, int a,
+                                            const ColRow& R, uint64_t cbase, Act& act) {
   const float u = P.unit;
   const float pa = (float)a * u;
-  const float pb = (float)b * u;
   int lo, hi;
-  if (!column_range(K, pa, pb, a, b, lo, hi)) return;
+  if (!column_range(K, pa, R, a, lo, hi)) return;
   const int w = K.w();
-  const float tua = K.f[F_NUA] * pa, tva = K.f[F_NVA] * pb;
-  const float Ka = (w == 2) ? (tua + tva) : tua;
-  const float Kb = (w == 2) ? 0.0f : tva;
-  const float cA = (w == 2) ? pb : pa;
-  const float cB = (w == 2) ? pa : pb;
-  float KA[3], KB[3];
+  const float tua = K.f[F_NUA] * pa;
+  const float Ka = (w == 2) ? (tua + R.tvb) : tua;
+  const float Kb = (w == 2) ? 0.0f : R.tvb;
+  float KQ[3];
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    KA[k] = K.f[F_AC + k] * cA;
-    KB[k] = K.f[F_BC + k] * cB;
-  }
+  for (int k = 0; k < 3; k++) KQ[k] = K.f[F_QC + k] * pa;
   const float nw = K.f[F_NW], d1 = K.f[F_D1], d2 = K.f[F_D2];
   uint64_t mc = spread3_s((uint32_t)lo) << w;
   for (int c = lo; c <= hi; c++, mc = morton_step(mc, w)) {
@@ -346,8 +366,8 @@ __device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, i
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      if ((K.f[F_AV + k] * pc + KA[k]) + K.f[F_AD + k] < 0.0f) ok = false;
-      if ((K.f[F_BV + k] * pc + KB[k]) + K.f[F_BD + k] < 0.0f) ok = false;
+      if ((K.f[F_PV + k] * pc + R.kp[k]) + K.f[F_PD + k] < 0.0f) ok = false;
+      if ((K.f[F_QV + k] * pc + KQ[k]) + K.f[F_QD + k] < 0.0f) ok = false;
     }
     if (!ok) continue;
     // cbase and mc occupy disjoint bits: the sum is their union, and an action's
@@ -363,7 +383,12 @@ __device__ __forceinline__ void walk_column_ab(const Walk& K, const VoxParams& P
   const int w = K.w();
   const int ua = (w == 0) ? 1 : 0;
   const int va = (w == 2) ? 1 : 2;
-  walk_column(K, P, a, b, (spread3_s((uint32_t)a) << ua) | (spread3_s((uint32_t)b) << va), act);
+  const ColRow R = col_row(K, P, b);
+  if constexpr (Act::kTouch)
+    touch_column(K, P, a, R, b, act);
+  else
+    walk_column(K, P, a, R, (spread3_s((uint32_t)a) << ua) | (spread3_s((uint32_t)b) << va),
+                act);
 }
 
 // All columns of a walk, row by row, the Morton parts of a and b stepped incrementally.
@@ -375,12 +400,13 @@ __device__ __forceinline__ void walk_all_columns(const Walk& K, const VoxParams&
   uint64_t mb = spread3_s((uint32_t)K.i[I_G0V]) << va;
   const uint64_t ma0 = spread3_s((uint32_t)K.i[I_G0U]) << ua;
   for (int b = K.i[I_G0V]; b <= K.i[I_G1V]; b++, mb = morton_step(mb, va)) {
+    const ColRow R = col_row(K, P, b);
     uint64_t ma = ma0;
     for (int a = K.i[I_G0U]; a <= K.i[I_G1U]; a++, ma = morton_step(ma, ua)) {
       if constexpr (Act::kTouch)
-        touch_column(K, P, a, b, act);
+        touch_column(K, P, a, R, b, act);
       else
-        walk_column(K, P, a, b, ma | mb, act);
+        walk_column(K, P, a, R, ma | mb, act);
     }
   }
 }
@@ -687,10 +713,7 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
       const uint32_t eu = (uint32_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
       const int a = K.i[I_G0U] + (int)(local % eu);
       const int b = K.i[I_G0V] + (int)(local / eu);
-      if constexpr (Act::kTouch)
-        touch_column(K, P, a, b, act);
-      else
-        walk_column_ab(K, P, a, b, act);
+      walk_column_ab(K, P, a, b, act);
     }
     act.flush();
     __syncthreads();  // LDS (setups, offsets, s_tile) is reused by the next tile
@@ -771,10 +794,7 @@ __global__ __launch_bounds__(256) void k_vox_large(const VoxParams* __restrict__
       b = K.i[I_G0V] + (int)(local / eu);
     }
     Act act = Act::make(A, P);
-    if constexpr (Act::kTouch)
-      touch_column(K, P, a, b, act);
-    else
-      walk_column_ab(K, P, a, b, act);
+    walk_column_ab(K, P, a, b, act);
     act.flush();
   }
 }
@@ -1231,41 +1251,9 @@ __global__ __launch_bounds__(256) void k_part_hitbox(const float* __restrict__ t
                                                      uint64_t* __restrict__ hitbox) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const float* p = tris + t * stride;
-  uint64_t packed = 0;
-  bool none = false;
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    const float mn = min_std(p[a], min_std(p[3 + a], p[6 + a]));
-    const float mx = max_std(p[a], max_std(p[3 + a], p[6 + a]));
-    // rejected on this axis: mx < lo_c or mn > hi_c (NaN never rejects)
-    int below = 0, above = 0;
-    for (int c = 0; c < K; c++) {
-      below += (cube_hi[c] < mn) ? 1 : 0;
-      above += (mx < cube_lo[c]) ? 1 : 0;
-    }
-    const int clo = below, chi = K - 1 - above;
-    if (clo > chi) none = true;
-    packed |= ((uint64_t)(clo & 0xFF) | ((uint64_t)(chi & 0xFF) << 8)) << (16 * a);
-  }
-  hitbox[t] = none ? ~0ull : packed;
+  hitbox[t] = part_hitbox_of(tris + t * stride, cube_lo, cube_hi, K);
 }
 
-struct PartLoad {
-  const uint64_t* hb;
-  uint32_t cx, cy, cz;
-  __device__ uint32_t operator()(uint64_t i) const {
-    const uint64_t h = hb[i];
-    if (h == ~0ull) return 0u;
-    const uint32_t c[3] = {cx, cy, cz};
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      const uint32_t lo = (uint32_t)(h >> (16 * a)) & 0xFF, hi = (uint32_t)(h >> (16 * a + 8)) & 0xFF;
-      if (c[a] < lo || c[a] > hi) return 0u;
-    }
-    return 1u;
-  }
-};
 struct PartStore {
   uint32_t* list;
   ull* total;
@@ -1300,12 +1288,12 @@ static unsigned fill_grid(uint64_t n) {
 }
 hipError_t fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid(n)), dim3(256), 0, s, p, v, n);
+  SVO_LAUNCH(k_fill_u32, dim3(fill_grid(n)), dim3(256), 0, s, p, v, n);
   return hipGetLastError();
 }
 hipError_t fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_u64, dim3(fill_grid(n)), dim3(256), 0, s, p, v, n);
+  SVO_LAUNCH(k_fill_u64, dim3(fill_grid(n)), dim3(256), 0, s, p, v, n);
   return hipGetLastError();
 }
 
@@ -1321,7 +1309,7 @@ hipError_t upload_async(Context& c, void* dst, const void* src, size_t bytes) {
   const size_t need = (bytes + 255) & ~(size_t)255;
   if (c.stage_used + need > c.stage_cap) {
     // every copy from the staging buffer has completed after this
-    SVO_TRY(hipStreamSynchronize(c.stream));
+    SVO_TRY(stream_sync(c.stream));
     c.stage_used = 0;
     if (need > c.stage_cap) {
       if (c.stage) (void)hipHostFree(c.stage);
@@ -1359,6 +1347,7 @@ __global__ __launch_bounds__(256) void k_gather_host(GatherArgs g, volatile uint
 }
 
 hipError_t read_back(Context& c, const ReadReq* reqs, int n) {
+  t_syncs++;  // one host wait, whichever way it is done
   size_t total = 0;
   bool small = n <= kGatherMax;
   for (int k = 0; k < n; k++) {
@@ -1389,7 +1378,7 @@ hipError_t read_back(Context& c, const ReadReq* reqs, int n) {
     }
     volatile uint32_t* flag = (volatile uint32_t*)c.hread;
     const uint32_t seq = ++c.hread_seq;
-    hipLaunchKernelGGL(k_gather_host, dim3(1), dim3(256), 0, c.stream, g, flag, seq);
+    SVO_LAUNCH(k_gather_host, dim3(1), dim3(256), 0, c.stream, g, flag, seq);
     SVO_TRY(hipGetLastError());
     // poll the flag for a short while (the usual case: the stream is almost done), then
     // fall back to a blocking synchronization (long kernels ahead, or a failed launch)
@@ -1462,7 +1451,7 @@ hipError_t partition_job(Context& c, uint32_t jid, float unitlength, uint64_t gr
   SVO_TRY(upload_async(c, d_lo, lohi.data(), 2 * 256 * sizeof(float)));
   if (n) {
     ktimer_begin(c, "part_hitbox");
-    hipLaunchKernelGGL(k_part_hitbox, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream,
+    SVO_LAUNCH(k_part_hitbox, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream,
                        J.tris, stride, n, d_lo, d_hi, K, c.hitbox.as<uint64_t>());
     ktimer_end(c);
     SVO_TRY(hipGetLastError());
@@ -1691,7 +1680,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   auto tile_pass = [&](auto tag, const char* name, ull* queue, int collect, int skip) {
     using Act = decltype(tag);
     ktimer_begin(c, name);
-    hipLaunchKernelGGL((k_vox_tile<Act>), dim3(tile_grid), dim3(kTile), 0, c.stream, d_units, nu,
+    SVO_LAUNCH((k_vox_tile<Act>), dim3(tile_grid), dim3(kTile), 0, c.stream, d_units, nu,
                        d_toff, args(), queue, l_unit, l_tri, l_cols, l_walk, walk_cap, tt,
                        collect, skip);
     ktimer_end(c);
@@ -1700,7 +1689,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   auto large_pass = [&](auto tag, const char* name, int skip) {
     using Act = decltype(tag);
     ktimer_begin(c, name);
-    hipLaunchKernelGGL((k_vox_large<Act>), dim3(large_grid), dim3(256), 0, c.stream, d_units,
+    SVO_LAUNCH((k_vox_large<Act>), dim3(large_grid), dim3(256), 0, c.stream, d_units,
                        args(), l_unit, l_tri, l_off, l_walk, walk_cap, skip);
     ktimer_end(c);
     return hipGetLastError();
@@ -1735,7 +1724,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     queues = st_bc + w_bc;
     {  // pass counters, scan status words and tile queues
       const uint64_t ns = w_col + w_occ + w_bc + 3 * kQueueWords;
-      hipLaunchKernelGGL(k_zero2_u64, dim3(fill_grid(kCtrPassClear + ns)), dim3(256), 0,
+      SVO_LAUNCH(k_zero2_u64, dim3(fill_grid(kCtrPassClear + ns)), dim3(256), 0,
                          c.stream, (uint64_t*)ctr, (uint64_t)kCtrPassClear,
                          c.scan_status.as<uint64_t>(), ns);
       SVO_TRY(hipGetLastError());
@@ -1772,7 +1761,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     }
     // per-brick voxel counts (bytes packed to bits, bytes cleared) and their offsets
     ktimer_begin(c, "brick_count");
-    hipLaunchKernelGGL(k_brick_count, dim3(brick_grid(bc_bound)), dim3(256), 0, c.stream,
+    SVO_LAUNCH(k_brick_count, dim3(brick_grid(bc_bound)), dim3(256), 0, c.stream,
                        c.occ_list.as<uint32_t>(), ctr, bc_bound, d_units, d_ubase, nu,
                        c.pool8.as<uint8_t>(), pool_words, c.bcount.as<uint64_t>(),
                        dense ? c.bcode.as<uint64_t>() + nb0 : (uint64_t*)nullptr,
@@ -1810,7 +1799,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
       uvoff[0] = 0;
       uvoff[1] = hc[kCtrNumVox] & kVoxMask;
     } else {
-      hipLaunchKernelGGL(k_unit_voxels, dim3((nu + 256) / 256), dim3(256), 0, c.stream,
+      SVO_LAUNCH(k_unit_voxels, dim3((nu + 256) / 256), dim3(256), 0, c.stream,
                          c.occ_list.as<uint32_t>(), ctr, d_ubase, nu, c.bcount.as<uint64_t>(),
                          d_uvoff);
       SVO_TRY(hipGetLastError());
@@ -1918,7 +1907,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
       ktimer_begin(c, "brick_emit");
       // dense payload units: their cells are staged through LDS, kBrickVox words per wave
       const bool stage_cells = codes_now && dense && any_payload;
-      hipLaunchKernelGGL(codes_now ? k_brick_emit<true> : k_brick_emit<false>,
+      SVO_LAUNCH(codes_now ? k_brick_emit<true> : k_brick_emit<false>,
                        dim3(brick_grid(n_occ)), dim3(256),
                        stage_cells ? (256 / kWave) * kBrickVox * 4 : 0, c.stream,
                        c.occ_list.as<uint32_t>(), ctr, d_units, d_ubase, nu, pool_words,
@@ -1946,7 +1935,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
       const uint64_t keep = batch ? c.records.cap : (1 + s0) * 32;
       SVO_TRY(ensure_keep(c.records, n_records * 32, keep, c.stream));
       ktimer_begin(c, "payload_records");
-      hipLaunchKernelGGL(k_payload_records, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
+      SVO_LAUNCH(k_payload_records, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
                          c.stream, c.jobdev.as<JobDev>(), (const VoxParams*)d_units,
                          (const uint64_t*)d_uvoff, nu, c.codes.as<uint64_t>(), s0,
                          c.winner.as<uint32_t>(), nvox, c.records.as<float>(), ctr);
@@ -1955,7 +1944,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     }
     if (!dense) {  // (dense passes cleared their table entries in k_brick_count)
       ktimer_begin(c, "brick_clear");
-      hipLaunchKernelGGL(k_brick_clear,
+      SVO_LAUNCH(k_brick_clear,
                          dim3((unsigned)std::min<uint64_t>((n_occ + 255) / 256, 4096)),
                          dim3(256), 0, c.stream, c.occ_list.as<uint32_t>(), ctr,
                          c.table.as<uint32_t>());
@@ -1974,7 +1963,7 @@ hipError_t ensure_codes(Context& c) {
   if (c.n_bricks) {
     const unsigned grid =
         (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((c.n_bricks * kWave + 255) / 256, 8192));
-    hipLaunchKernelGGL(k_bricks_codes, dim3(grid), dim3(256), 0, c.stream,
+    SVO_LAUNCH(k_bricks_codes, dim3(grid), dim3(256), 0, c.stream,
                        c.bwords.as<uint64_t>(), c.bcode.as<uint64_t>(), c.bvoff.as<uint64_t>(),
                        c.n_bricks, c.codes.as<uint64_t>());
     SVO_TRY(hipGetLastError());

@@ -43,7 +43,7 @@ class RunInfo(C.Structure):
     _fields_ = [("n_partitions", C.c_uint64), ("nfilled", C.c_uint64),
                 ("n_nodes", C.c_uint64), ("n_data", C.c_uint64), ("hazard", C.c_int),
                 ("ms_partition", C.c_float), ("ms_voxelize", C.c_float),
-                ("ms_build", C.c_float)]
+                ("ms_build", C.c_float), ("launches", C.c_uint32), ("host_syncs", C.c_uint32)]
 
 
 class Job(C.Structure):

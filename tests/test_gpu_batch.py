@@ -121,3 +121,34 @@ def test_batch_device_meshes_and_plain_after():
         info, nodes, data = _plain(b, jobs[4])
         info2, nodes2, data2 = _plain(p, jobs[4])
         assert nodes.tobytes() == nodes2.tobytes() and data.tobytes() == data2.tobytes()
+
+
+def test_batch_with_levels_jobs():
+    """-levels jobs in a batch (run through the plain path, their records appended after
+    the shared jobs'): every job byte-equal to svo_run alone; a batch of -levels jobs only."""
+    _say("batch with -levels jobs")
+    jobs = _mix(6)[:8]
+    jobs[1] = dict(jobs[1], levels=True)
+    jobs[5] = dict(jobs[5], levels=True)
+    jobs.append(dict(records=generators.terrain(n=41, seed=23, payload=True), payload=True,
+                     gridsize=256, memory_limit_mb=2, levels=True))  # 8 partitions
+    with SvoContext(0) as b, SvoContext(0) as p:
+        res, tot = b.batch_run(jobs)
+        assert tot.n_nodes == sum(r.n_nodes for r in res)
+        for i, (j, r) in enumerate(zip(jobs, res)):
+            info = p.run(j["records"], j["payload"], j["gridsize"], j.get("memory_limit_mb", 2048),
+                         bbox=j.get("bbox", (0, 0, 0, 1, 1, 1)),
+                         color_mode=j.get("color_mode", 0), levels=j.get("levels", False))
+            assert (r.n_nodes, r.n_data, r.nfilled) == (info.n_nodes, info.n_data, info.nfilled), i
+            assert b.nodes_range(r.node_offset, r.n_nodes).tobytes() == \
+                p.nodes(info.n_nodes).tobytes(), i
+            assert b.data_range(r.data_offset, r.n_data).tobytes() == \
+                p.data(info.n_data).tobytes(), i
+        only = [jobs[1], jobs[-1]]
+        res, _ = b.batch_run(only)
+        for j, r in zip(only, res):
+            nodes_w, data_w, _, _, _ = oracle.pipeline(
+                j["records"], j["payload"], j["gridsize"], j.get("memory_limit_mb", 2048),
+                color_mode=j.get("color_mode", 0), levels=True)
+            assert b.nodes_range(r.node_offset, r.n_nodes).tobytes() == nodes_w.tobytes()
+            assert b.data_range(r.data_offset, r.n_data).tobytes() == data_w.tobytes()

