@@ -291,6 +291,28 @@ typedef struct svo_server_stats {
 } svo_server_stats;
 int svo_server_stats_get(svo_server* srv, svo_server_stats* out);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc). The device
+ * work of svo_partition (partitioner.cpp:43-77,101-149) on a mesh already in device
+ * memory, without a context: its working memory is a fixed array in the library's device
+ * image (about 9 MiB per device, held until the process ends), so the call allocates
+ * nothing and does not synchronize; it is ordered on hip_stream (NULL: the null stream)
+ * of the calling thread's current device.
+ * d_records: n_triangles <= SVO_NOALLOC_MAX_TRIANGLES device records (9 or 21 floats);
+ * n_partitions a power of 8 up to SVO_NOALLOC_MAX_PARTITIONS. Results, in device memory:
+ * d_lists[lists_capacity] the per-partition triangle lists concatenated in partition
+ * order (partition j's list starts at the sum of the counts before j; one partition lists
+ * every triangle, as svo_partition does), d_counts[n_partitions + 1] the counts and then a
+ * status word: 0 when every list is complete, 1 when they exceed lists_capacity (entries
+ * beyond it are dropped), 2 on an internal scan failure. Calls that overlap on one device
+ * share its working memory and race: run them one after another. */
+#define SVO_NOALLOC_MAX_TRIANGLES (1u << 20)
+#define SVO_NOALLOC_MAX_PARTITIONS 4096u
+int svo_partition_noalloc(void* hip_stream, const float* d_records, uint64_t n_triangles,
+                          int payload, float unitlength, uint64_t gridsize,
+                          uint64_t n_partitions, uint32_t* d_lists, uint64_t lists_capacity,
+                          uint64_t* d_counts);
+
 /* Per-kernel device timing with HIP events on the context's stream, the counterpart of
  * the reference's stage timers (globals.h:8-29, main.cpp:45-56; those truncate to whole
  * milliseconds, timer.h:59). While on, each instrumented kernel launch is bracketed by

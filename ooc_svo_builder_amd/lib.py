@@ -100,6 +100,7 @@ SIGNATURES = {
     "svo_stream_end": (_i32, [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_i32)]),
     "svo_run_stream": (_i32, [_vp, _vp, _u64, _i32, _i32, _vp, _u64, _u64, _i32, _i32, _vp,
                               _vp, C.POINTER(RunInfo)]),
+    "svo_partition_noalloc": (_i32, [_vp, _vp, _u64, _i32, _f32, _u64, _u64, _vp, _u64, _vp]),
     "svo_batch_run": (_i32, [_vp, C.POINTER(Job), C.c_uint32, C.POINTER(JobResult),
                              C.POINTER(RunInfo)]),
     "svo_ctx_autotune": (_i32, [_vp, C.POINTER(Job), C.c_uint32, C.POINTER(C.c_int * 4)]),
@@ -150,6 +151,17 @@ def _check(rc: int):
 
 def estimate_partitions(gridsize: int, memory_limit_mb: int) -> int:
     return int(load_library().svo_estimate_partitions(gridsize, memory_limit_mb))
+
+
+def partition_noalloc(stream_ptr: int, records, payload: bool, unitlength: float, gridsize: int,
+                      n_partitions: int, lists, counts):
+    """svo_partition_noalloc on device tensors: records (N, 9|21) float32, lists int32/uint32
+    (its length is the capacity), counts int64 of n_partitions + 1 entries. Asynchronous
+    on the given stream."""
+    _check(load_library().svo_partition_noalloc(
+        C.c_void_p(stream_ptr), C.c_void_p(records.data_ptr()), int(records.shape[0]),
+        int(payload), float(np.float32(unitlength)), gridsize, n_partitions,
+        C.c_void_p(lists.data_ptr()), int(lists.numel()), C.c_void_p(counts.data_ptr())))
 
 
 def _ptr(a: np.ndarray):
