@@ -495,6 +495,10 @@ class SvoContext:
     def voxel_count(self) -> int:
         return int(load_library().svo_voxel_count(self._h))
 
+    def copy_codes_to(self, dst_ptr: int, first: int, count: int):
+        """Copy sorted voxel codes [first, first + count) to a device (or host) address."""
+        _check(load_library().svo_copy_codes(self._h, C.c_void_p(dst_ptr), first, count))
+
     def codes(self) -> np.ndarray:
         n = self.voxel_count()
         out = np.empty(n, np.uint64)

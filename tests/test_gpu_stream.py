@@ -202,26 +202,35 @@ def _node_properties(ctx, n_nodes, nfilled):
     assert children == n_nodes - 1
 
 
-def _node_checksums(ctx, n_nodes):
-    """Per chunk of 2^25 node records: the int64 sum of its words and the sum of each
-    word times (2 i + 1), i its index in the chunk (both wrapping mod 2^64)."""
+def _partition_codes(ctx, d_tris, G, P, parts):
+    """The sorted voxel codes of partitions 'parts' of a P-partition run, on the device
+    (voxelize only: svo_partition, then svo_voxelize per partition)."""
     import torch
-    chunk = 1 << 25
-    buf = torch.empty(chunk * 24, dtype=torch.uint8, device="cuda:0")
-    wts = torch.arange(chunk * 3, dtype=torch.int64, device="cuda:0") * 2 + 1
-    out = []
-    for s in range(0, n_nodes, chunk):
-        k = min(chunk, n_nodes - s)
-        ctx.copy_nodes_to(buf.data_ptr(), s, k)
-        w = buf[:k * 24].view(torch.int64)
-        out.append((int(w.sum()), int((w * wts[:3 * k]).sum())))
+    ctx.load_mesh(d_tris, False)
+    u = np.float32(1.0) / np.float32(G)
+    counts = ctx.partition(u, G, P)
+    span = G ** 3 // P
+    for p in parts:
+        if counts[p]:
+            ctx.voxelize(p, p * span, (p + 1) * span, u, G)
+    n = ctx.voxel_count()
+    out = torch.empty(n, dtype=torch.int64, device="cuda:0")
+    if n:
+        ctx.copy_codes_to(out.data_ptr(), 0, n)
     return out
 
 
 def test_sphere_soup_full_size_8_partitions():
     """-s 8192 -l 65536 on 5x10^7 triangles: 8 partitions of 4096^3 (64 GiB dense grid
-    each), streamed build in HBM; properties at full size, and the same records with 64
-    partitions (chunk checksums of the two runs' node arrays)."""
+    each), streamed build in HBM, with size-independent properties of its octree.
+
+    Against 64 partitions (-l 8192): the reference clamps a triangle's grid bbox to the
+    partition (voxelizer.cpp:198-204), so a triangle whose bbox starts exactly on a
+    partition face is also tested against the last voxel layer of the partition below
+    (trunc gives the face index, the clamp pulls it to the layer under it) and can mark
+    voxels there that a coarser split never tests. With u = 1/8192 (exact), every voxel of
+    the 8-partition run is therefore a voxel of the 64-partition run, and each extra voxel
+    lies in the last layer of a 2048^3 cube that is not the last layer of a 4096^3 one."""
     import torch
     workers = max(1, min(16, os.cpu_count() or 1))
     t = time.time()
@@ -229,21 +238,32 @@ def test_sphere_soup_full_size_8_partitions():
     _say("soup of 5e7 triangles generated in %.1f s" % (time.time() - t))
     d_tris = torch.from_numpy(tris).cuda()
     del tris
+    G = 8192
     with SvoContext(0) as ctx:
         t = time.time()
-        info = ctx.run(d_tris, False, 8192, 65536)
+        info = ctx.run(d_tris, False, G, 65536)
         _say("P=%d: %d voxels, %d nodes in %.1f s (partition %.1f, voxelize %.1f, build %.1f ms)"
              % (info.n_partitions, info.nfilled, info.n_nodes, time.time() - t,
                 info.ms_partition, info.ms_voxelize, info.ms_build))
         assert info.n_partitions == 8 and info.n_data == 2 and info.nfilled > 10 ** 8
         _node_properties(ctx, info.n_nodes, info.nfilled)
-        ref = _node_checksums(ctx, info.n_nodes)
-        t = time.time()
-        info64 = ctx.run(d_tris, False, 8192, 8192)
-        _say("P=%d: %d nodes in %.1f s" % (info64.n_partitions, info64.n_nodes, time.time() - t))
-        assert info64.n_partitions == 64
-        assert (info64.nfilled, info64.n_nodes, info64.n_data) == (
-            info.nfilled, info.n_nodes, info.n_data)
-        got = _node_checksums(ctx, info64.n_nodes)
-        bad = [i for i, (a, b) in enumerate(zip(ref, got)) if a != b]
-        assert not bad, "chunks %s of %d differ" % (bad[:10], len(ref))
+        info64 = ctx.run(d_tris, False, G, 8192)
+        _say("P=%d: %d voxels, %d nodes" % (info64.n_partitions, info64.nfilled, info64.n_nodes))
+        assert info64.n_partitions == 64 and info64.nfilled >= info.nfilled
+        extra_total = 0
+        for i in range(8):
+            a = _partition_codes(ctx, d_tris, G, 8, [i])
+            b = _partition_codes(ctx, d_tris, G, 64, range(8 * i, 8 * i + 8))
+            pos = torch.searchsorted(b, a)
+            assert bool((pos < len(b)).all()) and bool((b[pos.clamp(max=len(b) - 1)] == a).all()), i
+            keep = torch.ones(len(b), dtype=torch.bool, device="cuda:0")
+            keep[pos] = False
+            extra = b[keep].cpu().numpy().astype(np.uint64)
+            extra_total += len(extra)
+            for code in extra:
+                xyz = [sum(((int(code) >> (3 * k + a_)) & 1) << k for k in range(21))
+                       for a_ in range(3)]
+                assert any(c % 2048 == 2047 and c % 4096 != 4095 for c in xyz), (i, xyz)
+            del a, b, pos, keep
+        assert info.nfilled + extra_total == info64.nfilled
+        _say("64 partitions: %d extra voxels, all on the partition faces" % extra_total)

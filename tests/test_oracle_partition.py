@@ -82,3 +82,36 @@ def test_partition_matches_box_test():
         hit = ~((mx < lo).any(1) | (mn > hi).any(1))
         assert np.array_equal(np.nonzero(hit)[0], lists[j].astype(np.int64)), j
         assert counts[j] == hit.sum()
+
+
+def test_partition_clamp_marks_the_layer_below_a_face():
+    """voxelizer.cpp:198-204 clamps a triangle's truncated grid bbox to the partition: a
+    triangle whose bbox starts exactly on a partition face (y = 128 voxels, a face of the
+    64-partition split of a 512^3 grid but not of the 8-partition one) is in the list of
+    the partition below too (the bbox test is inclusive, intersection.h:49-53), where the
+    clamp turns its first row index 128 into 127, and the conservative test marks the
+    voxels whose top face it touches. So the output depends on the partition count: 8
+    partitions give a subset of 64 partitions' voxels, the difference in layer y = 127."""
+    G = 512
+    u = 1.0 / G
+    t = np.array([[100.3 * u, 128 * u, 50.2 * u, 103.7 * u, 128 * u, 51.1 * u,
+                   101.5 * u, 131.2 * u, 52.9 * u]], np.float32)
+    _, _, _, p8, n8 = oracle.pipeline(t, False, G, 16)
+    _, _, _, p64, n64 = oracle.pipeline(t, False, G, 2)
+    assert (p8, p64) == (8, 64)
+    c8, _ = oracle.voxelize(t, False, 0, G ** 3, np.float32(u))  # one partition: no clamp
+    c8 = np.sort(c8)
+    span = G ** 3 // 64
+    c64 = []
+    for p in range(64):
+        counts, lists = oracle.partition(t, False, (0, 0, 0, 1, 1, 1), G, 64)
+        if counts[p]:
+            codes, _ = oracle.voxelize(t[lists[p].astype(np.int64)], False, p * span,
+                                       (p + 1) * span, np.float32(u))
+            c64.append(codes)
+    c64 = np.sort(np.concatenate(c64))
+    assert (len(c8), len(c64)) == (n8, n64) and n64 > n8
+    assert np.all(np.isin(c8, c64))
+    extra = np.setdiff1d(c64, c8)
+    _, y, _ = oracle.morton_decode(extra)
+    assert np.all(y == 127)
