@@ -90,6 +90,17 @@ int svo_partition(svo_ctx* ctx, float unitlength, uint64_t gridsize, uint64_t n_
  * still write .tripdata files. */
 int svo_partition_indices(svo_ctx* ctx, uint64_t partition, uint32_t* host_out);
 
+/* partitioner.cpp:80-149 with trip_tools.h:104-125, for callers that still want the
+ * reference's partition files on disk (the reference writes them as temporaries and
+ * deletes them at exit, partitioner.cpp:31-40): after svo_partition, write
+ * <base>.trip (the header, bbox printed with 6 significant digits, geo_only from the
+ * loaded mesh) and <base>_<i>.tripdata with the records of partition i's triangles in mesh
+ * order, for every non-empty partition (one partition: the whole mesh). base is the
+ * reference's <tri base><gridsize>_<n_partitions>; bbox and version come from the .tri
+ * header. */
+int svo_write_trip(svo_ctx* ctx, const char* base, uint64_t gridsize, const float bbox[6],
+                   int version);
+
 /* voxelizer.cpp:138-307 followed by main.cpp:354-385: voxelize one partition (its list
  * from svo_partition, or the whole mesh when no partitioning was done) over Morton
  * codes [morton_start, morton_end) and append its voxels, in Morton order, to the

@@ -38,6 +38,7 @@ int color = SVO_COLOR_MODEL;
 bool generate_levels = false;
 bool verbose = false;
 int device = 0;
+bool write_trip = false;  // keep the reference's .trip/.tripdata partition files
 
 struct TriInfo {
   string base;
@@ -150,6 +151,8 @@ bool parseProgramParameters(int argc, char* argv[]) {
     } else if (a == "--device" && has_next) {
       device = atoi(argv[i + 1]);
       i++;
+    } else if (a == "--write-trip") {
+      write_trip = true;
     } else if (a == "-h") {
       printHelp();
       return false;
@@ -321,6 +324,8 @@ int main(int argc, char* argv[]) {
   if (verbose)
     for (uint64_t i = 0; i < n_partitions; i++)
       cout << "  partition " << i << " - tri_count: " << counts[i] << endl;
+  if (write_trip && svo_write_trip(ctx, base.c_str(), gridsize, ti.bbox, ti.version) != SVO_OK)
+    return die(ctx, "svo_write_trip");
 
   // VOXELIZATION + SVO (main.cpp:311-387): the octree is streamed into the output files
   // partition by partition (OctreeBuilder.cpp:4-168 owns the two files the same way),

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <new>
 #include <sstream>
 #include <string>
@@ -469,6 +470,83 @@ int svo_partition_indices(svo_ctx* ctx, uint64_t partition, uint32_t* host_out) 
     TRYC(copy_sync(c, host_out, c.part_lists.as<uint32_t>() + J.part_off[partition],
                    J.part_cnt[partition] * 4),
          "svo_partition_indices");
+  return SVO_OK;
+}
+
+int svo_write_trip(svo_ctx* ctx, const char* base, uint64_t gridsize, const float bbox[6],
+                   int version) {
+  CHECK_CTX();
+  Context& c = ctx->c;
+  const Job& J = c.jobs[0];
+  if (!base || !bbox || c.batch_state || J.n_parts == 0 || J.part_cnt.size() != J.n_parts) {
+    g_err = "svo_write_trip: partition the loaded mesh first (svo_partition)";
+    return SVO_ERR_INVALID;
+  }
+  const int stride = c.payload ? 21 : 9;
+  const uint64_t rec = (uint64_t)stride * 4;
+  // partition i's records in mesh order, through a device staging buffer and a pinned
+  // host buffer, 1 Mi triangles at a time (BBoxBuffer.h:56-67 opens a partition's file at
+  // its first flush, so empty partitions get none)
+  const uint64_t chunk = 1u << 20;
+  DevBuf dstage;
+  void* hstage = nullptr;
+  hipError_t e = ensure(dstage, chunk * rec);
+  if (e == hipSuccess) e = hipHostMalloc(&hstage, chunk * rec, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    release(dstage);
+    return fail(e, "svo_write_trip");
+  }
+  int rc = SVO_OK;
+  for (uint64_t i = 0; i < J.n_parts && rc == SVO_OK; i++) {
+    const uint64_t n = J.part_cnt[i];
+    if (!n) continue;
+    const std::string fn = std::string(base) + "_" + std::to_string(i) + ".tripdata";
+    FILE* f = fopen(fn.c_str(), "wb");
+    if (!f) {
+      g_err = "svo_write_trip: cannot open " + fn;
+      rc = SVO_ERR_IO;
+      break;
+    }
+    const uint32_t* list = J.n_parts > 1 ? c.part_lists.as<uint32_t>() + J.part_off[i] : nullptr;
+    for (uint64_t k = 0; k < n && rc == SVO_OK; k += chunk) {
+      const uint64_t m = std::min(chunk, n - k);
+      e = list ? gather_records(c, c.tris.as<float>(), stride, list + k, m, dstage.as<float>())
+               : hipMemcpyAsync(dstage.p, c.tris.as<char>() + k * rec, m * rec,
+                                hipMemcpyDeviceToDevice, c.stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(hstage, dstage.p, m * rec, hipMemcpyDeviceToHost, c.stream);
+      if (e == hipSuccess) e = stream_sync(c.stream);
+      if (e != hipSuccess) {
+        rc = fail(e, "svo_write_trip");
+      } else if (fwrite(hstage, 1, m * rec, f) != m * rec) {
+        g_err = "svo_write_trip: writing " + fn + " failed";
+        rc = SVO_ERR_IO;
+      }
+    }
+    if (fclose(f) != 0 && rc == SVO_OK) {
+      g_err = "svo_write_trip: writing " + fn + " failed";
+      rc = SVO_ERR_IO;
+    }
+  }
+  release(dstage);
+  (void)hipHostFree(hstage);
+  if (rc != SVO_OK) return rc;
+  // trip_tools.h:104-125 (floats in the default 6 significant digits)
+  std::ofstream o((std::string(base) + ".trip").c_str());
+  o << "#trip " << version << std::endl;
+  o << "gridsize " << gridsize << std::endl;
+  o << "n_triangles " << c.n_tris << std::endl;
+  o << "bbox  " << bbox[0] << " " << bbox[1] << " " << bbox[2] << " " << bbox[3] << " "
+    << bbox[4] << " " << bbox[5] << std::endl;
+  o << "geo_only " << (c.payload ? 0 : 1) << std::endl;
+  o << "n_partitions " << J.n_parts << std::endl;
+  for (uint64_t i = 0; i < J.n_parts; i++) o << i << " " << J.part_cnt[i] << std::endl;
+  o << "END" << std::endl;
+  o.close();
+  if (!o) {
+    g_err = std::string("svo_write_trip: writing ") + base + ".trip failed";
+    return SVO_ERR_IO;
+  }
   return SVO_OK;
 }
 

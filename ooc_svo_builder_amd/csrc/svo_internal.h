@@ -306,6 +306,10 @@ hipError_t read_back(Context& c, const ReadReq* reqs, int n);
 // c.part_lists (reset c.part_used first to start over).
 hipError_t partition_job(Context& c, uint32_t j, float unitlength, uint64_t gridsize,
                          uint64_t n_parts);
+// out[k] = the record (stride floats) of triangle list[k], or of triangle k when list is
+// null, for k < n, on c.stream.
+hipError_t gather_records(Context& c, const float* tris, int stride, const uint32_t* list,
+                          uint64_t n, float* out);
 // Voxelize the units in the same launches and append their voxels to the stream.
 // batch = false: every unit belongs to job 0, whose voxels accumulate over calls;
 // batch = true: the units hold the whole of their jobs, whose ranges are laid out here.

@@ -284,6 +284,44 @@ __device__ __forceinline__ ColRow col_row(const Walk& K, const VoxParams& P, int
   return r;
 }
 
+// Rows at least this many columns wide get their columns narrowed to the interval that
+// can pass the projection test (row_interval); narrower rows walk every column.
+constexpr int kRowSpanMin = 5;
+
+// Columns [a0, a1] of row b: a superset of the columns whose hoisted projection test
+// passes (every column outside fails it). An edge's test value at column a,
+// fl(fl(fl(h0 * fl(a * u)) + hb) + hd), differs from the real h0 u a + hb + hd by at most
+// 5 eps (|h0| |a u| + |hb| + |hd|) (four roundings, eps = 2^-24); with E above that bound,
+// a column that passes has h0 u a >= -(hb + hd) - E, a half-line in a. Its end is
+// widened by one column for the rounding of its own evaluation (an approximate
+// reciprocal, well under a column at these magnitudes). Non-finite constants narrow
+// nothing.
+__device__ __forceinline__ void row_interval(const Walk& K, const ColRow& R, float u, int& a0,
+                                             int& a1) {
+  a0 = K.i[I_G0U];
+  a1 = K.i[I_G1U];
+  if (a1 - a0 + 1 < kRowSpanMin) return;
+  const float flo = (float)a0 - 1.0f, fhi = (float)a1 + 1.0f;
+  const float amax = fmaxf(fabsf(flo), fabsf(fhi));
+  float lo = flo, hi = fhi;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float h0 = K.f[F_H0 + k], hb = R.hb[k], hd = K.f[F_HD + k];
+    const float E = 1e-6f * ((fabsf(h0) * amax) * fabsf(u) + fabsf(hb) + fabsf(hd)) + 1e-30f;
+    const float rhs = -(hb + hd) - E;
+    const float hu = h0 * u;
+    if (!(fabsf(rhs) <= 3.0e38f) || !(fabsf(hu) <= 3.0e38f)) continue;
+    if (hu > 0.0f)
+      lo = fmaxf(lo, rhs * __builtin_amdgcn_rcpf(hu) - 1.0f);
+    else if (hu < 0.0f)
+      hi = fminf(hi, rhs * __builtin_amdgcn_rcpf(hu) + 1.0f);
+    else if (rhs > 0.0f)  // h0 u = 0: no column of the row passes
+      hi = flo;
+  }
+  a0 = max(a0, (int)ceilf(__builtin_amdgcn_fmed3f(lo, flo, fhi)));
+  a1 = min(a1, (int)floorf(__builtin_amdgcn_fmed3f(hi, flo, fhi)));
+}
+
 // The voxels of column (a, b) that can pass: false when the column's projection test
 // fails, else [lo, hi] along w (the bbox, narrowed to the plane slab). Both passes of
 // the voxelizer call this with the same inputs, so they see the same ranges.
@@ -376,22 +414,42 @@ __device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, i
   }
 }
 
-// Column (a, b) of a walk, for callers that do not step the Morton parts themselves.
+// Columns [i0, i1] of row b (ma = i0 spread onto axis ua, mb = b onto axis va).
 template <typename Act>
-__device__ __forceinline__ void walk_column_ab(const Walk& K, const VoxParams& P, int a, int b,
-                                               Act& act) {
+__device__ __forceinline__ void walk_row_cols(const Walk& K, const VoxParams& P, int b,
+                                              const ColRow& R, int i0, int i1, uint64_t ma,
+                                              uint64_t mb, int ua, Act& act) {
+  for (int a = i0; a <= i1; a++, ma = morton_step(ma, ua)) {
+    if constexpr (Act::kTouch)
+      touch_column(K, P, a, R, b, act);
+    else
+      walk_column(K, P, a, R, ma | mb, act);
+  }
+}
+
+// Columns [a_lo, a_hi] of row b of a walk, narrowed to the row's interval when kNarrow
+// (a span of one or a few columns is cheaper to test column by column).
+template <bool kNarrow, typename Act>
+__device__ __forceinline__ void walk_row_span(const Walk& K, const VoxParams& P, int b, int a_lo,
+                                              int a_hi, Act& act) {
+  const ColRow R = col_row(K, P, b);
+  int i0 = a_lo, i1 = a_hi;
+  if constexpr (kNarrow) {
+    row_interval(K, R, P.unit, i0, i1);
+    i0 = max(i0, a_lo);
+    i1 = min(i1, a_hi);
+    if (i0 > i1) return;
+  }
   const int w = K.w();
   const int ua = (w == 0) ? 1 : 0;
   const int va = (w == 2) ? 1 : 2;
-  const ColRow R = col_row(K, P, b);
-  if constexpr (Act::kTouch)
-    touch_column(K, P, a, R, b, act);
-  else
-    walk_column(K, P, a, R, (spread3_s((uint32_t)a) << ua) | (spread3_s((uint32_t)b) << va),
-                act);
+  walk_row_cols(K, P, b, R, i0, i1, spread3_s((uint32_t)i0) << ua,
+                spread3_s((uint32_t)b) << va, ua, act);
 }
 
-// All columns of a walk, row by row, the Morton parts of a and b stepped incrementally.
+// All columns of a walk, row by row, the Morton parts of a and b stepped incrementally;
+// rows are narrowed to their interval for actions that walk segments of several columns
+// (rows narrower than kRowSpanMin columns are walked whole).
 template <typename Act>
 __device__ __forceinline__ void walk_all_columns(const Walk& K, const VoxParams& P, Act& act) {
   const int w = K.w();
@@ -401,12 +459,14 @@ __device__ __forceinline__ void walk_all_columns(const Walk& K, const VoxParams&
   const uint64_t ma0 = spread3_s((uint32_t)K.i[I_G0U]) << ua;
   for (int b = K.i[I_G0V]; b <= K.i[I_G1V]; b++, mb = morton_step(mb, va)) {
     const ColRow R = col_row(K, P, b);
-    uint64_t ma = ma0;
-    for (int a = K.i[I_G0U]; a <= K.i[I_G1U]; a++, ma = morton_step(ma, ua)) {
-      if constexpr (Act::kTouch)
-        touch_column(K, P, a, R, b, act);
-      else
-        walk_column(K, P, a, R, ma | mb, act);
+    if constexpr (Act::kSegCols > 1) {
+      int i0, i1;
+      row_interval(K, R, P.unit, i0, i1);
+      if (i0 > i1) continue;
+      const uint64_t ma = (i0 == K.i[I_G0U]) ? ma0 : (spread3_s((uint32_t)i0) << ua);
+      walk_row_cols(K, P, b, R, i0, i1, ma, mb, ua, act);
+    } else {
+      walk_row_cols(K, P, b, R, K.i[I_G0U], K.i[I_G1U], ma0, mb, ua, act);
     }
   }
 }
@@ -427,6 +487,10 @@ struct ActArgs {
 // flagged bricks in Morton order (table entry = slot + 1) and lists them.
 struct TouchAction {
   static constexpr bool kTouch = true;
+  // columns of a row per item of the small tier: with more than one, an item walks
+  // its segment narrowed to the row's interval; atomic actions take one (the lanes
+  // of a wave then update neighbouring cells, whose atomics share cache lines)
+  static constexpr int kSegCols = 8;
   uint32_t* table;
   uint64_t brick0;  // first brick of the unit's table segment
   __device__ static TouchAction make(const ActArgs& A, const VoxParams& P) {
@@ -459,6 +523,7 @@ typedef __attribute__((address_space(1))) uint32_t gword_t;
 template <bool kWide, bool kLaneOrder = true>
 struct MarkDenseT {
   static constexpr bool kTouch = false;
+  static constexpr int kSegCols = kWide ? 1 : 8;
   gword_t* table;
   gbyte_t* bytes;
   uint64_t base;  // code of the first voxel of the unit's first brick
@@ -508,6 +573,7 @@ using MarkDenseWideBatchAction = MarkDenseT<true, false>;
 
 struct MarkPoolAction {
   static constexpr bool kTouch = false;
+  static constexpr int kSegCols = 8;
   const gword_t* table;
   gbyte_t* bytes;  // the slot pool
   ull* ctr;
@@ -546,6 +612,7 @@ struct MarkPoolAction {
 // Second payload pass: lowest triangle index per voxel rank.
 struct WinnerAction {
   static constexpr bool kTouch = false;
+  static constexpr int kSegCols = 1;
   const uint32_t* table;
   const ull* pool;  // bit form of the bricks, written by the compaction
   const uint32_t* wprefix;
@@ -592,9 +659,10 @@ __device__ __forceinline__ uint32_t upper_index(const T* __restrict__ a, uint32_
 // Small tier, persistent: blocks take tiles of tt <= kTileTris consecutive triangles of one
 // unit from a device work queue (tile_off[u] = first tile of unit u); tt is smaller for
 // passes with few triangles, so that their tiles still spread over the CUs. For a tile, each thread builds
-// its triangle's walk constants into LDS; a block scan lays out the (triangle, column)
-// items; then the threads share the items, one column each, so the work per lane is a few
-// voxel tests regardless of triangle size. Triangles with more than small_cols columns
+// its triangle's walk constants into LDS; a block scan lays out the (triangle, row
+// segment) items, a segment being Act::kSegCols neighbouring columns of one row; then the
+// threads share the items, so the work per lane is a few voxel tests regardless of
+// triangle size. Triangles with more than small_cols columns
 // (or non-slab triangles with more than small_cols * 8 tests) go to the column-parallel
 // tier. Triangles with at most direct_cols columns skip the item layout: their own
 // thread walks their columns from registers (the per-item lookup and constant loads
@@ -627,6 +695,7 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
   __shared__ uint32_t soff[kTileTris + 1];
   __shared__ uint32_t wsum[kTile / kWave];
   __shared__ uint32_t s_tile;
+  constexpr int kSeg = Act::kSegCols;
   fill_spread_table();  // read after the first barrier of the tile loop
   ull* ctr = A.ctr;
   const uint32_t total_tiles = tile_off[n_units];
@@ -688,7 +757,9 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
         walk_all_columns(K, P, act);
         act.flush();
       } else {
-        items = (uint32_t)ncols;
+        // items: the rows' segments of kSeg columns
+        const uint32_t eu = (uint32_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
+        items = (uint32_t)(K.i[I_G1V] - K.i[I_G0V] + 1) * ((eu + kSeg - 1) / kSeg);
 #pragma unroll
         for (int f = 0; f < kWalkF; f++) sk[f][tid] = __float_as_uint(K.f[f]);
 #pragma unroll
@@ -711,9 +782,10 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
       for (int f = 0; f < kWalkI; f++) K.i[f] = (int)sk[kWalkF + f][lo];
       const uint32_t local = it - soff[lo];
       const uint32_t eu = (uint32_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
-      const int a = K.i[I_G0U] + (int)(local % eu);
-      const int b = K.i[I_G0V] + (int)(local / eu);
-      walk_column_ab(K, P, a, b, act);
+      const uint32_t nseg = (eu + kSeg - 1) / kSeg;
+      const int a = K.i[I_G0U] + (int)(local % nseg) * kSeg;
+      const int b = K.i[I_G0V] + (int)(local / nseg);
+      walk_row_span<(kSeg > 1)>(K, P, b, a, min(a + kSeg - 1, K.i[I_G1U]), act);
     }
     act.flush();
     __syncthreads();  // LDS (setups, offsets, s_tile) is reused by the next tile
@@ -794,7 +866,7 @@ __global__ __launch_bounds__(256) void k_vox_large(const VoxParams* __restrict__
       b = K.i[I_G0V] + (int)(local / eu);
     }
     Act act = Act::make(A, P);
-    walk_column_ab(K, P, a, b, act);
+    walk_row_span<false>(K, P, b, a, a, act);
     act.flush();
   }
 }
@@ -1252,6 +1324,28 @@ __global__ __launch_bounds__(256) void k_part_hitbox(const float* __restrict__ t
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   hitbox[t] = part_hitbox_of(tris + t * stride, cube_lo, cube_hi, K);
+}
+
+// out[k] = the record of triangle list[k] (list == nullptr: triangle k), stride floats each
+__global__ __launch_bounds__(256) void k_gather_records(const float* __restrict__ tris, int stride,
+                                                        const uint32_t* __restrict__ list,
+                                                        uint64_t n, float* __restrict__ out) {
+  const uint64_t total = n * (uint64_t)stride;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = e / (uint64_t)stride, f = e - k * (uint64_t)stride;
+    const uint64_t t = list ? list[k] : k;
+    out[e] = tris[t * stride + f];
+  }
+}
+
+hipError_t gather_records(Context& c, const float* tris, int stride, const uint32_t* list,
+                          uint64_t n, float* out) {
+  if (!n) return hipSuccess;
+  const uint64_t total = n * (uint64_t)stride;
+  SVO_LAUNCH(k_gather_records, dim3((unsigned)std::min<uint64_t>((total + 255) / 256, 16384)),
+             dim3(256), 0, c.stream, tris, stride, list, n, out);
+  return hipGetLastError();
 }
 
 struct PartStore {

@@ -81,6 +81,7 @@ SIGNATURES = {
     "svo_mesh_load_file": (_i32, [_vp, C.c_char_p, _u64, _i32, C.POINTER(_u64)]),
     "svo_partition": (_i32, [_vp, _f32, _u64, _u64, _vp]),
     "svo_partition_indices": (_i32, [_vp, _u64, _vp]),
+    "svo_write_trip": (_i32, [_vp, C.c_char_p, _u64, _vp, _i32]),
     "svo_voxelize": (_i32, [_vp, _u64, _u64, _u64, _f32, _u64, _i32, C.POINTER(_u64)]),
     "svo_add_voxels": (_i32, [_vp, _vp, _vp, _u64]),
     "svo_build": (_i32, [_vp, _u64, _i32, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_i32)]),
@@ -328,6 +329,11 @@ class SvoContext:
         _check(load_library().svo_partition(self._h, float(np.float32(unitlength)), gridsize,
                                             n_partitions, _ptr(counts)))
         return counts
+
+    def write_trip(self, base: str, gridsize: int, bbox=(0, 0, 0, 1, 1, 1), version: int = 1):
+        """Write <base>.trip and <base>_<i>.tripdata after partition()."""
+        b = np.asarray(bbox, np.float32)
+        _check(load_library().svo_write_trip(self._h, base.encode(), gridsize, _ptr(b), version))
 
     def partition_indices(self, part: int, count: int) -> np.ndarray:
         out = np.empty(count, np.uint32)

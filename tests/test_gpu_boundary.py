@@ -9,13 +9,14 @@
   header) and in-order triangle lists (the .tripdata order), with triangles on the
   partition planes, NaN coordinates and triangles outside the grid.
 """
+import os
 import subprocess
 
 import numpy as np
 import pytest
 
 import oracle
-from ooc_svo_builder_amd import CLI_BINARY, CLI_PAYLOAD, generators
+from ooc_svo_builder_amd import CLI_BINARY, CLI_PAYLOAD, generators, write_tri
 from ooc_svo_builder_amd.formats import files_identical
 
 pytestmark = pytest.mark.gpu
@@ -132,3 +133,59 @@ def test_partition_lists_match_oracle(gpu_ctx, G, P, payload):
         seen[got] += 1
     # boundary triangles land in several partitions, triangles outside the grid in none
     assert seen.max() >= 2 and (seen == 0).any()
+
+
+def _trip_text(G, n, bbox, payload, counts):
+    """trip_tools.h:104-125 (floats as C++'s default 6-significant-digit output)."""
+    from ooc_svo_builder_amd.formats import _fmt_float
+    t = "#trip 1\ngridsize %d\nn_triangles %d\n" % (G, n)
+    t += "bbox  " + " ".join(_fmt_float(b) for b in bbox) + "\n"
+    t += "geo_only %d\nn_partitions %d\n" % (0 if payload else 1, len(counts))
+    t += "".join("%d %d\n" % (i, c) for i, c in enumerate(counts))
+    return t + "END\n"
+
+
+@pytest.mark.parametrize("G,P,payload", [(256, 8, False), (128, 64, True), (64, 1, False)])
+def test_write_trip_files(gpu_ctx, tmp_path, G, P, payload):
+    """svo_write_trip writes the reference's partition files (partitioner.cpp:80-149): the
+    header, and per non-empty partition its triangles' records in mesh order."""
+    _say("write trip G=%d P=%d payload=%d" % (G, P, payload))
+    tris = _partition_soup(G, max(P, 8))
+    if payload:
+        tris = np.concatenate([tris, np.random.default_rng(3).uniform(
+            0, 1, (len(tris), 12)).astype(np.float32)], axis=1)
+    gpu_ctx.load_mesh(tris, payload)
+    gpu_ctx.partition(np.float32(1.0) / np.float32(G), G, P)
+    base = str(tmp_path / ("m%d_%d" % (G, P)))
+    bbox = (0.0, 0.0, 0.0, 1.0, 1.0, 1.0)
+    gpu_ctx.write_trip(base, G, bbox)
+    if P == 1:
+        counts, lists = np.array([len(tris)]), [np.arange(len(tris))]
+    else:
+        counts, lists = oracle.partition(tris, payload, bbox, G, P)
+    assert open(base + ".trip").read() == _trip_text(G, len(tris), bbox, payload, counts)
+    for p in range(P):
+        fn = "%s_%d.tripdata" % (base, p)
+        if counts[p] == 0:
+            assert not os.path.exists(fn)
+            continue
+        want = np.ascontiguousarray(tris[lists[p].astype(np.int64)], "<f4").tobytes()
+        assert open(fn, "rb").read() == want, p
+
+
+def test_cli_write_trip(tmp_path):
+    """--write-trip keeps the partition files next to the octree files."""
+    _say("cli --write-trip")
+    tris = generators.terrain(n=49, seed=24, payload=False)
+    base = str(tmp_path / "t")
+    tri = write_tri(base, tris, False)
+    r = subprocess.run([CLI_BINARY, "-f", tri, "-s", "256", "-l", "2", "--write-trip"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    counts, lists = oracle.partition(tris, False, (0, 0, 0, 1, 1, 1), 256, 8)
+    assert open(base + "256_8.trip").read() == _trip_text(256, len(tris), (0, 0, 0, 1, 1, 1),
+                                                           False, counts)
+    for p in range(8):
+        if counts[p]:
+            want = np.ascontiguousarray(tris[lists[p].astype(np.int64)], "<f4").tobytes()
+            assert open("%s256_8_%d.tripdata" % (base, p), "rb").read() == want
