@@ -536,13 +536,22 @@ class MixBench:
         _log("autotuned small_cols/tile_blocks/large_blocks = %s in %.1f s" % (
             knobs, time.perf_counter() - t))
         reps = 3
-        times = []
-        for _ in range(reps):
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            res, info = b.batch_run(self.jobs)
-            times.append(time.perf_counter() - t)
-        best = min(times)
+
+        def timed():
+            times = []
+            for _ in range(reps):
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                r, i = b.batch_run(self.jobs)
+                times.append(time.perf_counter() - t)
+            return min(times), r, i
+        # the same batch without graphs (every kernel launched by the host), then with
+        # them (the default: kernel sequences replayed from the graphs the warm-up kept)
+        b.set_graphs(False)
+        best_eager, _, info_eager = timed()
+        b.set_graphs(True)
+        b.batch_run(self.jobs)  # captures the graphs
+        best, res, info = timed()
         same = 0
         dev = torch.device("cuda:0")
         for r, w in zip(res, self.want):
@@ -565,6 +574,12 @@ class MixBench:
                 "speedup_vs_plain": self.plain_s / best,
                 "launches_per_batch": int(info.launches),
                 "host_syncs_per_batch": int(info.host_syncs),
+                "graph_launches_per_batch": int(info.graph_launches),
+                "graph_replays_per_batch": int(info.graph_replays),
+                "kernels_in_graphs_per_batch": int(info.graph_kernels),
+                "eager": {"seconds": best_eager, "jobs_per_s": n / best_eager,
+                          "launches_per_batch": int(info_eager.launches),
+                          "host_syncs_per_batch": int(info_eager.host_syncs)},
                 "identical_jobs": same, "identical": same == n,
                 "autotuned": {"small_cols": knobs[0], "direct_cols": knobs[1],
                               "tile_blocks": knobs[2], "large_blocks": knobs[3]},
@@ -586,28 +601,27 @@ class MixBench:
         rng = np.random.default_rng(5)
         rates = (50, 200, 1000, 2000, 4000, 6000)
         rows = []
-        # the collector thread must get the interpreter promptly while the arrival loop spins
-        sys.setswitchinterval(0.0002)
         srv = SvoServer(0, budget_mb=200 * 1024)
         # warm the slots (allocations) with one pass over a few jobs
         for tk in [srv.submit(j) for j in self.jobs[:16] + [large]]:
             srv.wait(tk)
             srv.release(tk)
 
-        def wait_until(t0, at):  # sleep through long gaps, spin through short ones
-            while True:
-                left = at - (time.perf_counter() - t0)
-                if left <= 0:
-                    return
-                if left > 0.002:
-                    time.sleep(left - 0.001)
+        def wait_until(t0, at):
+            # sleep, never spin: a spinning loop holds the interpreter lock that the
+            # collector needs to release finished jobs (whose held results count against
+            # admission). A late wake-up submits the overdue arrivals at once, so the
+            # arrival times stay those of the schedule.
+            left = at - (time.perf_counter() - t0)
+            if left > 0:
+                time.sleep(left)
 
         for lam in rates:
             n = max(1, int(lam * seconds))
             arrive = np.cumsum(rng.exponential(1.0 / lam, size=n))
             pick = [(-1 if rng.uniform() < 0.02 else int(rng.integers(0, len(self.jobs))))
                     for _ in range(n)]
-            lat, done_t, tickets, failed = [], [], [], []
+            lat, done_t, tickets, failed, late = [], [], [], [], []
             stride = max(1, n // 48)
             checked = [0, 0]  # sampled results compared, identical
             cv = threading.Condition()
@@ -620,7 +634,7 @@ class MixBench:
                         tk = tickets[k]
                     try:
                         r, ms = srv.wait(tk)
-                        lat.append(ms)
+                        lat.append(ms + late[k])  # from the scheduled arrival
                         w = large_want if pick[k] < 0 else self.want[pick[k]]
                         # a bounded sample (the copies and compares slow the releases,
                         # and with them admission, at high rates)
@@ -643,8 +657,10 @@ class MixBench:
             t0 = time.perf_counter()
             for i in range(n):
                 wait_until(t0, arrive[i])
+                lt = (time.perf_counter() - t0 - arrive[i]) * 1e3
                 tk = srv.submit(large if pick[i] < 0 else self.jobs[pick[i]])
                 with cv:
+                    late.append(lt)
                     tickets.append(tk)
                     cv.notify()
             th.join()

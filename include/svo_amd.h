@@ -63,6 +63,14 @@ void* svo_ctx_stream(svo_ctx* ctx);
  * the free HBM, at most 64 GiB). Neither changes the output. */
 int svo_ctx_set_tuning(svo_ctx* ctx, int small_cols, uint64_t dense_limit_mb);
 
+/* hipGraph use of svo_batch_run on this context (no reference counterpart; on by
+ * default): each kernel sequence between two host synchronizations of a batch runs as a
+ * graph, captured once and launched again by later batches that make the same launches
+ * (same jobs, buffers and knobs). A context that captures must not share its stream, and
+ * no other thread should synchronize the device or free device memory while it does
+ * (captures are relaxed-mode); turn graphs off for such uses. Output is the same. */
+int svo_ctx_set_graphs(svo_ctx* ctx, int on);
+
 /* partitioner.cpp:12-28: number of Morton-range partitions (a power of 8) such that the
  * gridsize^3 byte grid fits memory_limit_mb. Pure integer rule, no device work. */
 uint64_t svo_estimate_partitions(uint64_t gridsize, uint64_t memory_limit_mb);
@@ -155,8 +163,11 @@ typedef struct svo_run_info {
   uint64_t n_data;
   int hazard;
   float ms_partition, ms_voxelize, ms_build; /* device time per stage */
-  uint32_t launches;   /* kernel launches the call issued */
-  uint32_t host_syncs; /* host waits on the device the call made */
+  uint32_t launches;       /* kernel launches the call issued outside graphs */
+  uint32_t host_syncs;     /* host waits on the device the call made */
+  uint32_t graph_launches; /* hipGraph launches (svo_batch_run with graphs on) */
+  uint32_t graph_replays;  /* ... of them, graphs kept from an earlier call */
+  uint32_t graph_kernels;  /* kernels run inside those graphs */
 } svo_run_info;
 int svo_run(svo_ctx* ctx, const float* records, uint64_t n_triangles, int payload,
             int src_on_device, const float bbox[6], uint64_t gridsize,
@@ -233,7 +244,8 @@ typedef struct svo_job_result {
   uint64_t data_offset;
   int hazard;
 } svo_job_result;
-/* info (optional) receives batch totals and the device time of each stage. */
+/* info (optional) receives batch totals and the device time of each stage, and the
+ * launch counts (see svo_ctx_set_graphs for the graphs). */
 int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_result* results,
                   svo_run_info* info);
 

@@ -151,6 +151,7 @@ struct HeadLoad {
   const JobDev* jobs;
   const uint64_t* leaves;
   int h;
+  int pad = 0;  // no padding bytes: launch logs compare arguments byte by byte
   __device__ uint32_t operator()(uint64_t i) const {
     // an overflowed child level is not read (its tail lies beyond the arena)
     if (h > 0 && M.off[h] + M.n[h] > A.cap) return 0u;
@@ -174,6 +175,7 @@ struct ParentStore {
   unsigned long long* root_isize; // per job: internal nodes in its tree (-levels)
   const float* leaf_rec;          // -levels, payload: voxel i's record at leaf_rec + 8 i
   int h;
+  int pad = 0;  // no padding bytes: launch logs compare arguments byte by byte
   __device__ void operator()(uint64_t i, uint32_t ex, uint32_t head, bool) const {
     const bool leaf = (h == 0);
     if (leaf)
@@ -1094,97 +1096,103 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     unsigned long long* root_size = meta + 136;
     unsigned long long* root_isize = root_size + n_jobs;
     unsigned long long* brick_hazard = root_isize + n_jobs;
-    // n[0] = V, off[1] = 0, flags = 0, root sizes = 0, hazard flags = 0, scan status = 0
-    // (fast: n[4] = NB, off[4] = 0, off[5] = NB)
-    SVO_LAUNCH(k_build_init, dim3(1), dim3(256), 0, c.stream, meta,
-                       meta_words + st_words, V, NB, fast ? 1 : 0);
-    SVO_TRY(hipGetLastError());
-
-    if (up_once) {
-      const uint32_t ntiles = (uint32_t)((NB + kUpTile - 1) / kUpTile);
-      const uint64_t b_cnt = ((uint64_t)ntiles * kUpLevels * 4 + 255) & ~255ull;
-      const uint64_t b_s4 = ((uint64_t)ntiles * 8 + 255) & ~255ull;
-      const uint64_t b_end = ((uint64_t)kUpLevels * 8 + 255) & ~255ull;
+    // the upper levels' tile scratch (fast path)
+    const uint32_t ntiles = (uint32_t)((NB + kUpTile - 1) / kUpTile);
+    const uint64_t b_cnt = ((uint64_t)ntiles * kUpLevels * 4 + 255) & ~255ull;
+    const uint64_t b_s4 = ((uint64_t)ntiles * 8 + 255) & ~255ull;
+    const uint64_t b_end = ((uint64_t)kUpLevels * 8 + 255) & ~255ull;
+    if (up_once)
       SVO_TRY(ensure(c.up_tiles, b_cnt + b_s4 + b_end + (uint64_t)n_jobs * kUpLevels * 8));
-      char* ub = (char*)c.up_tiles.p;
-      uint32_t* tile_cnt = (uint32_t*)ub;
-      uint64_t* tile_s4 = (uint64_t*)(ub + b_cnt);
-      uint64_t* a_end = (uint64_t*)(ub + b_cnt + b_s4);
-      uint64_t* job_end = (uint64_t*)(ub + b_cnt + b_s4 + b_end);
-      ktimer_begin(c, "upper_levels");
-      SVO_LAUNCH(k_up_count, dim3(ntiles), dim3(kUpTile), 0, c.stream, A, T,
-                         (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),
-                         c.bcode.as<uint64_t>(), c.bshape.as<uint32_t>(), NB, H, tile_cnt,
-                         tile_s4, brick_hazard);
-      SVO_LAUNCH(k_up_scan, dim3(1), dim3(1024), 0, c.stream, A, M, NB, H, ntiles,
-                         tile_cnt, tile_s4, a_end);
-      SVO_LAUNCH(k_up_apply, dim3(ntiles), dim3(kUpTile), 0, c.stream, A, M, T,
-                         (const JobDev*)d_jobs, NB, H, (const uint32_t*)tile_cnt,
-                         (const uint64_t*)tile_s4, (const uint64_t*)a_end, job_end);
-      uint64_t nup = 0;
-      for (int h = 4; h <= H; h++) nup += bound[h];
-      SVO_LAUNCH(k_up_finish, dim3((unsigned)((nup + 255) / 256)), dim3(256), 0,
-                         c.stream, A, M, T, (const JobDev*)d_jobs, H,
-                         (const uint64_t*)job_end, root_size);
-      ktimer_end(c);
+    // the build's kernels up to its read-back: one graph segment
+    SVO_TRY(run_segment(c, [&]() -> hipError_t {
+      // n[0] = V, off[1] = 0, flags = 0, root sizes = 0, hazard flags = 0, scan status = 0
+      // (fast: n[4] = NB, off[4] = 0, off[5] = NB)
+      SVO_LAUNCH(k_build_init, dim3(1), dim3(256), 0, c.stream, meta,
+                         meta_words + st_words, V, NB, fast ? 1 : 0);
       SVO_TRY(hipGetLastError());
-    }
 
-    // ---- bottom-up: heights 1 .. H
-    unsigned long long* status = meta + meta_words;
-    for (int h = h0; h < h_scan_end; h++) {
-      HeadLoad ld{A, M, T, d_jobs, c.codes.as<uint64_t>(), h};
-      ParentStore st{A, M, T, d_jobs, c.codes.as<uint64_t>(), c.lvl_leafp.as<uint32_t>(),
-                     root_size, root_isize, c.records.as<float>() + 8, h};
-      SVO_TRY((scan_single<uint32_t>(ld, st, bound[h], status,
-                                     c.counters.as<unsigned long long>() + kCtrError, c.stream,
-                                     M.n + h)));
-      status += scan_single_words(bound[h]);
-    }
-    SVO_LAUNCH(k_levels_tail, dim3(1), dim3(kTailBlock), 0, c.stream, A, M, T, d_jobs,
-                       n_jobs, c.codes.as<uint64_t>(), c.lvl_leafp.as<uint32_t>(), root_size,
-                       root_isize, c.records.as<float>() + 8, h_tail, H,
-                       fast ? brick_hazard : nullptr, node_cap,
-                       c.nodes.as<uint64_t>(), c.records.as<float>(), out_rec);
-    SVO_TRY(hipGetLastError());
-    // ---- top-down: heights H .. 1 (one launch without -levels), then the leaves
-    const int hs = std::max(h0, 1);
-    if (!levels && H >= hs) {
-      uint64_t nup = 0;
-      for (int h = hs; h <= H; h++) nup += bound[h];
-      ktimer_begin(c, "emit_children");
-      SVO_LAUNCH(k_emit_upper, dim3((unsigned)std::max<uint64_t>((nup + 255) / 256, 1)),
-                         dim3(256), 0, c.stream, A, M, T, (const JobDev*)d_jobs, hs, H, node_cap,
-                         c.nodes.as<uint64_t>());
-      ktimer_end(c);
+      if (up_once) {
+        char* ub = (char*)c.up_tiles.p;
+        uint32_t* tile_cnt = (uint32_t*)ub;
+        uint64_t* tile_s4 = (uint64_t*)(ub + b_cnt);
+        uint64_t* a_end = (uint64_t*)(ub + b_cnt + b_s4);
+        uint64_t* job_end = (uint64_t*)(ub + b_cnt + b_s4 + b_end);
+        ktimer_begin(c, "upper_levels");
+        SVO_LAUNCH(k_up_count, dim3(ntiles), dim3(kUpTile), 0, c.stream, A, T,
+                           (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),
+                           c.bcode.as<uint64_t>(), c.bshape.as<uint32_t>(), NB, H, tile_cnt,
+                           tile_s4, brick_hazard);
+        SVO_LAUNCH(k_up_scan, dim3(1), dim3(1024), 0, c.stream, A, M, NB, H, ntiles,
+                           tile_cnt, tile_s4, a_end);
+        SVO_LAUNCH(k_up_apply, dim3(ntiles), dim3(kUpTile), 0, c.stream, A, M, T,
+                           (const JobDev*)d_jobs, NB, H, (const uint32_t*)tile_cnt,
+                           (const uint64_t*)tile_s4, (const uint64_t*)a_end, job_end);
+        uint64_t nup = 0;
+        for (int h = 4; h <= H; h++) nup += bound[h];
+        SVO_LAUNCH(k_up_finish, dim3((unsigned)((nup + 255) / 256)), dim3(256), 0,
+                           c.stream, A, M, T, (const JobDev*)d_jobs, H,
+                           (const uint64_t*)job_end, root_size);
+        ktimer_end(c);
+        SVO_TRY(hipGetLastError());
+      }
+
+      // ---- bottom-up: heights 1 .. H
+      unsigned long long* status = meta + meta_words;
+      for (int h = h0; h < h_scan_end; h++) {
+        HeadLoad ld{A, M, T, d_jobs, c.codes.as<uint64_t>(), h};
+        ParentStore st{A, M, T, d_jobs, c.codes.as<uint64_t>(), c.lvl_leafp.as<uint32_t>(),
+                       root_size, root_isize, c.records.as<float>() + 8, h};
+        SVO_TRY((scan_single<uint32_t>(ld, st, bound[h], status,
+                                       c.counters.as<unsigned long long>() + kCtrError, c.stream,
+                                       M.n + h)));
+        status += scan_single_words(bound[h]);
+      }
+      SVO_LAUNCH(k_levels_tail, dim3(1), dim3(kTailBlock), 0, c.stream, A, M, T, d_jobs,
+                         n_jobs, c.codes.as<uint64_t>(), c.lvl_leafp.as<uint32_t>(), root_size,
+                         root_isize, c.records.as<float>() + 8, h_tail, H,
+                         fast ? brick_hazard : nullptr, node_cap,
+                         c.nodes.as<uint64_t>(), c.records.as<float>(), out_rec);
       SVO_TRY(hipGetLastError());
-    }
-    for (int h = H; levels && h >= hs; h--) {
-      ktimer_begin(c, "emit_children");
-      SVO_LAUNCH(k_emit_level, dim3((unsigned)std::max<uint64_t>((bound[h] + 255) / 256, 1)),
-                         dim3(256), 0, c.stream, A, M, T, (const JobDev*)d_jobs, h, node_cap,
-                         c.nodes.as<uint64_t>(), out_rec);
-      ktimer_end(c);
-      SVO_TRY(hipGetLastError());
-    }
-    if (fast) {
-      ktimer_begin(c, "emit_bricks");
-      SVO_LAUNCH(k_emit_bricks, dim3((unsigned)((NB + kEmitWaves - 1) / kEmitWaves)),
-                         dim3(kWave * kEmitWaves), 0,
-                         c.stream, A, M, T, (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),
-                         c.bcode.as<uint64_t>(), c.bvoff.as<uint64_t>(), NB, node_cap,
-                         c.nodes.as<uint64_t>());
-      ktimer_end(c);
-      SVO_TRY(hipGetLastError());
-    } else if (V) {
-      ktimer_begin(c, "emit_leaves");
-      SVO_LAUNCH(k_emit_leaves, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, c.stream,
-                         A, M, T, (const JobDev*)d_jobs, node_cap, c.codes.as<uint64_t>(),
-                         c.lvl_leafp.as<uint32_t>(), c.nodes.as<uint64_t>(),
-                         c.records.as<float>(), out_rec);
-      ktimer_end(c);
-      SVO_TRY(hipGetLastError());
-    }
+      // ---- top-down: heights H .. 1 (one launch without -levels), then the leaves
+      const int hs = std::max(h0, 1);
+      if (!levels && H >= hs) {
+        uint64_t nup = 0;
+        for (int h = hs; h <= H; h++) nup += bound[h];
+        ktimer_begin(c, "emit_children");
+        SVO_LAUNCH(k_emit_upper, dim3((unsigned)std::max<uint64_t>((nup + 255) / 256, 1)),
+                           dim3(256), 0, c.stream, A, M, T, (const JobDev*)d_jobs, hs, H, node_cap,
+                           c.nodes.as<uint64_t>());
+        ktimer_end(c);
+        SVO_TRY(hipGetLastError());
+      }
+      for (int h = H; levels && h >= hs; h--) {
+        ktimer_begin(c, "emit_children");
+        SVO_LAUNCH(k_emit_level, dim3((unsigned)std::max<uint64_t>((bound[h] + 255) / 256, 1)),
+                           dim3(256), 0, c.stream, A, M, T, (const JobDev*)d_jobs, h, node_cap,
+                           c.nodes.as<uint64_t>(), out_rec);
+        ktimer_end(c);
+        SVO_TRY(hipGetLastError());
+      }
+      if (fast) {
+        ktimer_begin(c, "emit_bricks");
+        SVO_LAUNCH(k_emit_bricks, dim3((unsigned)((NB + kEmitWaves - 1) / kEmitWaves)),
+                           dim3(kWave * kEmitWaves), 0,
+                           c.stream, A, M, T, (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),
+                           c.bcode.as<uint64_t>(), c.bvoff.as<uint64_t>(), NB, node_cap,
+                           c.nodes.as<uint64_t>());
+        ktimer_end(c);
+        SVO_TRY(hipGetLastError());
+      } else if (V) {
+        ktimer_begin(c, "emit_leaves");
+        SVO_LAUNCH(k_emit_leaves, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, c.stream,
+                           A, M, T, (const JobDev*)d_jobs, node_cap, c.codes.as<uint64_t>(),
+                           c.lvl_leafp.as<uint32_t>(), c.nodes.as<uint64_t>(),
+                           c.records.as<float>(), out_rec);
+        ktimer_end(c);
+        SVO_TRY(hipGetLastError());
+      }
+      return hipSuccess;
+    }));
     unsigned long long flags[3] = {0, 0, 0};
     if (c.build_done_ev) SVO_TRY(hipEventRecord(c.build_done_ev, c.stream));
     {

@@ -281,6 +281,10 @@ void svo_ctx_destroy(svo_ctx* ctx) {
   Context& c = ctx->c;
   (void)hipSetDevice(c.device);
   if (c.stream) (void)stream_sync(c.stream);
+  for (hipGraphExec_t x : c.gexec)
+    if (x) (void)hipGraphExecDestroy(x);
+  c.gexec.clear();
+  c.glog.clear();
   DevBuf* bufs[] = {&c.tris, &c.jobdev, &c.batch_tris, &c.hitbox, &c.part_lists, &c.unitdev,
                     &c.table, &c.pool, &c.pool8, &c.grid8, &c.counters, &c.large_unit,
                     &c.large_tri, &c.large_cols, &c.large_off, &c.occ_list, &c.bcount,
@@ -330,6 +334,12 @@ int svo_ctx_set_tuning(svo_ctx* ctx, int small_cols, uint64_t dense_limit_mb) {
   CHECK_CTX();
   if (small_cols > 0) ctx->c.tune.small_cols = small_cols;
   if (dense_limit_mb != UINT64_MAX) ctx->c.dense_limit = dense_limit_mb << 20;
+  return SVO_OK;
+}
+
+int svo_ctx_set_graphs(svo_ctx* ctx, int on) {
+  CHECK_CTX();
+  ctx->c.graph_policy = on ? 1 : 0;
   return SVO_OK;
 }
 
@@ -886,6 +896,7 @@ static int run_impl(svo_ctx* ctx, const float* records, uint64_t n_triangles, in
     info->ms_build = ms_b;
     info->launches = (uint32_t)(t_launches - l0);
     info->host_syncs = (uint32_t)(t_syncs - s0);
+    info->graph_launches = info->graph_replays = info->graph_kernels = 0;
   }
   return SVO_OK;
 }
@@ -1011,6 +1022,8 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
   // meshes in host memory are uploaded into one buffer; device meshes are used in place
   reset_plain(c);
   c.batch_state = true;
+  const uint64_t gl0 = t_graph_launches, gr0 = t_graph_replays, gk0 = t_graph_kernels;
+  GraphScope graph_scope(c, c.graph_policy != 0);
   if (host_bytes) TRYC(ensure(c.batch_tris, host_bytes), "svo_batch_run");
   c.jobs.assign(n_jobs, Job{});
   uint64_t hoff = 0;
@@ -1139,6 +1152,9 @@ int svo_batch_run(svo_ctx* ctx, const svo_job* jobs, uint32_t n_jobs, svo_job_re
     info->ms_build = ms_b;
     info->launches = (uint32_t)(t_launches - l0);
     info->host_syncs = (uint32_t)(t_syncs - s0);
+    info->graph_launches = (uint32_t)(t_graph_launches - gl0);
+    info->graph_replays = (uint32_t)(t_graph_replays - gr0);
+    info->graph_kernels = (uint32_t)(t_graph_kernels - gk0);
   }
   return SVO_OK;
 }

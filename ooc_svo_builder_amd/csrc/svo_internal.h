@@ -4,17 +4,26 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstring>
 #include <map>
 #include <string>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 namespace svo {
 
 // Kernel launches and host synchronizations issued by this thread (a context is used by
-// one thread at a time, so the difference across a call is that call's count).
+// one thread at a time, so the difference across a call is that call's count). Launches
+// recorded into a graph count as graph nodes; t_graph_launches counts graph launches,
+// t_graph_replays those of a graph kept from an earlier call.
 inline thread_local uint64_t t_launches = 0;
 inline thread_local uint64_t t_syncs = 0;
-inline void count_launch() { t_launches++; }
+inline thread_local uint64_t t_graph_nodes = 0;
+inline thread_local uint64_t t_graph_launches = 0;
+inline thread_local uint64_t t_graph_replays = 0;
+inline thread_local uint64_t t_graph_kernels = 0;  // kernels inside the launched graphs
+inline thread_local uint64_t t_dry_launches = 0;   // launches recorded by dry passes
 inline hipError_t stream_sync(hipStream_t s) {
   t_syncs++;
   return hipStreamSynchronize(s);
@@ -23,12 +32,57 @@ inline hipError_t event_sync(hipEvent_t e) {
   t_syncs++;
   return hipEventSynchronize(e);
 }
-// every kernel launch of the library goes through this
-#define SVO_LAUNCH(...)       \
-  do {                        \
-    ::svo::count_launch();    \
-    hipLaunchKernelGGL(__VA_ARGS__); \
-  } while (0)
+
+// Launch modes of this thread: kLaunchRun launches, kLaunchCapture launches into a stream
+// capture (graph nodes), kLaunchDry only records. In the last two every launch appends its
+// kernel, grid, block, shared memory, stream and argument bytes (as passed to the kernel)
+// to t_launch_log: two segments of launches with equal logs make identical graphs.
+enum LaunchMode : int { kLaunchRun = 0, kLaunchCapture = 1, kLaunchDry = 2 };
+inline thread_local int t_launch_mode = kLaunchRun;
+inline thread_local std::vector<uint8_t>* t_launch_log = nullptr;
+
+template <typename T>
+inline void log_bytes(std::vector<uint8_t>& L, const T& v) {
+  const uint8_t* p = reinterpret_cast<const uint8_t*>(&v);
+  L.insert(L.end(), p, p + sizeof(T));
+}
+
+template <typename... KArgs, typename... Args, size_t... I>
+inline hipError_t launch_impl(void (*k)(KArgs...), dim3 g, dim3 b, uint32_t sh, hipStream_t s,
+                              std::index_sequence<I...>, Args&&... args) {
+  // the arguments converted to the kernel's parameter types, as the kernel receives them
+  std::tuple<KArgs...> t{static_cast<KArgs>(std::forward<Args>(args))...};
+  if (t_launch_log) {
+    std::vector<uint8_t>& L = *t_launch_log;
+    log_bytes(L, reinterpret_cast<const void*>(k));
+    log_bytes(L, g);
+    log_bytes(L, b);
+    log_bytes(L, sh);
+    log_bytes(L, s);
+    (log_bytes(L, std::get<I>(t)), ...);
+  }
+  if (t_launch_mode == kLaunchDry) {
+    t_dry_launches++;
+    return hipSuccess;
+  }
+  if (t_launch_mode == kLaunchCapture)
+    t_graph_nodes++;
+  else
+    t_launches++;
+  void* ptrs[sizeof...(KArgs) + 1] = {reinterpret_cast<void*>(&std::get<I>(t))..., nullptr};
+  return hipLaunchKernel(reinterpret_cast<const void*>(k), g, b, ptrs, sh, s);
+}
+
+template <typename... KArgs, typename... Args>
+inline hipError_t launch_kernel(void (*k)(KArgs...), dim3 g, dim3 b, uint32_t sh, hipStream_t s,
+                                Args&&... args) {
+  static_assert(sizeof...(KArgs) == sizeof...(Args), "kernel argument count");
+  return launch_impl(k, g, b, sh, s, std::index_sequence_for<KArgs...>{},
+                     std::forward<Args>(args)...);
+}
+
+// every kernel launch of the library goes through this (errors reach hipGetLastError)
+#define SVO_LAUNCH(...) (void)::svo::launch_kernel(__VA_ARGS__)
 
 // Sparse occupancy grid: the partition's Morton range is cut into aligned bricks of
 // 2^kBrickBits voxels (16^3). A dense table maps brick -> pool slot (0 = empty). A slot
@@ -274,7 +328,85 @@ struct Context {
   std::vector<PendingTimer> kpending;
   std::vector<hipEvent_t> evpool;
   std::map<std::string, KernelStat> kstats;
+
+  // graph segments (run_segment): while 'graphs' is set (a batch with graph_policy on),
+  // each kernel sequence between two host synchronizations is captured into a graph; the
+  // graph is kept, with the launch log it was captured from, under its segment index
+  // (gseg counts the segments of the current call), and launched again whenever a later
+  // call makes the same launches at that index
+  bool graphs = false;
+  int graph_policy = 1;
+  uint32_t gseg = 0;
+  std::vector<std::vector<uint8_t>> glog;
+  std::vector<hipGraphExec_t> gexec;
 };
+
+// Graph segments on for the lifetime of a scope (one batch call), numbered from 0.
+struct GraphScope {
+  Context& c;
+  GraphScope(Context& ctx, bool on) : c(ctx) {
+    c.graphs = on;
+    c.gseg = 0;
+  }
+  ~GraphScope() { c.graphs = false; }
+};
+
+// Run body (a sequence of SVO_LAUNCHes and nothing else that reaches the device; host
+// effects only where repeating them is harmless) as one graph segment of c: a dry pass
+// records its launch log; a kept graph with an equal log is launched in its place,
+// otherwise body runs again under stream capture and its graph replaces the kept one.
+// Without c.graphs (or while kernels are timed, or on the null stream) body just runs.
+template <typename F>
+hipError_t run_segment(Context& c, F&& body) {
+  if (!c.graphs || c.ktiming || !c.stream || t_launch_mode != kLaunchRun) return body();
+  const uint32_t idx = c.gseg++;
+  if (c.glog.size() <= idx) {
+    c.glog.resize(idx + 1);
+    c.gexec.resize(idx + 1, nullptr);
+  }
+  std::vector<uint8_t> log;
+  log.reserve(c.glog[idx].size());
+  const uint64_t d0 = t_dry_launches;
+  t_launch_mode = kLaunchDry;
+  t_launch_log = &log;
+  hipError_t e = body();
+  t_launch_mode = kLaunchRun;
+  t_launch_log = nullptr;
+  if (e != hipSuccess) return e;
+  const uint64_t n_kernels = t_dry_launches - d0;
+  if (n_kernels == 0) return hipSuccess;  // nothing to launch
+  if (c.gexec[idx] && log == c.glog[idx]) {
+    t_graph_launches++;
+    t_graph_replays++;
+    t_graph_kernels += n_kernels;
+    return hipGraphLaunch(c.gexec[idx], c.stream);
+  }
+  log.clear();
+  e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) return e;
+  t_launch_mode = kLaunchCapture;
+  t_launch_log = &log;
+  const hipError_t eb = body();
+  t_launch_mode = kLaunchRun;
+  t_launch_log = nullptr;
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(c.stream, &g);
+  if (eb != hipSuccess || ee != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return eb != hipSuccess ? eb : ee;
+  }
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return e;
+  // the kept graph was last launched by an earlier call, which waited for its stream
+  if (c.gexec[idx]) (void)hipGraphExecDestroy(c.gexec[idx]);
+  c.gexec[idx] = x;
+  c.glog[idx].swap(log);
+  t_graph_launches++;
+  t_graph_kernels += n_kernels;
+  return hipGraphLaunch(x, c.stream);
+}
 
 // Kernel timing helpers (no-ops unless c.ktiming): begin/end bracket one launch on
 // c.stream; flush folds finished pairs into c.kstats after a stream synchronization.

@@ -58,6 +58,11 @@ uint64_t estimate_job_bytes(const svo_job& q) {
   return (uint64_t)b + (1ull << 20);
 }
 
+// Dense voxel grid per slot, in MiB. A batch's units pass through it in groups whose
+// spans fit, each group a few kernel launches and one host synchronization, so a larger
+// grid means fewer groups for batches of large grids.
+constexpr uint64_t kSlotDenseMb = 32768;
+
 size_t size_class(size_t bytes) {
   size_t c = 64 * 1024;
   while (c < bytes) c <<= 1;
@@ -289,15 +294,19 @@ int svo_server_create(int device, uint64_t admission_budget_mb, uint32_t max_bat
   }
   s->device = device;
   s->budget = (admission_budget_mb ? admission_budget_mb : 200ull * 1024) << 20;
-  {
-    // never admit more than three quarters of what the device has free now (the slots'
-    // own scratch and the caller's allocations live beside the admitted jobs)
-    size_t free_b = 0, total_b = 0;
-    if (hipSetDevice(device) == hipSuccess && hipMemGetInfo(&free_b, &total_b) == hipSuccess)
-      s->budget = std::min<uint64_t>(s->budget, (uint64_t)free_b / 4 * 3);
-  }
   s->max_batch = max_batch_jobs ? std::min<uint32_t>(max_batch_jobs, svo::kMaxJobs) : 512;
   const uint32_t n = slots ? std::min<uint32_t>(slots, 8) : 2;
+  {
+    // never admit more than three quarters of what the device has free now once the
+    // slots' dense grids are set aside (their other scratch and the caller's allocations
+    // live beside the admitted jobs)
+    size_t free_b = 0, total_b = 0;
+    if (hipSetDevice(device) == hipSuccess && hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+      const uint64_t grids = (uint64_t)n * (kSlotDenseMb << 20);
+      const uint64_t avail = (uint64_t)free_b > grids ? (uint64_t)free_b - grids : 0;
+      s->budget = std::min<uint64_t>(s->budget, avail / 4 * 3);
+    }
+  }
   for (uint32_t i = 0; i < n; i++) {
     svo_ctx* c = nullptr;
     const int rc = svo_ctx_create(device, &c);
@@ -305,9 +314,12 @@ int svo_server_create(int device, uint64_t admission_budget_mb, uint32_t max_bat
       svo_server_destroy(s);
       return rc;
     }
-    // each slot caches dense voxel grids of at most 8 GiB (larger batches use the
-    // touched-brick pool); that scratch is outside the admission budget
-    (void)svo_ctx_set_tuning(c, 0, 8192);
+    // each slot caches a dense voxel grid of at most kSlotDenseMb (a batch's units pass
+    // through it in groups that fit; a larger unit uses the touched-brick pool)
+    (void)svo_ctx_set_tuning(c, 0, kSlotDenseMb);
+    // the slots run their batches from concurrent threads, which graph capture does not
+    // allow for (and a served batch seldom repeats an earlier one)
+    (void)svo_ctx_set_graphs(c, 0);
     s->ctx.push_back(c);
   }
   if (hipSetDevice(device) != hipSuccess ||

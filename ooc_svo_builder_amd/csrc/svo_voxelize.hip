@@ -327,9 +327,13 @@ __device__ __forceinline__ void row_interval(const Walk& K, const ColRow& R, flo
 // the voxelizer call this with the same inputs, so they see the same ranges.
 __device__ __forceinline__ bool column_range(const Walk& K, float pa, const ColRow& R, int a,
                                              int& lo, int& hi) {
+  // the three edge tests evaluated together, one branch (their outcome, not their order,
+  // decides)
+  bool in = true;
 #pragma unroll
   for (int k = 0; k < 3; k++)
-    if ((K.f[F_H0 + k] * pa + R.hb[k]) + K.f[F_HD + k] < 0.0f) return false;
+    in &= !((K.f[F_H0 + k] * pa + R.hb[k]) + K.f[F_HD + k] < 0.0f);
+  if (!in) return false;
   lo = K.i[I_G0W];
   hi = K.i[I_G1W];
   if (K.slab()) {
@@ -400,12 +404,12 @@ __device__ __forceinline__ void walk_column(const Walk& K, const VoxParams& P, i
   for (int c = lo; c <= hi; c++, mc = morton_step(mc, w)) {
     const float pc = (float)c * u;
     const float t = (nw * pc + Ka) + Kb;
-    if ((t + d1) * (t + d2) > 0.0f) continue;
-    bool ok = true;
+    // all seven tests, then one branch
+    bool ok = !((t + d1) * (t + d2) > 0.0f);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      if ((K.f[F_PV + k] * pc + R.kp[k]) + K.f[F_PD + k] < 0.0f) ok = false;
-      if ((K.f[F_QV + k] * pc + KQ[k]) + K.f[F_QD + k] < 0.0f) ok = false;
+      ok &= !((K.f[F_PV + k] * pc + R.kp[k]) + K.f[F_PD + k] < 0.0f);
+      ok &= !((K.f[F_QV + k] * pc + KQ[k]) + K.f[F_QD + k] < 0.0f);
     }
     if (!ok) continue;
     // cbase and mc occupy disjoint bits: the sum is their union, and an action's
@@ -448,10 +452,11 @@ __device__ __forceinline__ void walk_row_span(const Walk& K, const VoxParams& P,
 }
 
 // All columns of a walk, row by row, the Morton parts of a and b stepped incrementally;
-// rows are narrowed to their interval for actions that walk segments of several columns
-// (rows narrower than kRowSpanMin columns are walked whole).
+// with 'narrow' (units whose items are segments of several columns) rows are narrowed to
+// their interval (rows narrower than kRowSpanMin columns are walked whole).
 template <typename Act>
-__device__ __forceinline__ void walk_all_columns(const Walk& K, const VoxParams& P, Act& act) {
+__device__ __forceinline__ void walk_all_columns(const Walk& K, const VoxParams& P, Act& act,
+                                                 bool narrow) {
   const int w = K.w();
   const int ua = (w == 0) ? 1 : 0;
   const int va = (w == 2) ? 1 : 2;
@@ -459,7 +464,7 @@ __device__ __forceinline__ void walk_all_columns(const Walk& K, const VoxParams&
   const uint64_t ma0 = spread3_s((uint32_t)K.i[I_G0U]) << ua;
   for (int b = K.i[I_G0V]; b <= K.i[I_G1V]; b++, mb = morton_step(mb, va)) {
     const ColRow R = col_row(K, P, b);
-    if constexpr (Act::kSegCols > 1) {
+    if (narrow) {
       int i0, i1;
       row_interval(K, R, P.unit, i0, i1);
       if (i0 > i1) continue;
@@ -487,10 +492,10 @@ struct ActArgs {
 // flagged bricks in Morton order (table entry = slot + 1) and lists them.
 struct TouchAction {
   static constexpr bool kTouch = true;
-  // columns of a row per item of the small tier: with more than one, an item walks
-  // its segment narrowed to the row's interval; atomic actions take one (the lanes
-  // of a wave then update neighbouring cells, whose atomics share cache lines)
-  static constexpr int kSegCols = 8;
+  // columns of a row per item of the small tier, per unit: with more than one, an item
+  // walks its segment narrowed to the row's interval; atomic updates take one (the
+  // lanes of a wave then update neighbouring cells, whose atomics share cache lines)
+  __device__ static int seg_cols(const VoxParams&) { return 8; }
   uint32_t* table;
   uint64_t brick0;  // first brick of the unit's table segment
   __device__ static TouchAction make(const ActArgs& A, const VoxParams& P) {
@@ -523,7 +528,11 @@ typedef __attribute__((address_space(1))) uint32_t gword_t;
 template <bool kWide, bool kLaneOrder = true>
 struct MarkDenseT {
   static constexpr bool kTouch = false;
-  static constexpr int kSegCols = kWide ? 1 : 8;
+  __device__ static int seg_cols(const VoxParams& P) {
+    if (!kWide) return 16;
+    if (kLaneOrder) return 1;  // the single-unit payload pass: its unit is wide
+    return P.wide ? 1 : 16;
+  }
   gword_t* table;
   gbyte_t* bytes;
   uint64_t base;  // code of the first voxel of the unit's first brick
@@ -573,7 +582,7 @@ using MarkDenseWideBatchAction = MarkDenseT<true, false>;
 
 struct MarkPoolAction {
   static constexpr bool kTouch = false;
-  static constexpr int kSegCols = 8;
+  __device__ static int seg_cols(const VoxParams&) { return 8; }
   const gword_t* table;
   gbyte_t* bytes;  // the slot pool
   ull* ctr;
@@ -612,7 +621,7 @@ struct MarkPoolAction {
 // Second payload pass: lowest triangle index per voxel rank.
 struct WinnerAction {
   static constexpr bool kTouch = false;
-  static constexpr int kSegCols = 1;
+  __device__ static int seg_cols(const VoxParams&) { return 1; }
   const uint32_t* table;
   const ull* pool;  // bit form of the bricks, written by the compaction
   const uint32_t* wprefix;
@@ -660,7 +669,7 @@ __device__ __forceinline__ uint32_t upper_index(const T* __restrict__ a, uint32_
 // unit from a device work queue (tile_off[u] = first tile of unit u); tt is smaller for
 // passes with few triangles, so that their tiles still spread over the CUs. For a tile, each thread builds
 // its triangle's walk constants into LDS; a block scan lays out the (triangle, row
-// segment) items, a segment being Act::kSegCols neighbouring columns of one row; then the
+// segment) items, a segment being Act::seg_cols neighbouring columns of one row; then the
 // threads share the items, so the work per lane is a few voxel tests regardless of
 // triangle size. Triangles with more than small_cols columns
 // (or non-slab triangles with more than small_cols * 8 tests) go to the column-parallel
@@ -683,7 +692,7 @@ __device__ __forceinline__ bool skipped(const VoxParams& P, int skip) {
 }
 
 template <typename Act>
-__global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict__ units,
+__global__ __launch_bounds__(kTile, 4) void k_vox_tile(const VoxParams* __restrict__ units,
                                                     uint32_t n_units,
                                                     const uint32_t* __restrict__ tile_off,
                                                     ActArgs A, ull* queue,
@@ -695,7 +704,6 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
   __shared__ uint32_t soff[kTileTris + 1];
   __shared__ uint32_t wsum[kTile / kWave];
   __shared__ uint32_t s_tile;
-  constexpr int kSeg = Act::kSegCols;
   fill_spread_table();  // read after the first barrier of the tile loop
   ull* ctr = A.ctr;
   const uint32_t total_tiles = tile_off[n_units];
@@ -754,12 +762,13 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
       } else if (ncols <= (uint64_t)P.direct_cols) {
         // few columns: this thread walks them now, from registers
         Act act = Act::make(A, P);
-        walk_all_columns(K, P, act);
+        walk_all_columns(K, P, act, Act::seg_cols(P) > 1);
         act.flush();
       } else {
-        // items: the rows' segments of kSeg columns
+        // items: the rows' segments of seg columns
+        const uint32_t seg = (uint32_t)Act::seg_cols(P);
         const uint32_t eu = (uint32_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
-        items = (uint32_t)(K.i[I_G1V] - K.i[I_G0V] + 1) * ((eu + kSeg - 1) / kSeg);
+        items = (uint32_t)(K.i[I_G1V] - K.i[I_G0V] + 1) * ((eu + seg - 1) / seg);
 #pragma unroll
         for (int f = 0; f < kWalkF; f++) sk[f][tid] = __float_as_uint(K.f[f]);
 #pragma unroll
@@ -782,10 +791,17 @@ __global__ __launch_bounds__(kTile) void k_vox_tile(const VoxParams* __restrict_
       for (int f = 0; f < kWalkI; f++) K.i[f] = (int)sk[kWalkF + f][lo];
       const uint32_t local = it - soff[lo];
       const uint32_t eu = (uint32_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
-      const uint32_t nseg = (eu + kSeg - 1) / kSeg;
-      const int a = K.i[I_G0U] + (int)(local % nseg) * kSeg;
-      const int b = K.i[I_G0V] + (int)(local / nseg);
-      walk_row_span<(kSeg > 1)>(K, P, b, a, min(a + kSeg - 1, K.i[I_G1U]), act);
+      const int seg = Act::seg_cols(P);  // the tile's unit: uniform
+      if (seg > 1) {
+        const uint32_t nseg = (eu + seg - 1) / seg;
+        const int a = K.i[I_G0U] + (int)(local % nseg) * seg;
+        const int b = K.i[I_G0V] + (int)(local / nseg);
+        walk_row_span<true>(K, P, b, a, min(a + seg - 1, K.i[I_G1U]), act);
+      } else {
+        const int a = K.i[I_G0U] + (int)(local % eu);
+        const int b = K.i[I_G0V] + (int)(local / eu);
+        walk_row_span<false>(K, P, b, a, a, act);
+      }
     }
     act.flush();
     __syncthreads();  // LDS (setups, offsets, s_tile) is reused by the next tile
@@ -1816,75 +1832,85 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     st_occ = st_col + w_col;
     st_bc = st_occ + w_occ;
     queues = st_bc + w_bc;
-    {  // pass counters, scan status words and tile queues
-      const uint64_t ns = w_col + w_occ + w_bc + 3 * kQueueWords;
-      SVO_LAUNCH(k_zero2_u64, dim3(fill_grid(kCtrPassClear + ns)), dim3(256), 0,
-                         c.stream, (uint64_t*)ctr, (uint64_t)kCtrPassClear,
-                         c.scan_status.as<uint64_t>(), ns);
+    // the pass's kernels up to its read-back: one graph segment
+    SVO_TRY(run_segment(c, [&]() -> hipError_t {
+      {  // pass counters, scan status words and tile queues
+        const uint64_t ns = w_col + w_occ + w_bc + 3 * kQueueWords;
+        SVO_LAUNCH(k_zero2_u64, dim3(fill_grid(kCtrPassClear + ns)), dim3(256), 0,
+                           c.stream, (uint64_t*)ctr, (uint64_t)kCtrPassClear,
+                           c.scan_status.as<uint64_t>(), ns);
+        SVO_TRY(hipGetLastError());
+      }
+      // first pass: dense units mark directly, touch units flag their bricks; both collect
+      // the large triangles
+      // payload passes: the lane-neighbour check before the atomic pays on one mesh (a fine
+      // payload terrain at 1024: mark 357 -> 213 us) but cost the 512-mesh batched mix 11%
+      if (dense && geometry_only)
+        SVO_TRY(tile_pass(MarkDenseAction{}, "vox_small_mark", queues, 1, 0));
+      else if (dense && nu == 1)
+        SVO_TRY(tile_pass(MarkDenseWideAction{}, "vox_small_mark", queues, 1, 0));
+      else if (dense)
+        SVO_TRY(tile_pass(MarkDenseWideBatchAction{}, "vox_small_mark", queues, 1, 0));
+      else
+        SVO_TRY(tile_pass(TouchAction{}, "vox_small_touch", queues, 1, 0));
+      SVO_TRY((scan_single<uint64_t>(ColLoad{l_cols},
+                                     CountStore{l_off, ctr + kCtrCols},
+                                     ntri, st_col, ctr + kCtrError, c.stream,
+                                     ctr + kCtrLargeCount)));
+      if (dense) {
+        if (geometry_only)
+          SVO_TRY(large_pass(MarkDenseAction{}, "vox_large_mark", 0));
+        else  // (the large tier's lanes are not in triangle order)
+          SVO_TRY(large_pass(MarkDenseWideBatchAction{}, "vox_large_mark", 0));
+        // number the flagged bricks in Morton order: table entry = rank + 1, occ = list
+        SVO_TRY(occ_scan());
+      } else {
+        SVO_TRY(large_pass(TouchAction{}, "vox_large_touch", 0));
+        SVO_TRY(occ_scan());
+        // mark pass: byte stores into the slots
+        SVO_TRY(tile_pass(MarkPoolAction{}, "vox_small_mark", queues + kQueueWords, 0, 0));
+        SVO_TRY(large_pass(MarkPoolAction{}, "vox_large_mark", 0));
+      }
+      // per-brick voxel counts (bytes packed to bits, bytes cleared) and their offsets
+      ktimer_begin(c, "brick_count");
+      SVO_LAUNCH(k_brick_count, dim3(brick_grid(bc_bound)), dim3(256), 0, c.stream,
+                         c.occ_list.as<uint32_t>(), ctr, bc_bound, d_units, d_ubase, nu,
+                         c.pool8.as<uint8_t>(), pool_words, c.bcount.as<uint64_t>(),
+                         dense ? c.bcode.as<uint64_t>() + nb0 : (uint64_t*)nullptr,
+                         dense ? c.bshape.as<uint32_t>() + nb0 : (uint32_t*)nullptr,
+                         dense ? c.table.as<uint32_t>() : (uint32_t*)nullptr);
+      ktimer_end(c);
       SVO_TRY(hipGetLastError());
-    }
-    // first pass: dense units mark directly, touch units flag their bricks; both collect
-    // the large triangles
-    // payload passes: the lane-neighbour check before the atomic pays on one mesh (a fine
-    // payload terrain at 1024: mark 357 -> 213 us) but cost the 512-mesh batched mix 11%
-    if (dense && geometry_only)
-      SVO_TRY(tile_pass(MarkDenseAction{}, "vox_small_mark", queues, 1, 0));
-    else if (dense && nu == 1)
-      SVO_TRY(tile_pass(MarkDenseWideAction{}, "vox_small_mark", queues, 1, 0));
-    else if (dense)
-      SVO_TRY(tile_pass(MarkDenseWideBatchAction{}, "vox_small_mark", queues, 1, 0));
-    else
-      SVO_TRY(tile_pass(TouchAction{}, "vox_small_touch", queues, 1, 0));
-    SVO_TRY((scan_single<uint64_t>(ColLoad{l_cols},
-                                   CountStore{l_off, ctr + kCtrCols},
-                                   ntri, st_col, ctr + kCtrError, c.stream,
-                                   ctr + kCtrLargeCount)));
-    if (dense) {
-      if (geometry_only)
-        SVO_TRY(large_pass(MarkDenseAction{}, "vox_large_mark", 0));
-      else  // (the large tier's lanes are not in triangle order)
-        SVO_TRY(large_pass(MarkDenseWideBatchAction{}, "vox_large_mark", 0));
-      // number the flagged bricks in Morton order: table entry = rank + 1, occ = list
-      SVO_TRY(occ_scan());
-    } else {
-      SVO_TRY(large_pass(TouchAction{}, "vox_large_touch", 0));
-      SVO_TRY(occ_scan());
-      // mark pass: byte stores into the slots
-      SVO_TRY(tile_pass(MarkPoolAction{}, "vox_small_mark", queues + kQueueWords, 0, 0));
-      SVO_TRY(large_pass(MarkPoolAction{}, "vox_large_mark", 0));
-    }
-    // per-brick voxel counts (bytes packed to bits, bytes cleared) and their offsets
-    ktimer_begin(c, "brick_count");
-    SVO_LAUNCH(k_brick_count, dim3(brick_grid(bc_bound)), dim3(256), 0, c.stream,
-                       c.occ_list.as<uint32_t>(), ctr, bc_bound, d_units, d_ubase, nu,
-                       c.pool8.as<uint8_t>(), pool_words, c.bcount.as<uint64_t>(),
-                       dense ? c.bcode.as<uint64_t>() + nb0 : (uint64_t*)nullptr,
-                       dense ? c.bshape.as<uint32_t>() + nb0 : (uint32_t*)nullptr,
-                       dense ? c.table.as<uint32_t>() : (uint32_t*)nullptr);
-    ktimer_end(c);
-    SVO_TRY(hipGetLastError());
-    if (bc_single && dense)
-      SVO_TRY((scan_single<uint64_t>(
-          ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
-          BrickOffStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox,
-                        c.bvoff.as<uint64_t>() + nb0, s0},
-          bc_bound, st_bc, ctr + kCtrError, c.stream, ctr + kCtrNumOcc)));
-    else if (bc_single)
-      SVO_TRY((scan_single<uint64_t>(
-          ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
-          CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox}, bc_bound,
-          st_bc, ctr + kCtrError, c.stream, ctr + kCtrNumOcc)));
-    else if (dense)
-      SVO_TRY((scan_exclusive<uint64_t>(
-          ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
-          BrickOffStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox,
-                        c.bvoff.as<uint64_t>() + nb0, s0},
-          bc_bound, c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrNumOcc)));
-    else
-      SVO_TRY((scan_exclusive<uint64_t>(
-          ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
-          CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox}, bc_bound,
-          c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrNumOcc)));
+      if (bc_single && dense)
+        SVO_TRY((scan_single<uint64_t>(
+            ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
+            BrickOffStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox,
+                          c.bvoff.as<uint64_t>() + nb0, s0},
+            bc_bound, st_bc, ctr + kCtrError, c.stream, ctr + kCtrNumOcc)));
+      else if (bc_single)
+        SVO_TRY((scan_single<uint64_t>(
+            ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
+            CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox}, bc_bound,
+            st_bc, ctr + kCtrError, c.stream, ctr + kCtrNumOcc)));
+      else if (dense)
+        SVO_TRY((scan_exclusive<uint64_t>(
+            ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
+            BrickOffStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox,
+                          c.bvoff.as<uint64_t>() + nb0, s0},
+            bc_bound, c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrNumOcc)));
+      else
+        SVO_TRY((scan_exclusive<uint64_t>(
+            ArrayLoad<uint64_t>{c.bcount.as<uint64_t>()},
+            CountStore{c.bcount.as<uint64_t>(), ctr + kCtrNumVox}, bc_bound,
+            c.scan_scratch.as<uint64_t>(), c.stream, ctr + kCtrNumOcc)));
+      if (!(nu == 1 && geometry_only)) {
+        SVO_LAUNCH(k_unit_voxels, dim3((nu + 256) / 256), dim3(256), 0, c.stream,
+                   c.occ_list.as<uint32_t>(), ctr, d_ubase, nu, c.bcount.as<uint64_t>(),
+                   d_uvoff);
+        SVO_TRY(hipGetLastError());
+      }
+      return hipSuccess;
+    }));
     if (nu == 1 && geometry_only) {
       // one unit and no payload records (the only device reader of the unit offsets):
       // its voxels are the whole pass, [0, voxel count)
@@ -1893,10 +1919,6 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
       uvoff[0] = 0;
       uvoff[1] = hc[kCtrNumVox] & kVoxMask;
     } else {
-      SVO_LAUNCH(k_unit_voxels, dim3((nu + 256) / 256), dim3(256), 0, c.stream,
-                         c.occ_list.as<uint32_t>(), ctr, d_ubase, nu, c.bcount.as<uint64_t>(),
-                         d_uvoff);
-      SVO_TRY(hipGetLastError());
       const ReadReq rq[2] = {{uvoff.data(), d_uvoff, (nu + 1) * 8},
                              {hc, c.counters.p, kNumCounters * sizeof(ull)}};
       SVO_TRY(read_back(c, rq, 2));
@@ -1995,56 +2017,62 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
       SVO_TRY(ensure_keep(c.bvoff, (nb0 + n_occ) * 8, nb0 * 8, c.stream));
       SVO_TRY(ensure_keep(c.bshape, (nb0 + n_occ) * 4, nb0 * 4, c.stream));
     }
-    // a dense pass has its brick stream entries already (k_brick_count); the emit pass
-    // writes codes and winners (rewriting those entries with the same values)
-    if (!dense || codes_now) {
-      ktimer_begin(c, "brick_emit");
-      // dense payload units: their cells are staged through LDS, kBrickVox words per wave
-      const bool stage_cells = codes_now && dense && any_payload;
-      SVO_LAUNCH(codes_now ? k_brick_emit<true> : k_brick_emit<false>,
-                       dim3(brick_grid(n_occ)), dim3(256),
-                       stage_cells ? (256 / kWave) * kBrickVox * 4 : 0, c.stream,
-                       c.occ_list.as<uint32_t>(), ctr, d_units, d_ubase, nu, pool_words,
-                       c.bcount.as<uint64_t>(), c.codes.as<uint64_t>() + s0,
-                       winner_walk ? c.wprefix.as<uint32_t>() : (uint32_t*)nullptr, s0,
-                       c.bwords.as<uint64_t>() + nb0 * kBrickWords, c.bcode.as<uint64_t>() + nb0,
-                       c.bvoff.as<uint64_t>() + nb0, c.bshape.as<uint32_t>() + nb0,
-                       any_payload ? c.winner.as<uint32_t>() : (uint32_t*)nullptr,
-                       stage_cells ? 1 : 0);
-      ktimer_end(c);
-      SVO_TRY(hipGetLastError());
+    if (any_payload) {
+      // records: each job's first record is its null record
+      const uint64_t keep = batch ? c.records.cap : (1 + s0) * 32;
+      SVO_TRY(ensure_keep(c.records, n_records * 32, keep, c.stream));
     }
+    // the pass's remaining kernels: one graph segment
+    SVO_TRY(run_segment(c, [&]() -> hipError_t {
+      // a dense pass has its brick stream entries already (k_brick_count); the emit pass
+      // writes codes and winners (rewriting those entries with the same values)
+      if (!dense || codes_now) {
+        ktimer_begin(c, "brick_emit");
+        // dense payload units: their cells are staged through LDS, kBrickVox words per wave
+        const bool stage_cells = codes_now && dense && any_payload;
+        SVO_LAUNCH(codes_now ? k_brick_emit<true> : k_brick_emit<false>,
+                   dim3(brick_grid(n_occ)), dim3(256),
+                   stage_cells ? (256 / kWave) * kBrickVox * 4 : 0, c.stream,
+                   c.occ_list.as<uint32_t>(), ctr, d_units, d_ubase, nu, pool_words,
+                   c.bcount.as<uint64_t>(), c.codes.as<uint64_t>() + s0,
+                   winner_walk ? c.wprefix.as<uint32_t>() : (uint32_t*)nullptr, s0,
+                   c.bwords.as<uint64_t>() + nb0 * kBrickWords, c.bcode.as<uint64_t>() + nb0,
+                   c.bvoff.as<uint64_t>() + nb0, c.bshape.as<uint32_t>() + nb0,
+                   any_payload ? c.winner.as<uint32_t>() : (uint32_t*)nullptr,
+                   stage_cells ? 1 : 0);
+        ktimer_end(c);
+        SVO_TRY(hipGetLastError());
+      }
+      if (any_payload) {
+        if (winner_walk) {
+          SVO_TRY(fill_u32(c.winner.as<uint32_t>(), 0xFFFFFFFFu, nvox, c.stream));
+          SVO_TRY(tile_pass(WinnerAction{}, "vox_small_winner", queues + 2 * kQueueWords, 0,
+                            kSkipBinary));
+          if (n_large) SVO_TRY(large_pass(WinnerAction{}, "vox_large_winner", kSkipBinary));
+        }
+        ktimer_begin(c, "payload_records");
+        SVO_LAUNCH(k_payload_records, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
+                   c.stream, c.jobdev.as<JobDev>(), (const VoxParams*)d_units,
+                   (const uint64_t*)d_uvoff, nu, c.codes.as<uint64_t>(), s0,
+                   c.winner.as<uint32_t>(), nvox, c.records.as<float>(), ctr);
+        ktimer_end(c);
+        SVO_TRY(hipGetLastError());
+      }
+      if (!dense) {  // (dense passes cleared their table entries in k_brick_count)
+        ktimer_begin(c, "brick_clear");
+        SVO_LAUNCH(k_brick_clear,
+                   dim3((unsigned)std::min<uint64_t>((n_occ + 255) / 256, 4096)),
+                   dim3(256), 0, c.stream, c.occ_list.as<uint32_t>(), ctr,
+                   c.table.as<uint32_t>());
+        ktimer_end(c);
+        SVO_TRY(hipGetLastError());
+      }
+      return hipSuccess;
+    }));
     c.n_bricks = nb0 + n_nonempty;
     bool any_binary = false;
     for (uint32_t u = 0; u < nu; u++) any_binary |= units[u].stride == 9;
     if (!codes_now || any_binary) c.codes_lazy = true;
-
-    if (any_payload) {
-      if (winner_walk) {
-        SVO_TRY(fill_u32(c.winner.as<uint32_t>(), 0xFFFFFFFFu, nvox, c.stream));
-        SVO_TRY(tile_pass(WinnerAction{}, "vox_small_winner", queues + 2 * kQueueWords, 0, kSkipBinary));
-        if (n_large) SVO_TRY(large_pass(WinnerAction{}, "vox_large_winner", kSkipBinary));
-      }
-      // records: each job's first record is its null record
-      const uint64_t keep = batch ? c.records.cap : (1 + s0) * 32;
-      SVO_TRY(ensure_keep(c.records, n_records * 32, keep, c.stream));
-      ktimer_begin(c, "payload_records");
-      SVO_LAUNCH(k_payload_records, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
-                         c.stream, c.jobdev.as<JobDev>(), (const VoxParams*)d_units,
-                         (const uint64_t*)d_uvoff, nu, c.codes.as<uint64_t>(), s0,
-                         c.winner.as<uint32_t>(), nvox, c.records.as<float>(), ctr);
-      ktimer_end(c);
-      SVO_TRY(hipGetLastError());
-    }
-    if (!dense) {  // (dense passes cleared their table entries in k_brick_count)
-      ktimer_begin(c, "brick_clear");
-      SVO_LAUNCH(k_brick_clear,
-                         dim3((unsigned)std::min<uint64_t>((n_occ + 255) / 256, 4096)),
-                         dim3(256), 0, c.stream, c.occ_list.as<uint32_t>(), ctr,
-                         c.table.as<uint32_t>());
-      ktimer_end(c);
-      SVO_TRY(hipGetLastError());
-    }
   }
   c.n_vox = s0 + nvox;
   *nfilled = nvox;

@@ -43,7 +43,9 @@ class RunInfo(C.Structure):
     _fields_ = [("n_partitions", C.c_uint64), ("nfilled", C.c_uint64),
                 ("n_nodes", C.c_uint64), ("n_data", C.c_uint64), ("hazard", C.c_int),
                 ("ms_partition", C.c_float), ("ms_voxelize", C.c_float),
-                ("ms_build", C.c_float), ("launches", C.c_uint32), ("host_syncs", C.c_uint32)]
+                ("ms_build", C.c_float), ("launches", C.c_uint32), ("host_syncs", C.c_uint32),
+                ("graph_launches", C.c_uint32), ("graph_replays", C.c_uint32),
+                ("graph_kernels", C.c_uint32)]
 
 
 class Job(C.Structure):
@@ -76,6 +78,7 @@ SIGNATURES = {
     "svo_ctx_set_stream": (_i32, [_vp, _vp]),
     "svo_ctx_stream": (_vp, [_vp]),
     "svo_ctx_set_tuning": (_i32, [_vp, _i32, _u64]),
+    "svo_ctx_set_graphs": (_i32, [_vp, _i32]),
     "svo_estimate_partitions": (_u64, [_u64, _u64]),
     "svo_mesh_load": (_i32, [_vp, _vp, _u64, _i32, _i32]),
     "svo_mesh_load_file": (_i32, [_vp, C.c_char_p, _u64, _i32, C.POINTER(_u64)]),
@@ -302,6 +305,10 @@ class SvoContext:
         """Voxelizer split threshold and dense-grid limit (output is unchanged)."""
         lim = 0xFFFFFFFFFFFFFFFF if dense_limit_mb is None else int(dense_limit_mb)
         _check(load_library().svo_ctx_set_tuning(self._h, int(small_cols), lim))
+
+    def set_graphs(self, on: bool):
+        """hipGraph capture and replay of batch_run's kernel sequences (on by default)."""
+        _check(load_library().svo_ctx_set_graphs(self._h, 1 if on else 0))
 
     # -- stages -----------------------------------------------------------------
     def load_mesh(self, records, payload: bool):

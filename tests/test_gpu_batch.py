@@ -152,3 +152,48 @@ def test_batch_with_levels_jobs():
                 color_mode=j.get("color_mode", 0), levels=True)
             assert b.nodes_range(r.node_offset, r.n_nodes).tobytes() == nodes_w.tobytes()
             assert b.data_range(r.data_offset, r.n_data).tobytes() == data_w.tobytes()
+
+
+def _batch_outputs(ctx, jobs):
+    res, tot = ctx.batch_run(jobs)
+    out = [((r.n_partitions, r.nfilled, r.n_nodes, r.n_data, r.hazard),
+            ctx.nodes_range(r.node_offset, r.n_nodes).tobytes(),
+            ctx.data_range(r.data_offset, r.n_data).tobytes()) for r in res]
+    return out, tot
+
+
+def test_batch_graph_replays_match_eager():
+    """svo_batch_run with graphs: the first call captures each kernel sequence between
+    host synchronizations into a graph, later calls with the same launches replay the kept
+    graphs. Every replay (with eager batches, plain runs and synchronizations between
+    them) gives the eager batch's bytes; a different batch captures anew."""
+    jobs = _mix(3)
+    other = [jobs[i] for i in (12, 4, 0, 9)]
+    with SvoContext(0) as e, SvoContext(0) as g, SvoContext(0) as p:
+        e.set_graphs(False)
+        _say("graph test: eager batch of %d jobs" % len(jobs))
+        want, tot_e = _batch_outputs(e, jobs)
+        assert tot_e.graph_launches == 0
+        want_other, _ = _batch_outputs(e, other)
+        for rep in range(4):
+            _say("graph test: batch call %d" % rep)
+            got, tot = _batch_outputs(g, jobs)
+            assert tot.graph_launches > 0 and tot.graph_kernels > 0
+            if rep == 0:
+                assert tot.graph_replays == 0
+            elif rep == 1:  # a buffer first allocated after a segment's capture (the
+                # payload winners, after the marking) changes that segment's arguments
+                assert tot.graph_replays > 0
+            else:  # from then on every segment replays
+                assert tot.graph_replays == tot.graph_launches, (tot.graph_replays,
+                                                                 tot.graph_launches)
+            for i, (a, b) in enumerate(zip(got, want)):
+                assert a == b, (rep, i)
+            # eager work between the replays: a plain run and an eager batch, synchronized
+            _plain(p, jobs[rep])
+            assert _batch_outputs(e, other)[0] == want_other
+        _say("graph test: a different batch, then the first again")
+        got, tot = _batch_outputs(g, other)
+        assert got == want_other
+        got, tot = _batch_outputs(g, jobs)
+        assert got == want
