@@ -599,7 +599,7 @@ class MixBench:
         self._plain(large)
         large_ms = (time.perf_counter() - t) * 1e3
         rng = np.random.default_rng(5)
-        rates = (50, 200, 1000, 2000, 4000, 6000)
+        rates = (50, 200, 1000, 2000, 4000, 6000, 9000, 12000)
         rows = []
         srv = SvoServer(0, budget_mb=200 * 1024)
         # warm the slots (allocations) with one pass over a few jobs

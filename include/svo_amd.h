@@ -311,6 +311,15 @@ typedef struct svo_server_stats {
   uint64_t peak_admitted_bytes;
   uint64_t batched_jobs; /* jobs over all batches */
   double batch_ms;       /* wall time the slots spent running batches (summed) */
+  uint64_t fallback_batches; /* batches that failed and ran job by job */
+  uint64_t buffer_retries;   /* result-buffer allocations that emptied the cache first */
+  uint64_t admitted_bytes;   /* now: running estimates + held results */
+  uint64_t held_bytes;       /* now: results held until released */
+  uint64_t queued;           /* now: jobs waiting for admission */
+  double shared_ms;  /* wall time in the shared batches (svo_batch_run), summed */
+  double device_ms;  /* ... of which their stages' device time */
+  double collect_ms; /* wall time copying results into the tickets' buffers */
+  double alone_ms;   /* wall time of jobs run alone (svo_run) */
 } svo_server_stats;
 int svo_server_stats_get(svo_server* srv, svo_server_stats* out);
 

@@ -179,24 +179,26 @@ hipError_t scan_exclusive_items(Load ld, Store st, uint64_t n, T* scratch, hipSt
 }
 
 // ---------------------------------------------------------------- single-pass scan
-// One launch (decoupled look-back, Merrill & Garland 2016): block b scans tile b,
-// publishes its tile's sum, then its wave 0 walks back over the predecessors'
-// status words until it meets an inclusive prefix, publishes its own inclusive prefix and
-// the block applies the stores. A status word is the flag (bits 63..62: 0 = not yet
-// published, 1 = the tile's own sum, 2 = inclusive prefix through the tile) and the value
-// (bits 61..0; every prefix must stay below 2^62) in one word, written and read with
-// device-scope atomics: those bypass the per-XCD L2 caches, so the word is coherent
-// across the chip without a fence. Workgroups are dispatched in index order, so every
-// tile a block waits on belongs to a block that is already running (a shared ticket
-// counter would serialize the blocks on one atomic). The wait is bounded all the same:
-// after kScanSpinLimit polls the block counts an error in *err and goes on with a zero
-// prefix.
+// One launch (decoupled look-back, Merrill & Garland 2016): a block takes the next tile
+// number from a ticket counter, scans that tile, publishes its tile's sum, then its wave 0
+// walks back over the predecessors' status words until it meets an inclusive prefix,
+// publishes its own inclusive prefix and the block applies the stores. A status word is
+// the flag (bits 63..62: 0 = not yet published, 1 = the tile's own sum, 2 = inclusive
+// prefix through the tile) and the value (bits 61..0; every prefix must stay below 2^62)
+// in one word, written and read with device-scope atomics: those bypass the per-XCD L2
+// caches, so the word is coherent across the chip without a fence. With tiles numbered in
+// the order blocks start, every tile a block waits on belongs to a block that is already
+// running. (Numbering by block index is not enough: the chip hands workgroups to its
+// eight XCDs in turn, and a block whose XCD is full waits while its successor runs on
+// another; with a second stream's look-back blocks filling the XCDs, two scans could wait
+// on each other.) The wait is bounded all the same: after kScanSpinLimit polls the block
+// counts an error in *err and goes on with a zero prefix.
 //
 // Blocks are 1024 threads (tiles of 8192 elements for large inputs), so that the walk
 // back is short; tiles at or beyond the actual element count exit at once (no later tile
 // waits on them).
 //
-// status: 1 + tiles words, zero before the launch ([0] is unused).
+// status: 1 + tiles words, zero before the launch ([0] is the ticket counter).
 constexpr int kSingleBlock = 1024;
 constexpr uint64_t kSingleSmall = 1ull << 16;  // below: one item per thread
 constexpr unsigned long long kStAgg = 1ull << 62;
@@ -219,7 +221,10 @@ __global__ __launch_bounds__(kSingleBlock) void k_scan_single(Load ld, Store st,
                                                               unsigned long long* err) {
   __shared__ T sm[kSingleBlock / kWave];
   __shared__ T s_prefix;
-  const uint32_t tile = blockIdx.x;
+  __shared__ uint32_t s_tile;
+  if (threadIdx.x == 0) s_tile = (uint32_t)atomicAdd(status, 1ull);
+  __syncthreads();
+  const uint32_t tile = s_tile;
   unsigned long long* tiles = status + 1;
   if (n_dev) n = min(n, (uint64_t)*n_dev);
   if ((uint64_t)tile * kSingleBlock * ITEMS >= n) return;  // uniform per block

@@ -107,6 +107,7 @@ struct svo_server {
     std::vector<void*> drop;
     {
       std::lock_guard<std::mutex> lk(m);
+      stats.buffer_retries++;
       for (auto& kv : free_bufs) {
         drop.insert(drop.end(), kv.second.begin(), kv.second.end());
         kv.second.clear();
@@ -182,13 +183,20 @@ static int run_alone(svo_server* s, svo_ctx* c, Ticket* t) {
 // Run a batch on a slot: a lone job and every -levels job through the plain path, the
 // others as one batch; on a batch failure each job is run alone so that one bad job fails
 // alone, with its own message.
+static double ms_since(Clock::time_point t) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
+}
+
 int svo_server::run_jobs(uint32_t slot, std::vector<Ticket*>& batch) {
   svo_ctx* c = ctx[slot];
   std::vector<Ticket*> shared;
+  double alone = 0, run = 0, dev = 0, coll = 0;
   for (Ticket* t : batch) {
     if (batch.size() == 1 || t->job.generate_levels) {
+      const Clock::time_point t0 = Clock::now();
       t->status = run_alone(this, c, t);
       (void)hipStreamSynchronize((hipStream_t)svo_ctx_stream(c));
+      alone += ms_since(t0);
     } else {
       shared.push_back(t);
     }
@@ -197,21 +205,40 @@ int svo_server::run_jobs(uint32_t slot, std::vector<Ticket*>& batch) {
     std::vector<svo_job> jobs;
     for (Ticket* t : shared) jobs.push_back(t->job);
     std::vector<svo_job_result> res(shared.size());
-    const int rc = svo_batch_run(c, jobs.data(), (uint32_t)jobs.size(), res.data(), nullptr);
+    svo_run_info info{};
+    const Clock::time_point t0 = Clock::now();
+    const int rc = svo_batch_run(c, jobs.data(), (uint32_t)jobs.size(), res.data(), &info);
+    run = ms_since(t0);
     if (rc == SVO_OK) {
+      dev = (double)info.ms_partition + info.ms_voxelize + info.ms_build;
+      const Clock::time_point t1 = Clock::now();
       for (size_t i = 0; i < shared.size(); i++) shared[i]->status = collect(this, c, shared[i], res[i]);
+      coll = ms_since(t1);
     } else if (shared.size() == 1) {
       shared[0]->status = rc;
       shared[0]->error = svo_last_error();
     } else {
+      {
+        std::lock_guard<std::mutex> lk(m);
+        stats.fallback_batches++;
+      }
       for (Ticket* t : shared) {
         t->status = run_alone(this, c, t);
         (void)hipStreamSynchronize((hipStream_t)svo_ctx_stream(c));
       }
     }
   }
-  return hipStreamSynchronize((hipStream_t)svo_ctx_stream(c)) == hipSuccess ? SVO_OK
-                                                                             : SVO_ERR_DEVICE;
+  const Clock::time_point t2 = Clock::now();
+  const bool ok = hipStreamSynchronize((hipStream_t)svo_ctx_stream(c)) == hipSuccess;
+  coll += ms_since(t2);  // the result copies finish here
+  {
+    std::lock_guard<std::mutex> lk(m);
+    stats.alone_ms += alone;
+    stats.shared_ms += run;
+    stats.device_ms += dev;
+    stats.collect_ms += coll;
+  }
+  return ok ? SVO_OK : SVO_ERR_DEVICE;
 }
 
 void svo_server::worker(uint32_t slot) {
@@ -502,6 +529,11 @@ int svo_server_stats_get(svo_server* s, svo_server_stats* out) {
   if (!s || !out) return SVO_ERR_INVALID;
   std::lock_guard<std::mutex> lk(s->m);
   *out = s->stats;
+  out->admitted_bytes = s->admitted;
+  out->held_bytes = 0;
+  for (const auto& kv : s->tickets)
+    if (kv.second->state == SVO_JOB_DONE) out->held_bytes += kv.second->held_bytes;
+  out->queued = s->queue.size();
   return SVO_OK;
 }
 

@@ -67,7 +67,11 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
                 ("batches", C.c_uint64), ("max_batch_jobs_seen", C.c_uint64),
                 ("peak_admitted_bytes", C.c_uint64), ("batched_jobs", C.c_uint64),
-                ("batch_ms", C.c_double)]
+                ("batch_ms", C.c_double), ("fallback_batches", C.c_uint64),
+                ("buffer_retries", C.c_uint64), ("admitted_bytes", C.c_uint64),
+                ("held_bytes", C.c_uint64), ("queued", C.c_uint64),
+                ("shared_ms", C.c_double), ("device_ms", C.c_double),
+                ("collect_ms", C.c_double), ("alone_ms", C.c_double)]
 
 
 SIGNATURES = {
