@@ -602,8 +602,15 @@ class MixBench:
         rates = (50, 200, 1000, 2000, 4000, 6000, 9000, 12000)
         rows = []
         srv = SvoServer(0, budget_mb=200 * 1024)
-        # warm the slots (allocations) with one pass over a few jobs
-        for tk in [srv.submit(j) for j in self.jobs[:16] + [large]]:
+        # warm the slots: one burst of the whole mix and a few large jobs, so that their
+        # scratch reaches the sizes of the largest batches before anything is timed (a
+        # batch larger than any before grows the slot's buffers with the device drained;
+        # in a cold server such batches left the GPU idle for 1-2 s each, measured)
+        for tk in [srv.submit(j) for j in self.jobs + [large] * 8]:
+            srv.wait(tk)
+            srv.release(tk)
+        for _ in range(4):  # and large jobs alone (the plain path's buffers of each slot)
+            tk = srv.submit(large)
             srv.wait(tk)
             srv.release(tk)
 

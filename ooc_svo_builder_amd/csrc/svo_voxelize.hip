@@ -1700,10 +1700,17 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   }
   if (dense) {
     if (c.grid_zeroed < dense_bytes) {
-      if (c.grid8.cap < dense_bytes) release(c.grid8);  // exact size, no growth slack
-      SVO_TRY(ensure(c.grid8, dense_bytes));
-      SVO_TRY(fill_u64(c.grid8.as<uint64_t>(), 0, dense_bytes / 8, c.stream));
-      c.grid_zeroed = dense_bytes;
+      // grown to at least twice its size (at most the dense limit), so that passes of
+      // slowly growing spans (a server's batches) reallocate it a few times only
+      uint64_t target = dense_bytes;
+      if (c.grid8.cap < dense_bytes) {
+        target = std::max<uint64_t>(dense_bytes,
+                                    std::min<uint64_t>(c.dense_limit, 2 * (uint64_t)c.grid8.cap));
+        release(c.grid8);
+        SVO_TRY(ensure(c.grid8, target));
+      }
+      SVO_TRY(fill_u64(c.grid8.as<uint64_t>(), 0, target / 8, c.stream));
+      c.grid_zeroed = target;
     }
     for (uint32_t u = 0; u < nu; u++) {
       P[u].dense = 1;
