@@ -699,10 +699,12 @@ class MixBench:
                  " %d/%d sampled identical" % (
                      lam, row["jobs_per_s"], row["p50_ms"], row["p99_ms"],
                      row["plain_jobs_per_s"], row["plain_p99_ms"], checked[1], checked[0]))
+        budget = srv.stats().budget_bytes
         srv.close()
         cap_plain = len(self.jobs) / self.plain_s
         peak = max(r["jobs_per_s"] for r in rows)
-        return {"budget_gb": 200, "slots": 2, "window_s": seconds, "sweep": rows,
+        return {"budget_gb": 200, "budget_gb_in_force": budget / 1e9, "slots": 2,
+                "window_s": seconds, "sweep": rows,
                 "plain_capacity_jobs_per_s": cap_plain,
                 "peak_served_jobs_per_s": peak, "peak_vs_plain_capacity": peak / cap_plain,
                 "large_job_plain_ms": large_ms,

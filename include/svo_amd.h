@@ -320,6 +320,7 @@ typedef struct svo_server_stats {
   double device_ms;  /* ... of which their stages' device time */
   double collect_ms; /* wall time copying results into the tickets' buffers */
   double alone_ms;   /* wall time of jobs run alone (svo_run) */
+  uint64_t budget_bytes; /* the admission budget in force (the request, capped) */
 } svo_server_stats;
 int svo_server_stats_get(svo_server* srv, svo_server_stats* out);
 

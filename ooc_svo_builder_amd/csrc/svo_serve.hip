@@ -534,6 +534,7 @@ int svo_server_stats_get(svo_server* s, svo_server_stats* out) {
   for (const auto& kv : s->tickets)
     if (kv.second->state == SVO_JOB_DONE) out->held_bytes += kv.second->held_bytes;
   out->queued = s->queue.size();
+  out->budget_bytes = s->budget;
   return SVO_OK;
 }
 

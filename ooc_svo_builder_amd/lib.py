@@ -71,7 +71,8 @@ class ServerStats(C.Structure):
                 ("buffer_retries", C.c_uint64), ("admitted_bytes", C.c_uint64),
                 ("held_bytes", C.c_uint64), ("queued", C.c_uint64),
                 ("shared_ms", C.c_double), ("device_ms", C.c_double),
-                ("collect_ms", C.c_double), ("alone_ms", C.c_double)]
+                ("collect_ms", C.c_double), ("alone_ms", C.c_double),
+                ("budget_bytes", C.c_uint64)]
 
 
 SIGNATURES = {
