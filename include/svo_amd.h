@@ -59,8 +59,8 @@ void* svo_ctx_stream(svo_ctx* ctx);
 /* Voxelizer tuning (no reference counterpart): small_cols = the column count above which
  * a triangle goes to the column-parallel tier (<= 0 keeps the current value);
  * dense_limit_mb = largest partition voxelized in a dense byte grid, larger ones use
- * touched bricks only (UINT64_MAX keeps the current value; the default is a third of
- * the free HBM, at most 64 GiB). Neither changes the output. */
+ * touched bricks only (UINT64_MAX keeps the current value; the default is half of the
+ * free HBM at context creation, at most 128 GiB). Neither changes the output. */
 int svo_ctx_set_tuning(svo_ctx* ctx, int small_cols, uint64_t dense_limit_mb);
 
 /* hipGraph use of svo_batch_run on this context (no reference counterpart; on by

@@ -1136,8 +1136,13 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
       uint32_t* sc = s_cells + (threadIdx.x / kWave) * kBrickVox;
       uint4* src = (uint4*)((uint32_t*)units[u].grid8 + (ti - ubase[u]) * kBrickVox);
       uint4 v[kBrickVox / 4 / kWave];
+      // uint4 r * 64 + l holds cells of word 4 r + l / 16: the cells of an empty word are
+      // all zero, so only those of non-empty words are read
+      const uint64_t ne = __ballot(word != 0ull);
 #pragma unroll
-      for (int r = 0; r < kBrickVox / 4 / kWave; r++) v[r] = src[r * kWave + lane_id()];
+      for (int r = 0; r < kBrickVox / 4 / kWave; r++)
+        v[r] = ((ne >> (4 * r + (lane_id() >> 4))) & 1ull) ? src[r * kWave + lane_id()]
+                                                           : make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
       for (int r = 0; r < kBrickVox / 4 / kWave; r++) {
         ((uint4*)sc)[r * kWave + lane_id()] = v[r];
