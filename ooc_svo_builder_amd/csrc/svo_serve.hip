@@ -180,13 +180,13 @@ static int run_alone(svo_server* s, svo_ctx* c, Ticket* t) {
   return collect(s, c, t, r);
 }
 
-// Run a batch on a slot: a lone job and every -levels job through the plain path, the
-// others as one batch; on a batch failure each job is run alone so that one bad job fails
-// alone, with its own message.
 static double ms_since(Clock::time_point t) {
   return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
 }
 
+// Run a batch on a slot: a lone job and every -levels job through the plain path, the
+// others as one batch; on a batch failure each job is run alone so that one bad job fails
+// alone, with its own message.
 int svo_server::run_jobs(uint32_t slot, std::vector<Ticket*>& batch) {
   svo_ctx* c = ctx[slot];
   std::vector<Ticket*> shared;
