@@ -117,3 +117,32 @@ def test_oracle_cli_matches_in_memory_pipeline(tmp_path):
         assert nodes.tobytes() == n2.tobytes() and data.tobytes() == d2.tobytes()
         assert open(base + ".octree").read() == octree_header_text(G, len(n2), len(d2))
         assert not os.path.exists(base + ".trip")
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every struct of include/svo_amd.h that the Python binding mirrors has the C
+    compiler's size and field offsets (a field added on one side only would shift every
+    field after it)."""
+    from ooc_svo_builder_amd import lib
+    pairs = [("svo_run_info", lib.RunInfo), ("svo_job", lib.Job),
+             ("svo_job_result", lib.JobResult), ("svo_server_stats", lib.ServerStats)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "svo_amd.h"',
+             "int main(void) {"]
+    for cname, py in pairs:
+        lines.append('  printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for fname, _ in py._fields_:
+            lines.append('  printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, fname, cname,
+                                                                         fname))
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    got = dict(line.rsplit(" ", 1) for line in
+               subprocess.run([str(exe)], check=True, capture_output=True,
+                              text=True).stdout.splitlines())
+    for cname, py in pairs:
+        assert int(got["%s size" % cname]) == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got["%s.%s" % (cname, fname)]) == getattr(py, fname).offset, (cname, fname)
