@@ -703,10 +703,14 @@ class MixBench:
         srv.close()
         cap_plain = len(self.jobs) / self.plain_s
         peak = max(r["jobs_per_s"] for r in rows)
+        peak_plain = max(r["plain_jobs_per_s"] for r in rows)
         return {"budget_gb": 200, "budget_gb_in_force": budget / 1e9, "slots": 2,
                 "window_s": seconds, "sweep": rows,
                 "plain_capacity_jobs_per_s": cap_plain,
                 "peak_served_jobs_per_s": peak, "peak_vs_plain_capacity": peak / cap_plain,
+                # the same arrivals (mix and large jobs) through the plain loop, at its peak
+                "peak_plain_same_arrivals_jobs_per_s": peak_plain,
+                "peak_vs_plain_same_arrivals": peak / peak_plain,
                 "large_job_plain_ms": large_ms,
                 "plain_latency_model": "measured: svo_run one job after another in arrival order",
                 "sampled_check": "per rate up to 24 results (jobs at even spacing in the "
