@@ -496,6 +496,8 @@ struct TouchAction {
   // walks its segment narrowed to the row's interval; atomic updates take one (the
   // lanes of a wave then update neighbouring cells, whose atomics share cache lines)
   __device__ static int seg_cols(const VoxParams&) { return 8; }
+  // one-column items of a unit laid out over its rows' intervals only (see k_vox_tile)
+  static constexpr bool kCompact = false;
   uint32_t* table;
   uint64_t brick0;  // first brick of the unit's table segment
   __device__ static TouchAction make(const ActArgs& A, const VoxParams& P) {
@@ -533,6 +535,7 @@ struct MarkDenseT {
     if (kLaneOrder) return 1;  // the single-unit payload pass: its unit is wide
     return P.wide ? 1 : 16;
   }
+  static constexpr bool kCompact = kWide;
   gword_t* table;
   gbyte_t* bytes;
   uint64_t base;  // code of the first voxel of the unit's first brick
@@ -583,6 +586,7 @@ using MarkDenseWideBatchAction = MarkDenseT<true, false>;
 struct MarkPoolAction {
   static constexpr bool kTouch = false;
   __device__ static int seg_cols(const VoxParams&) { return 8; }
+  static constexpr bool kCompact = false;
   const gword_t* table;
   gbyte_t* bytes;  // the slot pool
   ull* ctr;
@@ -622,6 +626,7 @@ struct MarkPoolAction {
 struct WinnerAction {
   static constexpr bool kTouch = false;
   __device__ static int seg_cols(const VoxParams&) { return 1; }
+  static constexpr bool kCompact = false;
   const uint32_t* table;
   const ull* pool;  // bit form of the bricks, written by the compaction
   const uint32_t* wprefix;
@@ -680,6 +685,13 @@ __device__ __forceinline__ uint32_t upper_index(const T* __restrict__ a, uint32_
 constexpr int kTile = 256;      // threads per block
 constexpr int kTileTris = 192;  // triangles per tile: 35 KB of walk constants in LDS, so
                                 // that four blocks fit a CU
+// Actions with kCompact (atomic payload marks, one column per item) lay out the items of a
+// triangle of at most kCompactRows rows and 255 columns per row over its rows' intervals
+// only (row_interval: the columns outside fail the projection test), 255 items at most;
+// the rows' first columns and item offsets take 32 bytes of LDS per triangle, so their
+// tiles hold kTileTrisCompact triangles (four blocks still fit a CU).
+constexpr int kCompactRows = 16;
+constexpr int kTileTrisCompact = 160;
 constexpr uint32_t kQueues = 8;        // tile queues per persistent launch
 constexpr uint32_t kQueueStride = 16;  // words between their counters (own cache lines)
 constexpr uint32_t kQueueWords = kQueues * kQueueStride;
@@ -700,10 +712,17 @@ __global__ __launch_bounds__(kTile, 4) void k_vox_tile(const VoxParams* __restri
                                                     uint32_t* large_cols, uint32_t* large_walk,
                                                     uint32_t walk_cap, uint32_t tt, int collect,
                                                     int skip) {
-  __shared__ uint32_t sk[kWalkWords][kTileTris];
-  __shared__ uint32_t soff[kTileTris + 1];
+  constexpr bool kCompact = Act::kCompact;
+  constexpr int kTT = kCompact ? kTileTrisCompact : kTileTris;
+  __shared__ uint32_t sk[kWalkWords][kTT];
+  __shared__ uint32_t soff[kTT + 1];
   __shared__ uint32_t wsum[kTile / kWave];
   __shared__ uint32_t s_tile;
+  // compact layout, per triangle: items before each row, each row's first column - g0u,
+  // and whether the triangle uses the layout
+  __shared__ uint8_t s_rpre[kCompact ? kTT : 1][kCompactRows];
+  __shared__ uint8_t s_ri0[kCompact ? kTT : 1][kCompactRows];
+  __shared__ uint8_t s_cmp[kCompact ? kTT : 1];
   fill_spread_table();  // read after the first barrier of the tile loop
   ull* ctr = A.ctr;
   const uint32_t total_tiles = tile_off[n_units];
@@ -768,7 +787,28 @@ __global__ __launch_bounds__(kTile, 4) void k_vox_tile(const VoxParams* __restri
         // items: the rows' segments of seg columns
         const uint32_t seg = (uint32_t)Act::seg_cols(P);
         const uint32_t eu = (uint32_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
-        items = (uint32_t)(K.i[I_G1V] - K.i[I_G0V] + 1) * ((eu + seg - 1) / seg);
+        const uint32_t nrows = (uint32_t)(K.i[I_G1V] - K.i[I_G0V] + 1);
+        items = nrows * ((eu + seg - 1) / seg);
+        if constexpr (kCompact) {
+          // one-column items over the rows' intervals only
+          bool cmp = false;
+          if (seg == 1 && nrows <= (uint32_t)kCompactRows && eu <= 255u) {
+            uint32_t cum = 0;
+            for (uint32_t r = 0; r < nrows; r++) {
+              const ColRow R = col_row(K, P, K.i[I_G0V] + (int)r);
+              int i0, i1;
+              row_interval(K, R, P.unit, i0, i1);
+              s_rpre[tid][r] = (uint8_t)min(cum, 255u);
+              s_ri0[tid][r] = (uint8_t)min(i0 - K.i[I_G0U], 255);
+              if (i1 >= i0) cum += (uint32_t)(i1 - i0 + 1);
+            }
+            if (cum <= 255u) {
+              cmp = true;
+              items = cum;
+            }
+          }
+          s_cmp[tid] = cmp ? 1 : 0;
+        }
 #pragma unroll
         for (int f = 0; f < kWalkF; f++) sk[f][tid] = __float_as_uint(K.f[f]);
 #pragma unroll
@@ -792,7 +832,21 @@ __global__ __launch_bounds__(kTile, 4) void k_vox_tile(const VoxParams* __restri
       const uint32_t local = it - soff[lo];
       const uint32_t eu = (uint32_t)(K.i[I_G1U] - K.i[I_G0U] + 1);
       const int seg = Act::seg_cols(P);  // the tile's unit: uniform
-      if (seg > 1) {
+      if (kCompact && s_cmp[kCompact ? lo : 0]) {
+        // row: the last r with s_rpre[r] <= local among the triangle's rows (an empty
+        // row shares its offset with the next one, which is the one taken)
+        const uint32_t nrows = (uint32_t)(K.i[I_G1V] - K.i[I_G0V] + 1);
+        uint32_t r0 = 0, r1 = nrows - 1;
+        while (r0 < r1) {
+          const uint32_t mid = (r0 + r1 + 1) >> 1;
+          if (s_rpre[lo][mid] <= local)
+            r0 = mid;
+          else
+            r1 = mid - 1;
+        }
+        const int a = K.i[I_G0U] + (int)s_ri0[lo][r0] + (int)(local - s_rpre[lo][r0]);
+        walk_row_span<false>(K, P, K.i[I_G0V] + (int)r0, a, a, act);
+      } else if (seg > 1) {
         const uint32_t nseg = (eu + seg - 1) / seg;
         const int a = K.i[I_G0U] + (int)(local % nseg) * seg;
         const int b = K.i[I_G0V] + (int)(local / nseg);
@@ -1655,12 +1709,18 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
   uint64_t ntab = 0, ntiles = 0, ntri = 0;
   // triangles per tile: kTileTris once the pass has about four tiles of them per block of
   // the persistent grid, else fewer (a multiple of 32, at least 32)
+  // (passes with payload units: kTileTrisCompact, the tile of the compact item layout)
   uint32_t tt = (uint32_t)kTileTris;
   {
     uint64_t all = 0;
-    for (uint32_t u = 0; u < nu; u++) all += units[u].n;
+    bool any_payload_unit = false;
+    for (uint32_t u = 0; u < nu; u++) {
+      all += units[u].n;
+      any_payload_unit |= units[u].stride == 21;
+    }
+    const uint64_t cap = any_payload_unit ? kTileTrisCompact : kTileTris;
     const uint64_t want = all / (4 * (uint64_t)std::max(1, c.tune.tile_blocks));
-    tt = (uint32_t)std::min<uint64_t>(kTileTris, std::max<uint64_t>(32, (want + 31) / 32 * 32));
+    tt = (uint32_t)std::min<uint64_t>(cap, std::max<uint64_t>(32, (want + 31) / 32 * 32));
   }
   for (uint32_t u = 0; u < nu; u++) {
     // a unit smaller than a brick shares its brick with its neighbours: the brick stream
