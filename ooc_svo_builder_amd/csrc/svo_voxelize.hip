@@ -1156,19 +1156,22 @@ __global__ __launch_bounds__(256) void k_brick_emit(const uint32_t* __restrict__
                                                     uint64_t* __restrict__ bvoff,
                                                     uint32_t* __restrict__ bshape,
                                                     uint32_t* __restrict__ winner,
-                                                    int stage_cells) {
+                                                    int stage_cells, int write_stream) {
   extern __shared__ uint32_t s_cells[];  // stage_cells: kBrickVox per wave
   const uint64_t n_occ = ctr[kCtrNumOcc];
   for (uint64_t wv = wave_global(); wv < n_occ; wv += waves_total()) {
     const uint64_t ti = (uint32_t)__builtin_amdgcn_readfirstlane((int)occ[wv]);  // wave's brick
     const uint32_t u = upper_index<uint64_t>(ubase, n_units + 1, ti);
+    // a dense pass has its brick stream already (k_brick_count): only the payload units'
+    // bricks have work left
+    if (!write_stream && units[u].stride == 9) continue;
     const uint64_t slot = wv * kBrickWords + lane_id();
     ull word = pool[slot];
     const uint64_t bo = boff[wv];
     const uint64_t tag = (uint64_t)units[u].job << kJobShift;
     const uint64_t b0 = units[u].bstart + ((ti - ubase[u]) << kBrickBits);
     // the brick stream (non-empty bricks, appended in stream order)
-    if (__ballot(word != 0ull)) {
+    if (write_stream && __ballot(word != 0ull)) {
       const uint64_t sb = bo >> kNonEmptyShift;
       bwords[sb * kBrickWords + lane_id()] = word;
       const BrickShape B = brick_shape_of(word);
@@ -2097,7 +2100,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     // the pass's remaining kernels: one graph segment
     SVO_TRY(run_segment(c, [&]() -> hipError_t {
       // a dense pass has its brick stream entries already (k_brick_count); the emit pass
-      // writes codes and winners (rewriting those entries with the same values)
+      // writes the codes and winners of its payload units
       if (!dense || codes_now) {
         ktimer_begin(c, "brick_emit");
         // dense payload units: their cells are staged through LDS, kBrickVox words per wave
@@ -2111,7 +2114,7 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
                    c.bwords.as<uint64_t>() + nb0 * kBrickWords, c.bcode.as<uint64_t>() + nb0,
                    c.bvoff.as<uint64_t>() + nb0, c.bshape.as<uint32_t>() + nb0,
                    any_payload ? c.winner.as<uint32_t>() : (uint32_t*)nullptr,
-                   stage_cells ? 1 : 0);
+                   stage_cells ? 1 : 0, dense ? 0 : 1);
         ktimer_end(c);
         SVO_TRY(hipGetLastError());
       }
