@@ -294,7 +294,8 @@ void svo_ctx_destroy(svo_ctx* ctx) {
                     &c.lvl_code, &c.lvl_size, &c.lvl_mask, &c.lvl_first, &c.lvl_start,
                     &c.lvl_aux, &c.lvl_leafp, &c.lvl_pidx, &c.lvl_isize, &c.lvl_cache,
                     &c.lvl_pstart, &c.records2, &c.bwords, &c.bcode, &c.bvoff,
-                    &c.scan_status, &c.lvl_sib, &c.up_tiles, &c.bshape, &c.large_walk};
+                    &c.scan_status, &c.lvl_sib, &c.up_tiles, &c.bshape, &c.large_walk,
+                    &c.payload_map};
   for (DevBuf* b : bufs) release(*b);
   stream_release(c);
   if (c.aux) svo_ctx_destroy((svo_ctx*)c.aux);

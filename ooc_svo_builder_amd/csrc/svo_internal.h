@@ -252,6 +252,7 @@ struct Context {
 
   // voxelizer scratch (kept zeroed between passes: table, grid8, pool8)
   DevBuf unitdev;  // VoxParams[n] | ubase[n+1] | tile_off[n+1] | uvoff[n+1]
+  DevBuf payload_map;  // a pass's payload units: voxel offsets, first voxels, unit indices
   std::vector<uint8_t> unit_img;  // what unitdev holds before uvoff (empty: unknown)
   DevBuf table, pool, pool8;  // pool: bit words per slot; pool8: bytes per slot
   DevBuf grid8;               // dense byte grids of a pass's units (when they fit)

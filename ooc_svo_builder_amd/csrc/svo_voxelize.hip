@@ -1291,23 +1291,26 @@ __global__ void k_unit_voxels(const uint32_t* __restrict__ occ, const ull* __res
 // Voxel i of the pass (stream index s0 + i) belongs to the unit whose voxel range holds
 // it, and to that unit's job; its record goes to dbase[job] + 1 + (s0 + i - voff[job]).
 // Voxels of geometry-only jobs have no record here.
+// Threads cover the payload units' voxels only: payload voxel k lies in the payload unit
+// j with prange[j] <= k < prange[j + 1] (a block searches for its first voxel, a thread
+// steps forward over the rare unit boundaries), unit punit[j], whose voxels start at pass
+// voxel pstart[j].
 __global__ __launch_bounds__(256) void k_payload_records(
     const JobDev* __restrict__ jobs, const VoxParams* __restrict__ units,
-    const uint64_t* __restrict__ uvoff, uint32_t n_units, const uint64_t* __restrict__ codes,
-    uint64_t s0, const uint32_t* __restrict__ winner, uint64_t nvox, float* __restrict__ out,
+    const uint64_t* __restrict__ prange, const uint64_t* __restrict__ pstart,
+    const uint32_t* __restrict__ punit, uint32_t n_pu, const uint64_t* __restrict__ codes,
+    uint64_t s0, const uint32_t* __restrict__ winner, uint64_t n_pv, float* __restrict__ out,
     ull* ctr) {
-  __shared__ uint32_t s_u0;
+  __shared__ uint32_t s_j0;
   const uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x;
-  const uint64_t i = b0 + threadIdx.x;
-  // the unit holding voxel i of the pass (uvoff: first voxel of each unit); only payload
-  // units have their codes written. The block's voxels are consecutive: one binary search
-  // for its first voxel, then a thread steps forward over the (rare) unit boundaries
-  if (threadIdx.x == 0) s_u0 = upper_index<uint64_t>(uvoff, n_units + 1, b0);
+  const uint64_t k = b0 + threadIdx.x;
+  if (threadIdx.x == 0) s_j0 = upper_index<uint64_t>(prange, n_pu + 1, b0);
   __syncthreads();
-  if (i >= nvox) return;
-  uint32_t u = s_u0;
-  while (u + 1 < n_units && uvoff[u + 1] <= i) u++;
-  if (units[u].stride == 9) return;
+  if (k >= n_pv) return;
+  uint32_t j = s_j0;
+  while (j + 1 < n_pu && prange[j + 1] <= k) j++;
+  const uint32_t u = punit[j];
+  const uint64_t i = pstart[j] + (k - prange[j]);  // voxel i of the pass
   const uint64_t tagged = codes[s0 + i];
   const JobDev& J = jobs[units[u].job];
   if (!J.payload) return;
@@ -2092,10 +2095,39 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
       SVO_TRY(ensure_keep(c.bvoff, (nb0 + n_occ) * 8, nb0 * 8, c.stream));
       SVO_TRY(ensure_keep(c.bshape, (nb0 + n_occ) * 4, nb0 * 4, c.stream));
     }
+    // the payload units' voxel ranges, for the records kernel: prange (n_pu + 1) |
+    // pstart (n_pu) | punit (n_pu, as u64)
+    uint64_t n_pv = 0;
+    uint32_t n_pu = 0;
+    const uint64_t* d_prange = nullptr;
+    const uint64_t* d_pstart = nullptr;
+    const uint32_t* d_punit = nullptr;
     if (any_payload) {
       // records: each job's first record is its null record
       const uint64_t keep = batch ? c.records.cap : (1 + s0) * 32;
       SVO_TRY(ensure_keep(c.records, n_records * 32, keep, c.stream));
+      std::vector<uint64_t> pr(1, 0), ps, pu;
+      for (uint32_t u = 0; u < nu; u++) {
+        const uint64_t cnt = uvoff[u + 1] - uvoff[u];
+        if (units[u].stride != 21 || cnt == 0) continue;
+        pr.push_back(pr.back() + cnt);
+        ps.push_back(uvoff[u]);
+        pu.push_back(u);
+      }
+      n_pu = (uint32_t)ps.size();
+      n_pv = pr.back();
+      if (n_pv) {
+        std::vector<uint64_t> img(pr);
+        img.insert(img.end(), ps.begin(), ps.end());
+        std::vector<uint32_t> pu32(pu.begin(), pu.end());
+        const size_t b64 = img.size() * 8;
+        SVO_TRY(ensure(c.payload_map, b64 + pu32.size() * 4));
+        SVO_TRY(upload_async(c, c.payload_map.p, img.data(), b64));
+        SVO_TRY(upload_async(c, (char*)c.payload_map.p + b64, pu32.data(), pu32.size() * 4));
+        d_prange = c.payload_map.as<uint64_t>();
+        d_pstart = d_prange + n_pu + 1;
+        d_punit = (const uint32_t*)((const char*)c.payload_map.p + b64);
+      }
     }
     // the pass's remaining kernels: one graph segment
     SVO_TRY(run_segment(c, [&]() -> hipError_t {
@@ -2125,13 +2157,15 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
                             kSkipBinary));
           if (n_large) SVO_TRY(large_pass(WinnerAction{}, "vox_large_winner", kSkipBinary));
         }
-        ktimer_begin(c, "payload_records");
-        SVO_LAUNCH(k_payload_records, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
-                   c.stream, c.jobdev.as<JobDev>(), (const VoxParams*)d_units,
-                   (const uint64_t*)d_uvoff, nu, c.codes.as<uint64_t>(), s0,
-                   c.winner.as<uint32_t>(), nvox, c.records.as<float>(), ctr);
-        ktimer_end(c);
-        SVO_TRY(hipGetLastError());
+        if (n_pv) {
+          ktimer_begin(c, "payload_records");
+          SVO_LAUNCH(k_payload_records, dim3((unsigned)((n_pv + 255) / 256)), dim3(256), 0,
+                     c.stream, c.jobdev.as<JobDev>(), (const VoxParams*)d_units, d_prange,
+                     d_pstart, d_punit, n_pu, c.codes.as<uint64_t>(), s0,
+                     c.winner.as<uint32_t>(), n_pv, c.records.as<float>(), ctr);
+          ktimer_end(c);
+          SVO_TRY(hipGetLastError());
+        }
       }
       if (!dense) {  // (dense passes cleared their table entries in k_brick_count)
         ktimer_begin(c, "brick_clear");
