@@ -376,6 +376,32 @@ def test_voxelizer_paths_identical(small_cols, dense_mb, direct):
         _run_both(ctx, generators.terrain(n=49, seed=8, payload=True), True, 128, 2048)
 
 
+@pytest.mark.parametrize("subdiv", [2, 3, 4])
+def test_payload_item_layouts(subdiv):
+    """Payload icospheres walked through one-column items (direct walks off, no large
+    tier) at 256: triangles of ~14 rows and ~100 columns take the layout over their rows'
+    intervals only, those of ~28 rows and ~400 columns (subdivision 2) the full one; the
+    same bytes as the oracle either way, batched as well as alone."""
+    _say("payload item layouts subdivision %d" % subdiv)
+    tris = generators.icosphere(subdivisions=subdiv, seed=subdiv, payload=True)
+    with SvoContext(0) as ctx, SvoContext(0) as b:
+        for c in (ctx, b):
+            c.set_tuning(small_cols=1 << 16)
+            k = c.get_tuning()
+            c.set_knobs(k[0], 0, k[2], k[3])
+        _run_both(ctx, tris, True, 256, 2048)
+        # in a batch pass beside a geometry-only mesh (whose items are row segments)
+        jobs = [dict(records=tris, payload=True, gridsize=256),
+                dict(records=generators.icosphere(subdivisions=subdiv, seed=9), payload=False,
+                     gridsize=256)]
+        res, _ = b.batch_run(jobs)
+        for j, r in zip(jobs, res):
+            nodes_w, data_w, _, _, nf_w = oracle.pipeline(j["records"], j["payload"], 256, 2048)
+            assert r.nfilled == nf_w
+            assert b.nodes_range(r.node_offset, r.n_nodes).tobytes() == nodes_w.tobytes()
+            assert b.data_range(r.data_offset, r.n_data).tobytes() == data_w.tobytes()
+
+
 def test_payload_dense_coverage(gpu_ctx):
     """A fine payload terrain on coarse grids (about 30 and 120 triangles per column): the
     lowest covering triangle wins every voxel although neighbouring lanes of a wave mark
