@@ -39,11 +39,18 @@ __device__ __forceinline__ uint64_t offsets_of(uint32_t m) {
   return o;
 }
 
-__device__ __forceinline__ void put_node(uint64_t* nodes, uint64_t pos, uint64_t data,
-                                         uint64_t cb, uint64_t offs) {
+// A node's data index and children_base as written: moved by the job's placement in a
+// larger node stream (zero but for a streamed partition), data indices only past the
+// leading records (the null record, and the white record of geometry-only leaves).
+__device__ __forceinline__ uint64_t placed_data(const JobDev& J, uint64_t data) {
+  return data >= (J.payload ? 1ull : 2ull) ? data + J.data_add : data;
+}
+
+__device__ __forceinline__ void put_node(const JobDev& J, uint64_t* nodes, uint64_t pos,
+                                         uint64_t data, uint64_t cb, uint64_t offs) {
   uint64_t* r = nodes + 3 * pos;
-  r[0] = data;
-  r[1] = cb;
+  r[0] = placed_data(J, data);
+  r[1] = offs != ~0ull ? cb + J.cb_add : cb;
   r[2] = offs;
 }
 
@@ -305,10 +312,7 @@ __device__ __forceinline__ void job_bases(JobDev* jobs, uint32_t n_jobs,
       J.hazard = hz;
       if (trivial) {
         if (nb < node_cap) {
-          uint64_t* r = nodes + 3 * nb;
-          r[0] = J.nvox ? 1ull : 0ull;
-          r[1] = 0;
-          r[2] = ~0ull;
+          put_node(J, nodes, nb, J.nvox ? 1ull : 0ull, 0ull, ~0ull);
         } else {
           M.flag[1] = 1;
         }
@@ -438,7 +442,7 @@ __global__ __launch_bounds__(256) void k_emit_level(Arena A, LevelMeta M, Tags T
     data = idx;
     put_data(out_rec, J.dbase + idx, A.cache + (co + i) * 6);
   }
-  put_node(nodes, gpos, data, st + sz - 1 - (uint32_t)__popc(cm), offsets_of(cm));
+  put_node(J, nodes, gpos, data, st + sz - 1 - (uint32_t)__popc(cm), offsets_of(cm));
 }
 
 // Every node of heights hs..H in one launch (geometry records, no -levels data): one
@@ -480,7 +484,7 @@ __global__ __launch_bounds__(256) void k_emit_upper(Arena A, LevelMeta M, Tags T
     return;
   }
   A.start[g] = st;
-  put_node(nodes, gpos, 0ull, st + sz - 1 - (uint32_t)__popc(cm), offsets_of(cm));
+  put_node(J, nodes, gpos, 0ull, st + sz - 1 - (uint32_t)__popc(cm), offsets_of(cm));
 }
 
 // One thread per leaf: position = children_base(parent) + rank among its siblings.
@@ -523,7 +527,7 @@ __global__ __launch_bounds__(256) void k_emit_leaves(Arena A, LevelMeta M, Tags 
     dst[1] = src[1];
     data = idx;
   }
-  put_node(nodes, gpos, data, 0ull, ~0ull);
+  put_node(J, nodes, gpos, data, 0ull, ~0ull);
 }
 
 // ---------------------------------------------------------------- brick fast path
@@ -697,8 +701,8 @@ __global__ __launch_bounds__(kWave * kEmitWaves) void k_emit_bricks(
       }
     }
     const uint64_t d2 = leaf ? ~0ull : s_offs[m];
-    out[3 * (uint64_t)r] = d0;
-    out[3 * (uint64_t)r + 1] = d1;
+    out[3 * (uint64_t)r] = placed_data(J, d0);
+    out[3 * (uint64_t)r + 1] = leaf ? d1 : d1 + J.cb_add;
     out[3 * (uint64_t)r + 2] = d2;
   }
 }
@@ -970,9 +974,11 @@ __global__ __launch_bounds__(256) void k_up_finish(Arena A, LevelMeta M, Tags T,
     if (_e != hipSuccess) return _e; \
   } while (0)
 
-hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
-  if (levels && (tagged || n_jobs != 1)) {
-    set_error("-levels is supported for single builds only");
+hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels,
+                      const BuildOut* out) {
+  if ((levels || out) && (tagged || n_jobs != 1)) {
+    set_error(levels ? "-levels is supported for single builds only"
+                     : "internal: a placed build is a single build");
     return hipErrorNotSupported;
   }
   const uint64_t V = c.n_vox;
@@ -1011,6 +1017,10 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     J.dbase = n_records;
     J.n_data = J.payload ? 1 + J.nvox : 2;
     d.dbase = J.dbase;
+    if (out) {
+      d.cb_add = out->cb_add;
+      d.data_add = out->data_add;
+    }
     n_records += J.n_data;
     if (J.nvox) H = std::max(H, D);
   }
@@ -1085,7 +1095,14 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
     float* out_rec = levels ? c.records2.as<float>() : c.records.as<float>();
     if (!fast) SVO_TRY(ensure(c.lvl_leafp, std::max<uint64_t>(V, 1) * 4));
     const uint64_t node_cap = cap + V + in_brick + n_jobs;
-    SVO_TRY(ensure(c.nodes, node_cap * 24));
+    uint64_t* nodes;
+    if (out) {
+      SVO_TRY(ensure_keep(*out->nodes, (out->at + node_cap) * 24, out->at * 24, c.stream));
+      nodes = out->nodes->as<uint64_t>() + 3 * out->at;
+    } else {
+      SVO_TRY(ensure(c.nodes, node_cap * 24));
+      nodes = c.nodes.as<uint64_t>();
+    }
     Arena A{c.lvl_code.as<uint64_t>(), c.lvl_size.as<uint64_t>(), c.lvl_mask.as<uint8_t>(),
             c.lvl_first.as<uint32_t>(), c.lvl_start.as<uint64_t>(), c.lvl_pidx.as<uint32_t>(),
             cap, levels ? c.lvl_isize.as<uint64_t>() : nullptr,
@@ -1151,7 +1168,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
                          n_jobs, c.codes.as<uint64_t>(), c.lvl_leafp.as<uint32_t>(), root_size,
                          root_isize, c.records.as<float>() + 8, h_tail, H,
                          fast ? brick_hazard : nullptr, node_cap,
-                         c.nodes.as<uint64_t>(), c.records.as<float>(), out_rec);
+                         nodes, c.records.as<float>(), out_rec);
       SVO_TRY(hipGetLastError());
       // ---- top-down: heights H .. 1 (one launch without -levels), then the leaves
       const int hs = std::max(h0, 1);
@@ -1161,7 +1178,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
         ktimer_begin(c, "emit_children");
         SVO_LAUNCH(k_emit_upper, dim3((unsigned)std::max<uint64_t>((nup + 255) / 256, 1)),
                            dim3(256), 0, c.stream, A, M, T, (const JobDev*)d_jobs, hs, H, node_cap,
-                           c.nodes.as<uint64_t>());
+                           nodes);
         ktimer_end(c);
         SVO_TRY(hipGetLastError());
       }
@@ -1169,7 +1186,7 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
         ktimer_begin(c, "emit_children");
         SVO_LAUNCH(k_emit_level, dim3((unsigned)std::max<uint64_t>((bound[h] + 255) / 256, 1)),
                            dim3(256), 0, c.stream, A, M, T, (const JobDev*)d_jobs, h, node_cap,
-                           c.nodes.as<uint64_t>(), out_rec);
+                           nodes, out_rec);
         ktimer_end(c);
         SVO_TRY(hipGetLastError());
       }
@@ -1179,14 +1196,14 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels) {
                            dim3(kWave * kEmitWaves), 0,
                            c.stream, A, M, T, (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),
                            c.bcode.as<uint64_t>(), c.bvoff.as<uint64_t>(), NB, node_cap,
-                           c.nodes.as<uint64_t>());
+                           nodes);
         ktimer_end(c);
         SVO_TRY(hipGetLastError());
       } else if (V) {
         ktimer_begin(c, "emit_leaves");
         SVO_LAUNCH(k_emit_leaves, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, c.stream,
                            A, M, T, (const JobDev*)d_jobs, node_cap, c.codes.as<uint64_t>(),
-                           c.lvl_leafp.as<uint32_t>(), c.nodes.as<uint64_t>(),
+                           c.lvl_leafp.as<uint32_t>(), nodes,
                            c.records.as<float>(), out_rec);
         ktimer_end(c);
         SVO_TRY(hipGetLastError());

@@ -180,6 +180,10 @@ struct JobDev {
   uint64_t gridsize;
   uint64_t voff, nvox, dbase;
   uint64_t n_nodes, nbase, hazard, n_data;  // written by the build
+  // placement of the records in a larger node stream (a streamed partition; 0 otherwise):
+  // children_base of nodes with children moves by cb_add, data indices past the leading
+  // records (null, and white for geometry-only builds) by data_add
+  uint64_t cb_add, data_add;
 };
 
 // Host description of one voxelize unit (one partition of one job).
@@ -450,10 +454,19 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& units, bool ba
                           uint64_t* nfilled);
 // Write the codes of every voxel that only the brick stream holds (no-op when none).
 hipError_t ensure_codes(Context& c);
+// Where a single build's node records go instead of Context::nodes from 0: the array
+// 'nodes' from record 'at' on (its first 'at' records are kept when it grows), with the
+// record fields moved by cb_add and data_add (JobDev).
+struct BuildOut {
+  DevBuf* nodes;
+  uint64_t at;
+  uint64_t cb_add, data_add;
+};
 // Build the octrees of jobs [0, n_jobs) from the voxel stream; 'tagged' = the stream
 // carries job tags (a batch). levels (-levels, OctreeBuilder.cpp:82-99) is supported for
-// a single job.
-hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels);
+// a single job, and so is 'out'.
+hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels,
+                      const BuildOut* out = nullptr);
 
 // Streamed build (svo_stream.hip): begin (emits the leading data records), one call per
 // partition after its voxels were added (builds, emits and empties the voxel stream),

@@ -15,7 +15,9 @@
 // the chain after it. Records [0, size(Q_i) - 1) are the partition's output once their
 // children_base moves by the node records emitted before, and their data index by the
 // data records emitted before (the 0 of internal nodes and the shared white record 1 of
-// geometry-only leaves stay).
+// geometry-only leaves stay); the build writes them so moved, straight after the records
+// emitted before (or to the start of its own array, for the sink), so no record is
+// copied again.
 //
 // Data records follow the same order: payload voxel records in voxel order; with -levels
 // every record in post-order, so partition i contributes its subtree's records (Q_i's
@@ -41,28 +43,6 @@ namespace svo {
   } while (0)
 
 constexpr uint64_t kNoChildren = ~0ull;  // offsets[8] all -1: a leaf
-
-// dst[i] = src[i] with children_base moved by cb_add (nodes with children) and the data
-// index by data_add (indices from hdr up). src == dst rebases in place.
-__global__ __launch_bounds__(256) void k_rebase_nodes(const uint64_t* src, uint64_t* dst,
-                                                      uint64_t n, uint64_t cb_add,
-                                                      uint64_t data_add, uint64_t hdr) {
-  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
-    uint64_t d = src[3 * i];
-    uint64_t cb = src[3 * i + 1];
-    const uint64_t off = src[3 * i + 2];
-    if (off != kNoChildren) cb += cb_add;
-    if (d >= hdr) d += data_add;
-    dst[3 * i] = d;
-    dst[3 * i + 1] = cb;
-    dst[3 * i + 2] = off;
-  }
-}
-
-static void rebase_host(uint64_t* r, uint64_t cb_add, uint64_t data_add, uint64_t hdr) {
-  if (r[2] != kNoChildren) r[1] += cb_add;
-  if (r[0] >= hdr) r[0] += data_add;
-}
 
 static int ilog2(uint64_t v) {
   int d = 0;
@@ -166,8 +146,9 @@ hipError_t stream_begin(Context& c, uint64_t gridsize, uint64_t n_parts, int lev
   c.n_nodes = c.n_data = 0;
   if (!sink) {
     // the larger of the two node (data) arrays, usually the previous run's output,
-    // collects this run's records and the other takes the partitions' builds: no growth
-    // copies when a mesh is built again
+    // collects this run's records (the partitions' builds write their node records into
+    // it; their data records go through the other): no growth copies when a mesh is built
+    // again
     if (c.nodes.cap > s.out_nodes.cap) std::swap(c.nodes, s.out_nodes);
     if (c.records.cap > s.out_data.cap) std::swap(c.records, s.out_data);
   }
@@ -280,7 +261,12 @@ hipError_t stream_partition(Context& c, uint64_t part) {
   J.gridsize = s.gridsize;
   J.voff = 0;
   J.nvox = c.n_vox;
-  SVO_TRY(build_jobs(c, 1, false, s.levels));
+  // the build writes the partition's records already placed: appended to the collected
+  // node records, or from 0 of the build's own array when they go to the sink
+  const uint64_t hdr = s.payload ? 1 : 2;
+  const BuildOut out{s.sink ? &c.nodes : &s.out_nodes, s.sink ? 0 : s.nodes_out, s.nodes_out,
+                     s.data_out - hdr};
+  SVO_TRY(build_jobs(c, 1, false, s.levels, &out));
   if (c.err_count) {
     set_error("internal: " + std::to_string(c.err_count) +
               " consistency failures in partition " + std::to_string(part) +
@@ -295,39 +281,25 @@ hipError_t stream_partition(Context& c, uint64_t part) {
     return hipErrorUnknown;
   }
   const uint64_t q = c.n_nodes - 1 - chain;  // Q's record; [0, q) is the partition's output
-  const uint64_t hdr = s.payload ? 1 : 2;
-  const uint64_t cb_add = s.nodes_out, data_add = s.data_out - hdr;
+  const uint64_t* placed = out.nodes->as<uint64_t>() + 3 * out.at;
   TopNode pr;
   pr.null = false;
   {
-    const ReadReq rq = {pr.rec, c.nodes.as<uint64_t>() + 3 * q, 24};
+    const ReadReq rq = {pr.rec, placed + 3 * q, 24};
     SVO_TRY(read_back(c, &rq, 1));
   }
   // -levels: Q's average (an internal node's record; a payload leaf's own voxel record;
-  // geometry-only leaves carry zero caches)
+  // geometry-only leaves carry zero caches), read at its index within the partition
   if (s.levels && (pr.rec[2] != kNoChildren || s.payload)) {
+    const uint64_t di = pr.rec[0] >= hdr ? pr.rec[0] - out.data_add : pr.rec[0];
     float rec[8];
-    const ReadReq rq = {rec, (const char*)c.data_out().p + pr.rec[0] * 32, 32};
+    const ReadReq rq = {rec, (const char*)c.data_out().p + di * 32, 32};
     SVO_TRY(read_back(c, &rq, 1));
     memcpy(pr.cache, rec + 2, sizeof(pr.cache));
   }
-  rebase_host(pr.rec, cb_add, data_add, hdr);
-  if (q) {
-    const unsigned grid = (unsigned)std::min<uint64_t>((q + 255) / 256, 16384);
-    if (s.sink) {
-      SVO_LAUNCH(k_rebase_nodes, dim3(grid), dim3(256), 0, c.stream,
-                         (const uint64_t*)c.nodes.as<uint64_t>(), c.nodes.as<uint64_t>(), q,
-                         cb_add, data_add, hdr);
-      SVO_TRY(hipGetLastError());
-      SVO_TRY(drain_to_sink(c, 0, c.nodes.p, q * 24, 24));
-    } else {
-      SVO_TRY(ensure_keep(s.out_nodes, (s.nodes_out + q) * 24, s.nodes_out * 24, c.stream));
-      SVO_LAUNCH(k_rebase_nodes, dim3(grid), dim3(256), 0, c.stream,
-                         (const uint64_t*)c.nodes.as<uint64_t>(),
-                         s.out_nodes.as<uint64_t>() + 3 * s.nodes_out, q, cb_add, data_add, hdr);
-      SVO_TRY(hipGetLastError());
-    }
-  }
+  // records [q, n_nodes) (Q and its ancestor chain) lie past the partition's output: the
+  // next records emitted overwrite them
+  if (q && s.sink) SVO_TRY(drain_to_sink(c, 0, placed, q * 24, 24));
   uint64_t d0 = 0, dn = 0;
   if (s.levels) {
     d0 = hdr;
