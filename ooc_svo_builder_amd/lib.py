@@ -167,8 +167,9 @@ def partition_noalloc(stream_ptr: int, records, payload: bool, unitlength: float
     """svo_partition_noalloc on device tensors: records (N, 9|21) float32, lists int32/uint32
     (its length is the capacity), counts int64 of n_partitions + 1 entries. Asynchronous
     on the given stream."""
+    ptr, n, _, _keep = _mesh(records, payload)
     _check(load_library().svo_partition_noalloc(
-        C.c_void_p(stream_ptr), C.c_void_p(records.data_ptr()), int(records.shape[0]),
+        C.c_void_p(stream_ptr), ptr, n,
         int(payload), float(np.float32(unitlength)), gridsize, n_partitions,
         C.c_void_p(lists.data_ptr()), int(lists.numel()), C.c_void_p(counts.data_ptr())))
 
@@ -177,22 +178,35 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _mesh(records, payload: bool):
+    """(pointer, triangle count, on device, object to keep alive) of a mesh given as an
+    (N, 9) binary or (N, 21) payload float32 array: a numpy array or a contiguous device
+    tensor. The C ABI reads N records of that width, so any other shape is refused here."""
+    width = 21 if payload else 9
+    if hasattr(records, "data_ptr"):
+        if records.dim() != 2 or records.shape[1] != width or \
+                str(records.dtype) != "torch.float32" or not records.is_contiguous():
+            raise ValueError("mesh records must be a contiguous (N, %d) float32 tensor for "
+                             "payload=%s, got %s %s" % (width, bool(payload),
+                                                        tuple(records.shape), records.dtype))
+        return (C.c_void_p(records.data_ptr()), int(records.shape[0]),
+                1 if records.is_cuda else 0, records)
+    r = np.ascontiguousarray(records, np.float32)
+    if r.ndim != 2 or r.shape[1] != width:
+        raise ValueError("mesh records must be an (N, %d) float32 array for payload=%s, got "
+                         "shape %s" % (width, bool(payload), r.shape))
+    return _ptr(r), int(r.shape[0]), 0, r
+
+
 def fill_job(q: Job, j: dict):
     """Fill an svo_job from a dict with keys records ((N, 9|21) float32 numpy array, or a
     device tensor), payload, gridsize and optionally memory_limit_mb (2048), bbox
     ((0,0,0,1,1,1)), color_mode (0) and levels (False). Returns the object that must stay
     alive while the job's records are read."""
-    r = j["records"]
-    if hasattr(r, "data_ptr"):
-        q.records = r.data_ptr()
-        q.n_triangles = int(r.shape[0])
-        q.src_on_device = 1 if r.is_cuda else 0
-        keep = r
-    else:
-        keep = np.ascontiguousarray(r, np.float32)
-        q.records = keep.ctypes.data
-        q.n_triangles = keep.shape[0]
-        q.src_on_device = 0
+    ptr, n, on_dev, keep = _mesh(j["records"], j["payload"])
+    q.records = ptr.value
+    q.n_triangles = n
+    q.src_on_device = on_dev
     q.payload = int(bool(j["payload"]))
     for k, v in enumerate(j.get("bbox", (0, 0, 0, 1, 1, 1))):
         q.bbox[k] = float(v)
@@ -318,14 +332,8 @@ class SvoContext:
     # -- stages -----------------------------------------------------------------
     def load_mesh(self, records, payload: bool):
         """records: (N, 9|21) float32 numpy array, or a torch tensor on the device."""
-        L = load_library()
-        if hasattr(records, "data_ptr"):
-            n = int(records.shape[0])
-            on_dev = 1 if records.is_cuda else 0
-            _check(L.svo_mesh_load(self._h, C.c_void_p(records.data_ptr()), n, int(payload), on_dev))
-        else:
-            r = np.ascontiguousarray(records, np.float32)
-            _check(L.svo_mesh_load(self._h, _ptr(r), r.shape[0], int(payload), 0))
+        ptr, n, on_dev, _keep = _mesh(records, payload)
+        _check(load_library().svo_mesh_load(self._h, ptr, n, int(payload), on_dev))
         self.payload = bool(payload)
 
     def load_mesh_file(self, tridata_path: str, n_triangles: int, payload: bool) -> int:
@@ -375,18 +383,11 @@ class SvoContext:
     def run(self, records, payload: bool, gridsize: int, memory_limit_mb: int = 2048,
             bbox=(0, 0, 0, 1, 1, 1), color_mode: int = 0, levels: bool = False) -> RunInfo:
         """main.cpp's flow in memory; the mesh may be a numpy array or a device tensor."""
-        L = load_library()
         b = np.asarray(bbox, np.float32)
         info = RunInfo()
-        if hasattr(records, "data_ptr"):
-            n = int(records.shape[0])
-            _check(L.svo_run(self._h, C.c_void_p(records.data_ptr()), n, int(payload),
-                             1 if records.is_cuda else 0, _ptr(b), gridsize, memory_limit_mb,
-                             color_mode, int(levels), C.byref(info)))
-        else:
-            r = np.ascontiguousarray(records, np.float32)
-            _check(L.svo_run(self._h, _ptr(r), r.shape[0], int(payload), 0, _ptr(b), gridsize,
-                             memory_limit_mb, color_mode, int(levels), C.byref(info)))
+        ptr, n, on_dev, _keep = _mesh(records, payload)
+        _check(load_library().svo_run(self._h, ptr, n, int(payload), on_dev, _ptr(b), gridsize,
+                                      memory_limit_mb, color_mode, int(levels), C.byref(info)))
         self.payload = bool(payload)
         self.last_info = info
         return info
@@ -409,14 +410,12 @@ class SvoContext:
                 errors.append(e)
                 return 1
         cb = SINK_FN(_cb)
+        _keep = None
         if records is None:
             args = (None, 0, int(payload), 0)
-        elif hasattr(records, "data_ptr"):
-            args = (C.c_void_p(records.data_ptr()), int(records.shape[0]), int(payload),
-                    1 if records.is_cuda else 0)
         else:
-            r = np.ascontiguousarray(records, np.float32)
-            args = (_ptr(r), r.shape[0], int(payload), 0)
+            ptr, n, on_dev, _keep = _mesh(records, payload)
+            args = (ptr, n, int(payload), on_dev)
         rc = L.svo_run_stream(self._h, *args, _ptr(b), gridsize, memory_limit_mb, color_mode,
                               int(levels), C.cast(cb, C.c_void_p), None, C.byref(info))
         if errors:

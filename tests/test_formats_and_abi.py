@@ -41,6 +41,28 @@ def test_tri_roundtrip(tmp_path):
         assert back.tobytes() == rec.tobytes()
 
 
+def test_mesh_shape_is_checked_before_the_c_abi():
+    """The C ABI reads N records of 9 (binary) or 21 (payload) floats; the host wrapper
+    refuses meshes of any other shape instead of passing a mis-strided pointer."""
+    from ooc_svo_builder_amd import lib
+    torch = pytest.importorskip("torch")
+    assert lib._mesh(np.zeros((5, 9), np.float32), False)[1:3] == (5, 0)
+    assert lib._mesh(np.zeros((0, 21), np.float32), True)[1] == 0
+    assert lib._mesh(torch.zeros(4, 21), True)[1:3] == (4, 0)
+    for bad, payload in [(np.zeros((5, 21), np.float32), False),
+                         (np.zeros((5, 9), np.float32), True),
+                         (np.zeros(45, np.float32), False),
+                         (torch.zeros(5, 21), False),
+                         (torch.zeros(5, 9, dtype=torch.float64), False),
+                         (torch.zeros(9, 5).t(), False)]:
+        with pytest.raises(ValueError):
+            lib._mesh(bad, payload)
+    job = lib.Job()
+    with pytest.raises(ValueError):
+        lib.fill_job(job, dict(records=np.zeros((3, 21), np.float32), payload=False,
+                               gridsize=64))
+
+
 def test_octree_header_text():
     assert octree_header_text(1024, 1474721, 484322) == (
         "#octreeheader 1\ngridlength 1024\nn_nodes 1474721\nn_data 484322\nEND\n")
