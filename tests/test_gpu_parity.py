@@ -285,6 +285,45 @@ def test_pipeline_partitions(gpu_ctx, G, L, parts):
     assert info.n_partitions == parts
 
 
+@pytest.mark.parametrize("payload", [False, True])
+def test_many_triangle_pass_mid_size_triangles(gpu_ctx, payload):
+    """A one-mesh pass of more than 16 tiles per block of the persistent grid (3.2M binary
+    or 2.7M payload triangles at -s 512) walks triangles of up to 16 x small_cols columns
+    in the small tier: tiny soup triangles plus 300 triangles of 45-90 voxel edges
+    (roughly 1K-8K columns each), against the oracle, with the default threshold and with
+    small_cols = 256 (whose mid-size triangles then split between both tiers)."""
+    G = 512
+    n = 2_700_000 if payload else 3_200_000
+    _say("many-triangle pass payload=%d" % payload)
+    rng = np.random.default_rng(41 + payload)
+    tiny = generators.sphere_soup(n, n_spheres=20, seed=43, median_edge=0.5 / G)
+    c = rng.uniform(0.25, 0.75, (300, 1, 3))
+    d = rng.normal(0, 1, (300, 2, 3))
+    d /= np.linalg.norm(d, axis=2, keepdims=True)
+    edge = rng.uniform(45, 90, (300, 2, 1)) / G
+    big = np.concatenate([c, c + d[:, :1] * edge[:, :1], c + d[:, 1:] * edge[:, 1:]], axis=1)
+    geo = np.concatenate([tiny, big.reshape(-1, 9).astype(np.float32)])
+    if payload:
+        tris = np.zeros((len(geo), 21), np.float32)
+        tris[:, :9] = geo
+        tris[:, 9:12] = rng.normal(0, 1, (len(geo), 3))
+        tris[:, 12:21] = rng.uniform(0, 1, (len(geo), 9))
+    else:
+        tris = geo
+    nodes_w, data_w, hz_w, np_w, nf_w = oracle.pipeline(tris, payload, G, 2048)
+    try:
+        for knobs in [None, (256, 16, 1024, 2048)]:
+            if knobs:
+                gpu_ctx.set_knobs(*knobs)
+            info = gpu_ctx.run(tris, payload, G, 2048)
+            assert (info.n_partitions, info.nfilled, info.hazard) == (np_w, nf_w, hz_w), knobs
+            assert (info.n_nodes, info.n_data) == (len(nodes_w), len(data_w)), knobs
+            assert gpu_ctx.nodes(info.n_nodes).tobytes() == nodes_w.tobytes(), knobs
+            assert gpu_ctx.data(info.n_data).tobytes() == data_w.tobytes(), knobs
+    finally:
+        gpu_ctx.set_knobs(1024, 16, 1024, 2048)
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_pipeline_payload_color_modes(gpu_ctx, mode):
     _say("pipeline payload color mode %d" % mode)
