@@ -564,6 +564,9 @@ __device__ __forceinline__ int last_le8(const uint32_t* a, uint32_t v) {
   return k;
 }
 
+// kPlaced: the job's records go to a larger node stream (a streamed partition), their
+// children_base and data indices moved by its placement.
+template <bool kPlaced>
 __global__ __launch_bounds__(kWave * kEmitWaves) void k_emit_bricks(
     Arena A, LevelMeta M, Tags T, const JobDev* __restrict__ jobs,
     const uint64_t* __restrict__ bwords, const uint64_t* __restrict__ bcode,
@@ -661,6 +664,9 @@ __global__ __launch_bounds__(kWave * kEmitWaves) void k_emit_bricks(
   }
   uint64_t* out = nodes + 3 * (J.nbase + S);
   const int payload = J.payload;
+  // the job's placement, read once (the record stores below could alias the job table)
+  const uint64_t cb_add = kPlaced ? J.cb_add : 0, data_add = kPlaced ? J.data_add : 0;
+  const uint64_t d_hdr = payload ? 1ull : 2ull;
   for (uint32_t r = lane; r < R; r += kWave) {
     // find the record's node: its child mask m (internal nodes) and the first two words
     uint64_t d0 = 0, d1;
@@ -701,8 +707,12 @@ __global__ __launch_bounds__(kWave * kEmitWaves) void k_emit_bricks(
       }
     }
     const uint64_t d2 = leaf ? ~0ull : s_offs[m];
-    out[3 * (uint64_t)r] = placed_data(J, d0);
-    out[3 * (uint64_t)r + 1] = leaf ? d1 : d1 + J.cb_add;
+    if constexpr (kPlaced) {
+      d0 = d0 >= d_hdr ? d0 + data_add : d0;
+      d1 = leaf ? d1 : d1 + cb_add;
+    }
+    out[3 * (uint64_t)r] = d0;
+    out[3 * (uint64_t)r + 1] = d1;
     out[3 * (uint64_t)r + 2] = d2;
   }
 }
@@ -1192,11 +1202,15 @@ hipError_t build_jobs(Context& c, uint32_t n_jobs, bool tagged, int levels,
       }
       if (fast) {
         ktimer_begin(c, "emit_bricks");
-        SVO_LAUNCH(k_emit_bricks, dim3((unsigned)((NB + kEmitWaves - 1) / kEmitWaves)),
-                           dim3(kWave * kEmitWaves), 0,
-                           c.stream, A, M, T, (const JobDev*)d_jobs, c.bwords.as<uint64_t>(),
-                           c.bcode.as<uint64_t>(), c.bvoff.as<uint64_t>(), NB, node_cap,
-                           nodes);
+        const dim3 eg((unsigned)((NB + kEmitWaves - 1) / kEmitWaves)), eb(kWave * kEmitWaves);
+        if (out)
+          SVO_LAUNCH(k_emit_bricks<true>, eg, eb, 0, c.stream, A, M, T, (const JobDev*)d_jobs,
+                     c.bwords.as<uint64_t>(), c.bcode.as<uint64_t>(), c.bvoff.as<uint64_t>(),
+                     NB, node_cap, nodes);
+        else
+          SVO_LAUNCH(k_emit_bricks<false>, eg, eb, 0, c.stream, A, M, T, (const JobDev*)d_jobs,
+                     c.bwords.as<uint64_t>(), c.bcode.as<uint64_t>(), c.bvoff.as<uint64_t>(),
+                     NB, node_cap, nodes);
         ktimer_end(c);
         SVO_TRY(hipGetLastError());
       } else if (V) {
