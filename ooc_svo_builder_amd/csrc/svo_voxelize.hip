@@ -1735,7 +1735,8 @@ hipError_t voxelize_units(Context& c, const std::vector<VoxUnit>& hu, bool batch
     // which is faster than the large tier's thread per bbox column (the soup of 5x10^7
     // triangles at 8192, 8 partitions: 68.6 -> 61.1 ms per step). Batches keep their
     // autotuned threshold.
-    if (!batch && all >= 16 * blocks * tt) small_cols = std::min(16 * small_cols, 1 << 16);
+    if (!batch && all >= 16 * blocks * tt)
+      small_cols = std::max(small_cols, (int)std::min<int64_t>(16ll * small_cols, 1 << 16));
   }
   for (uint32_t u = 0; u < nu; u++) {
     // a unit smaller than a brick shares its brick with its neighbours: the brick stream
